@@ -1,0 +1,188 @@
+/*
+ * matvans.h -- C ABI of the MI355X (gfx950) fused rANS-decode x int8 matvec library.
+ *
+ * Every entry point takes plain pointers and sizes.  Device pointers are HIP device
+ * memory owned by the caller; `stream` is a hipStream_t passed as void* (NULL = the
+ * null stream).  Launch entry points are stream-ordered, never synchronise the host,
+ * never allocate and never exit the process: they return a matvans_status and leave a
+ * message in matvans_last_error() (thread-local).
+ *
+ * Reference interfaces are cited as file:line of wildug/MatVANS.
+ */
+#ifndef MATVANS_H
+#define MATVANS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference reports errors by fprintf + exit(-1)
+ * (matvansWarpPerRow/matvansWarpPerRow.cu:416-424); this library returns them. */
+typedef enum {
+    MATVANS_OK = 0,
+    MATVANS_E_INVALID = 1,     /* null pointer or out-of-range argument */
+    MATVANS_E_SHAPE = 2,       /* shape the layout cannot represent (cols % 512 != 0, ...) */
+    MATVANS_E_ALIGN = 3,       /* pointer not 16-byte aligned */
+    MATVANS_E_FORMAT = 4,      /* malformed or truncated .bin data */
+    MATVANS_E_IO = 5,          /* file cannot be opened / written */
+    MATVANS_E_HIP = 6,         /* HIP runtime error (launch, ...) */
+    MATVANS_E_RANGE = 7,       /* symbol outside the entropy model */
+    MATVANS_E_UNSUPPORTED = 8, /* valid request this build does not implement */
+    MATVANS_E_CAPACITY = 9     /* caller buffer too small */
+} matvans_status;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *matvans_last_error(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int matvans_version(void);
+
+/* ------------------------------------------------------------------------------
+ * A1. Fused decode + int32 matvec, the drop-in for the kernel
+ *     __global__ void decmpressAndMultiply(int32_t* dst, int8_t* vec, uint32_t rows,
+ *         uint32_t cols, float grid_spacing, const uint32_t* cursors, int8_t min_value,
+ *         uint8_t G, const uint8_t* cdf_data, const uint8_t* ppf_data,
+ *         uint16_t* payload, uint32_t payload_size)
+ *     (matvansWarpPerRow/matvansWarpPerRow-kernel.cu:46-50).
+ * Same arguments with the same meaning, all pointers device pointers:
+ *   dst[rows] = sum_c W[r,c] * vec[c] (int32, bit-exact), W decoded from the 32
+ *   interleaved rANS streams of each row (payload in u16 words, row r at cursors[r]).
+ * grid_spacing is accepted and unused, as in the reference.  payload_size bounds every
+ * read (the reference ignores it and may read past a row's end).
+ * Preconditions: cols % 512 == 0; payload 16-byte aligned; vec 4-byte aligned.
+ * ----------------------------------------------------------------------------*/
+int matvans_decompress_and_multiply(int32_t *dst, const int8_t *vec, uint32_t rows,
+                                    uint32_t cols, float grid_spacing,
+                                    const uint32_t *cursors, int8_t min_value, uint8_t G,
+                                    const uint8_t *cdf_data, const uint8_t *ppf_data,
+                                    const uint16_t *payload, uint32_t payload_size,
+                                    void *stream);
+
+/* A1 with 1..8 activation vectors decoded once: vecs[v*vec_stride + c] (int8),
+ * dst[v*rows + r] (int32).  No reference counterpart (the reference multiplies one
+ * vector per launch, matvansWarpPerRow.cu:161-164). */
+int matvans_decompress_and_multiply_nvec(int32_t *dst, const int8_t *vecs, uint32_t n_vec,
+                                         uint32_t vec_stride, uint32_t rows, uint32_t cols,
+                                         const uint32_t *cursors, int8_t min_value,
+                                         uint8_t G, const uint8_t *cdf_data,
+                                         const uint8_t *ppf_data, const uint16_t *payload,
+                                         uint32_t payload_size, void *stream);
+
+/* ------------------------------------------------------------------------------
+ * A6. Host composite CompressedMatrix::decompressAndMult(int8_t* d_result8,
+ *     int32_t* d_result32, int8_t* vector, float v_delta) -> float
+ *     (matvansWarpPerRow/matvansWarpPerRow.cu:147,150-176).
+ * A matrix already resident on the device is described by this struct of device
+ * pointers (the reference keeps them in CompressedMatrix::d_*, :89-92).
+ * The composite launches the fused decode-matvec and the A7 epilogue; the new
+ * v_delta (absmax/127) is written to *d_delta on the device instead of being returned
+ * through a host synchronisation (the reference's thrust reduce syncs, :167).
+ * ----------------------------------------------------------------------------*/
+typedef struct {
+    uint32_t rows, cols;
+    float grid_spacing;
+    int8_t min_value;
+    uint8_t G;
+    uint16_t reserved;
+    uint32_t payload_size;     /* u16 words */
+    const uint32_t *cursors;   /* device, [rows] */
+    const uint8_t *cdf;        /* device, [G+1] */
+    const uint8_t *ppf;        /* device, [256] */
+    const uint16_t *payload;   /* device, [payload_size], 16-byte aligned */
+} matvans_dev_matrix;
+
+int matvans_decompress_and_mult(const matvans_dev_matrix *m, int8_t *d_result8,
+                                int32_t *d_result32, const int8_t *d_vector,
+                                float *d_delta, void *stream);
+
+/* ------------------------------------------------------------------------------
+ * A7. Epilogue: a = float(max|acc|) (absMaxWithThrustDevice, matvansWarpPerRow.cu:51-60),
+ *     delta = a / 127, out8[i] = (int8) rint(float(acc[i]) / delta)
+ *     (normalizeAndRoundtoInt8, :62-72; clamp != 0 adds the clamp to [-128,127] of
+ *     baselines/baseline_int8/baseline_matVec_custom.cu:75-86).  delta written to
+ *     *d_delta (device).  All in IEEE fp32, round-half-even, no host sync.
+ * ----------------------------------------------------------------------------*/
+int matvans_absmax_requant(const int32_t *d_acc, uint32_t n, int8_t *d_out8, float *d_delta,
+                           int clamp, void *stream);
+
+/* Scaled fp32 output y[v*rows + r] = s_x[v] * s_w[r] * float(acc[v*rows + r]).
+ * s_w is the per-row weight scale (device, [rows]), s_x per vector (device, [n_vec]).
+ * No reference counterpart (the .bin grid_spacing is always 1.0,
+ * create_matrices_WarpPerRow.py:430). */
+int matvans_scale_output(const int32_t *d_acc, uint32_t rows, uint32_t n_vec,
+                         const float *d_s_w, const float *d_s_x, float *d_y, void *stream);
+
+/* ------------------------------------------------------------------------------
+ * A11. Uncompressed int8 comparator, replaces vecMat_int8_warpRows
+ *      (baselines/baseline_int8/baseline_matVec_custom.cu:99-155):
+ *      dst[v*rows + r] = sum_c mat[r*cols + c] * vecs[v*vec_stride + c] (int32), mat
+ *      row-major int8, 16-byte aligned, cols % 16 == 0, 1 <= n_vec <= 8.
+ * A12. fp32 comparator, replaces cublasSgemv (baselines/cuBLAS_baseline_float/
+ *      baseline_matVec_cuBLAS.cu:111-144): y[r] = alpha * sum_c mat[r*cols+c] * x[c],
+ *      mat row-major fp32, cols % 4 == 0.
+ * ----------------------------------------------------------------------------*/
+int matvans_int8_matvec(int32_t *dst, const int8_t *mat, const int8_t *vecs, uint32_t n_vec,
+                        uint32_t vec_stride, uint32_t rows, uint32_t cols, void *stream);
+int matvans_f32_gemv(float *y, const float *mat, const float *x, uint32_t rows, uint32_t cols,
+                     float alpha, void *stream);
+
+/* ------------------------------------------------------------------------------
+ * A10. FxHash of an int8 array (matvansWarpPerRow.cu:31-42,
+ *      create_matrices_WarpPerRow.py:58-70).  Host memory.
+ * ----------------------------------------------------------------------------*/
+uint32_t matvans_fxhash(const int8_t *data, uint64_t n);
+
+/* ------------------------------------------------------------------------------
+ * A5 / A9. The .bin v0 container (writer create_matrices_WarpPerRow.py:286-318,440-461;
+ * reader matvansWarpPerRow.cu:100-145,225-258).  Host memory.
+ * ----------------------------------------------------------------------------*/
+typedef struct {
+    uint32_t rows, cols;
+    float grid_spacing;
+    int8_t min_value;
+    uint8_t G;
+    uint16_t reserved;
+    uint32_t payload_size;     /* u16 words */
+    const uint32_t *cursors;   /* [rows] */
+    const uint8_t *cdf;        /* [G+1], cdf & 0xFF */
+    const uint8_t *ppf;        /* [256] */
+    const uint16_t *payload;   /* [payload_size] */
+} matvans_host_matrix;
+
+typedef struct matvans_bin matvans_bin;   /* parsed file, owned by the library */
+
+/* Parse a whole .bin file.  Validates every length against the file size and every
+ * record's cursors (nondecreasing multiples of 256 inside the payload). */
+int matvans_bin_read(const char *path, matvans_bin **out);
+void matvans_bin_free(matvans_bin *bin);
+/* Header: num_matrices, result_hash, max_word_count (:440-442); start vector (:444-453). */
+int matvans_bin_header(const matvans_bin *bin, uint32_t *num_matrices, uint32_t *result_hash,
+                       uint32_t *max_word_count, const int8_t **vector, uint32_t *len_v);
+int matvans_bin_matrix(const matvans_bin *bin, uint32_t k, matvans_host_matrix *out);
+/* Write a .bin v0 file; max_word_count is computed as the writer does (:286-288,455). */
+int matvans_bin_write(const char *path, uint32_t result_hash, const int8_t *vector,
+                      uint32_t len_v, uint32_t num_matrices, const matvans_host_matrix *mats);
+
+/* Entropy model (create_matrices_WarpPerRow.py:131-187) and ppf (:189-208).
+ * cdf16 receives G+1 <= 257 uint16 entries (cdf16[G] == 256), ppf 256 entries.
+ * Counts with equal frequency are ordered by descending value (the reference inherits
+ * numpy's argsort order there, which is not specified). */
+int matvans_build_model(const int8_t *data, uint64_t n, int8_t *min_value, uint32_t *G,
+                        uint16_t *cdf16, uint8_t *ppf);
+
+/* Upper bound on the payload words matvans_encode_rows can produce. */
+uint64_t matvans_encode_bound(uint32_t rows, uint32_t cols);
+/* 32-stream row-interleaved encode (create_matrices_WarpPerRow.py:353-430), rows x cols
+ * (cols % 512 == 0), threaded over rows.  Padding slots are written as 0.  Writes
+ * cursors[rows] and payload; *payload_size = words used. */
+int matvans_encode_rows(const int8_t *mat, uint32_t rows, uint32_t cols, int8_t min_value,
+                        uint32_t G, const uint16_t *cdf16, uint32_t n_threads,
+                        uint32_t *cursors, uint16_t *payload, uint64_t payload_cap,
+                        uint64_t *payload_size);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MATVANS_H */

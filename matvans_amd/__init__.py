@@ -1,0 +1,321 @@
+"""MI355X-native fused rANS-decode x int8 matvec (drop-in for MatVANS WarpPerRow).
+
+Host-side mirror of the reference's operator interface over the C ABI of
+``lib/libmatvans.so`` (``include/matvans.h``):
+
+  CompressedMatrix            matvansWarpPerRow/matvansWarpPerRow.cu:75-176
+    .deserialize_file(path)   the .bin reader (:100-145, :225-258)
+    .decompressAndMult(...)   decode-matvec + absmax + requant (:150-176)
+  decmpressAndMultiply(...)   the kernel entry (matvansWarpPerRow-kernel.cu:46-50)
+  hash_int8_array(a)          FxHash (matvansWarpPerRow.cu:31-42)
+  run_chain(...)              the chained benchmark loop of main (:280-367)
+
+PyTorch provides device memory and streams only.  Every compute call goes through the
+HIP library; a missing library raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import capi
+from .capi import DevMatrix, HostMatrix, MatvansError, call, lib
+
+__all__ = ["CompressedMatrix", "MatvansError", "decmpressAndMultiply", "decompress_and_multiply",
+           "int8_matvec", "f32_gemv", "absmax_requant", "scale_output", "hash_int8_array",
+           "build_model", "encode_rows", "read_bin", "write_bin", "run_chain", "lib"]
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _np_ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+# ----------------------------------------------------------------------- host codec
+def hash_int8_array(arr) -> int:
+    """FxHash of an int8 array (matvansWarpPerRow.cu:31-42)."""
+    a = np.ascontiguousarray(np.asarray(arr), dtype=np.int8)
+    return int(lib().matvans_fxhash(_np_ptr(a), a.size))
+
+
+def build_model(data):
+    """8-bit entropy model (create_matrices_WarpPerRow.py:131-208).
+    Returns (min_value, G, cdf16 uint16[G+1], ppf uint8[256])."""
+    d = np.ascontiguousarray(np.asarray(data), dtype=np.int8).ravel()
+    mn = ctypes.c_int8()
+    G = ctypes.c_uint32()
+    cdf = np.zeros(257, np.uint16)
+    ppf = np.zeros(256, np.uint8)
+    call("matvans_build_model", _np_ptr(d), d.size, ctypes.byref(mn), ctypes.byref(G),
+         _np_ptr(cdf), _np_ptr(ppf))
+    return int(mn.value), int(G.value), cdf[:G.value + 1].copy(), ppf
+
+
+def encode_rows(matrix, min_value, cdf16, n_threads=0):
+    """32-stream row-interleaved encode (create_matrices_WarpPerRow.py:353-430).
+    Returns (cursors uint32[rows], payload uint16[payload_size])."""
+    m = np.ascontiguousarray(np.asarray(matrix), dtype=np.int8)
+    rows, cols = m.shape
+    c16 = np.ascontiguousarray(cdf16, np.uint16)
+    cap = int(lib().matvans_encode_bound(rows, cols))
+    cur = np.empty(rows, np.uint32)
+    pay = np.empty(cap, np.uint16)
+    size = ctypes.c_uint64()
+    call("matvans_encode_rows", _np_ptr(m), rows, cols, int(min_value), len(c16) - 1,
+         _np_ptr(c16), int(n_threads), _np_ptr(cur), _np_ptr(pay), cap, ctypes.byref(size))
+    return cur, pay[:size.value].copy()
+
+
+class CompressedMatrix:
+    """One rANS-compressed int8 matrix (matvansWarpPerRow.cu:75-148).
+
+    Host arrays: cursors (uint32[rows]), cdf_data (uint8[G+1], cdf & 0xFF), ppf_data
+    (uint8[256]), payload (uint16[payload_size]).  ``to_device`` uploads them as torch
+    tensors (``d_cursors``, ``d_cdf_data``, ``d_ppf_data``, ``d_payload``)."""
+
+    # words of zero slack after the payload; reads never reach it, but a caller-built
+    # payload may be sliced by row and this keeps 16-byte loads inside the allocation
+    SLACK_WORDS = 512
+
+    def __init__(self, rows, cols, grid_spacing, cursors, min_value, G, cdf_data, ppf_data,
+                 payload, payload_size=None):
+        self.rows, self.cols = int(rows), int(cols)
+        self.grid_spacing = float(grid_spacing)
+        self.cursors = np.ascontiguousarray(cursors, np.uint32)
+        self.min_value, self.G = int(min_value), int(G)
+        self.cdf_data = np.ascontiguousarray(cdf_data, np.uint8)
+        self.ppf_data = np.ascontiguousarray(ppf_data, np.uint8)
+        self.payload = np.ascontiguousarray(payload, np.uint16)
+        self.payload_size = int(len(self.payload) if payload_size is None else payload_size)
+        self.d_cursors = self.d_cdf_data = self.d_ppf_data = self.d_payload = None
+
+    # -- construction
+    @classmethod
+    def encode(cls, matrix, model=None, n_threads=0):
+        """Encode an int8 matrix (rows x cols, cols % 512 == 0) with the library encoder.
+        ``model`` = (min_value, cdf16) reuses a given model; otherwise it is built."""
+        m = np.ascontiguousarray(np.asarray(matrix), dtype=np.int8)
+        if model is None:
+            mn, G, cdf16, ppf = build_model(m)
+        else:
+            mn, cdf16 = model
+            cdf16 = np.asarray(cdf16, np.uint16)
+            G = len(cdf16) - 1
+            ppf = _ppf_from_cdf(cdf16)
+        cur, pay = encode_rows(m, mn, cdf16, n_threads)
+        cdf8 = (cdf16.astype(np.int64) & 0xFF).astype(np.uint8)
+        return cls(m.shape[0], m.shape[1], 1.0, cur, mn, G, cdf8, ppf, pay)
+
+    @staticmethod
+    def deserialize_file(path):
+        """Read a .bin v0 file.  Returns (header dict, vector int8, [CompressedMatrix])."""
+        return read_bin(path)
+
+    # -- device
+    def to_device(self, device="cuda"):
+        torch = _torch()
+        dev = torch.device(device)
+        self.d_cursors = torch.from_numpy(self.cursors.view(np.int32)).to(dev)
+        cdf = np.zeros(257, np.uint8)
+        cdf[:len(self.cdf_data)] = self.cdf_data
+        self.d_cdf_data = torch.from_numpy(cdf).to(dev)
+        self.d_ppf_data = torch.from_numpy(self.ppf_data.copy()).to(dev)
+        pay = np.zeros(self.payload_size + self.SLACK_WORDS, np.uint16)
+        pay[:self.payload_size] = self.payload[:self.payload_size]
+        self.d_payload = torch.from_numpy(pay.view(np.int16)).to(dev)
+        return self
+
+    def dev_struct(self) -> DevMatrix:
+        if self.d_payload is None:
+            raise RuntimeError("matrix is not on the device; call to_device() first")
+        return DevMatrix(self.rows, self.cols, self.grid_spacing, self.min_value, self.G, 0,
+                         self.payload_size, self.d_cursors.data_ptr(),
+                         self.d_cdf_data.data_ptr(), self.d_ppf_data.data_ptr(),
+                         self.d_payload.data_ptr())
+
+    def multiply(self, vec, out=None, stream=None):
+        """int32 W @ vec for one vector [cols] or a batch [n_vec, cols] (n_vec <= 8)."""
+        torch = _torch()
+        if self.d_payload is None:
+            raise RuntimeError("matrix is not on the device; call to_device() first")
+        squeeze = vec.dim() == 1
+        v = vec.unsqueeze(0) if squeeze else vec
+        if v.dtype != torch.int8 or v.shape[-1] != self.cols:
+            raise ValueError(f"expected int8 vectors of length {self.cols}")
+        n = v.shape[0]
+        if out is None:
+            out = torch.empty((n, self.rows), dtype=torch.int32, device=v.device)
+        call("matvans_decompress_and_multiply_nvec", _ptr(out), _ptr(v), n, v.stride(0),
+             self.rows, self.cols, _ptr(self.d_cursors), self.min_value, self.G,
+             _ptr(self.d_cdf_data), _ptr(self.d_ppf_data), _ptr(self.d_payload),
+             self.payload_size, _stream(stream))
+        return out[0] if squeeze else out
+
+    def decompress_and_mult(self, d_result8, d_result32, vector, d_delta, stream=None):
+        """Stream-ordered composite: decode-matvec -> absmax -> requant; delta stays on
+        the device (matvansWarpPerRow.cu:150-176)."""
+        s = self.dev_struct()
+        call("matvans_decompress_and_mult", ctypes.byref(s), _ptr(d_result8), _ptr(d_result32),
+             _ptr(vector), _ptr(d_delta), _stream(stream))
+
+    def decompressAndMult(self, d_result8, d_result32, vector, v_delta=1.0):
+        """Reference-shaped call (matvansWarpPerRow.cu:147): returns the new v_delta as a
+        host float (synchronises, as the reference's thrust reduction does).  The input
+        v_delta is ignored, as in the reference."""
+        torch = _torch()
+        d = torch.empty(1, dtype=torch.float32, device=d_result32.device)
+        self.decompress_and_mult(d_result8, d_result32, vector, d)
+        return float(d.item())
+
+
+def _ppf_from_cdf(cdf16):
+    c8 = (np.asarray(cdf16).astype(np.int64) & 0xFF)[:-1]
+    ppf = np.zeros(256, np.uint8)
+    for q in range(256):
+        ok = np.nonzero(c8 <= q)[0]
+        ppf[q] = ok.max() if len(ok) else 0
+    return ppf
+
+
+def read_bin(path):
+    """Parse a .bin v0 file with the library reader.
+    Returns (header dict, vector int8, [CompressedMatrix])."""
+    h = ctypes.c_void_p()
+    call("matvans_bin_read", str(path).encode(), ctypes.byref(h))
+    try:
+        num, rh, mwc, lv = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        vp = ctypes.c_void_p()
+        call("matvans_bin_header", h, ctypes.byref(num), ctypes.byref(rh), ctypes.byref(mwc),
+             ctypes.byref(vp), ctypes.byref(lv))
+        vec = np.ctypeslib.as_array(ctypes.cast(vp, ctypes.POINTER(ctypes.c_int8)),
+                                    (lv.value,)).copy() if lv.value else np.zeros(0, np.int8)
+        mats = []
+        for k in range(num.value):
+            m = HostMatrix()
+            call("matvans_bin_matrix", h, k, ctypes.byref(m))
+
+            def arr(p, ct, n):
+                return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ct)), (n,)).copy()
+            mats.append(CompressedMatrix(
+                m.rows, m.cols, m.grid_spacing, arr(m.cursors, ctypes.c_uint32, m.rows),
+                m.min_value, m.G, arr(m.cdf, ctypes.c_uint8, m.G + 1),
+                arr(m.ppf, ctypes.c_uint8, 256),
+                arr(m.payload, ctypes.c_uint16, m.payload_size) if m.payload_size
+                else np.zeros(0, np.uint16)))
+        hdr = {"num_matrices": num.value, "result_hash": rh.value, "max_word_count": mwc.value}
+        return hdr, vec, mats
+    finally:
+        lib().matvans_bin_free(h)
+
+
+def write_bin(path, result_hash, vector, matrices):
+    """Write a .bin v0 file with the library writer."""
+    vec = np.ascontiguousarray(vector, np.int8)
+    keep = []
+    arr = (HostMatrix * max(1, len(matrices)))()
+    for i, m in enumerate(matrices):
+        cdf = np.ascontiguousarray(m.cdf_data, np.uint8)
+        keep += [m.cursors, cdf, m.ppf_data, m.payload]
+        arr[i] = HostMatrix(m.rows, m.cols, m.grid_spacing, m.min_value, len(cdf) - 1, 0,
+                            m.payload_size, m.cursors.ctypes.data, cdf.ctypes.data,
+                            m.ppf_data.ctypes.data, m.payload.ctypes.data)
+    call("matvans_bin_write", str(path).encode(), int(result_hash), _np_ptr(vec), len(vec),
+         len(matrices), arr)
+
+
+# ------------------------------------------------------------------- device ops
+def decmpressAndMultiply(dst, vec, rows, cols, grid_spacing, cursors, min_value, G, cdf_data,
+                         ppf_data, payload, payload_size, stream=None):
+    """The reference kernel's argument list (matvansWarpPerRow-kernel.cu:46-50) on torch
+    device tensors; launches the fused HIP kernel on ``stream``."""
+    call("matvans_decompress_and_multiply", _ptr(dst), _ptr(vec), int(rows), int(cols),
+         float(grid_spacing), _ptr(cursors), int(min_value), int(G), _ptr(cdf_data),
+         _ptr(ppf_data), _ptr(payload), int(payload_size), _stream(stream))
+
+
+def decompress_and_multiply(matrix: CompressedMatrix, vec, out=None, stream=None):
+    return matrix.multiply(vec, out, stream)
+
+
+def int8_matvec(mat, vecs, out=None, stream=None):
+    """Uncompressed int8 comparator: int32 mat @ vecs^T, vecs [n_vec, cols] or [cols]."""
+    torch = _torch()
+    squeeze = vecs.dim() == 1
+    v = vecs.unsqueeze(0) if squeeze else vecs
+    rows, cols = mat.shape
+    if out is None:
+        out = torch.empty((v.shape[0], rows), dtype=torch.int32, device=mat.device)
+    call("matvans_int8_matvec", _ptr(out), _ptr(mat), _ptr(v), v.shape[0], v.stride(0), rows,
+         cols, _stream(stream))
+    return out[0] if squeeze else out
+
+
+def f32_gemv(mat, x, alpha=1.0, out=None, stream=None):
+    """fp32 comparator: alpha * mat @ x."""
+    torch = _torch()
+    rows, cols = mat.shape
+    if out is None:
+        out = torch.empty(rows, dtype=torch.float32, device=mat.device)
+    call("matvans_f32_gemv", _ptr(out), _ptr(mat), _ptr(x), rows, cols, float(alpha),
+         _stream(stream))
+    return out
+
+
+def absmax_requant(acc, clamp=False, out8=None, delta=None, stream=None):
+    """(int8 requantised, fp32 delta tensor[1]) of an int32 accumulator vector."""
+    torch = _torch()
+    n = acc.numel()
+    if out8 is None:
+        out8 = torch.empty(n, dtype=torch.int8, device=acc.device)
+    if delta is None:
+        delta = torch.empty(1, dtype=torch.float32, device=acc.device)
+    call("matvans_absmax_requant", _ptr(acc), n, _ptr(out8), _ptr(delta), int(bool(clamp)),
+         _stream(stream))
+    return out8, delta
+
+
+def scale_output(acc, s_w, s_x, out=None, stream=None):
+    """y[v, r] = s_x[v] * s_w[r] * float(acc[v, r])."""
+    torch = _torch()
+    a = acc if acc.dim() == 2 else acc.unsqueeze(0)
+    n_vec, rows = a.shape
+    if out is None:
+        out = torch.empty((n_vec, rows), dtype=torch.float32, device=acc.device)
+    call("matvans_scale_output", _ptr(a), rows, n_vec, _ptr(s_w), _ptr(s_x), _ptr(out),
+         _stream(stream))
+    return out if acc.dim() == 2 else out[0]
+
+
+def run_chain(matrices, vector, stream=None):
+    """The reference benchmark chain v_i = requant(W_i v_{i-1}) on the device
+    (matvansWarpPerRow.cu:320-350).  ``matrices`` are on the device; ``vector`` is an
+    int8 device tensor.  Returns the final int8 device vector (one host sync at the end
+    is left to the caller)."""
+    torch = _torch()
+    dev = vector.device
+    max_rows = max(m.rows for m in matrices)
+    r32 = torch.empty(max_rows, dtype=torch.int32, device=dev)
+    bufs = [torch.empty(max(max_rows, vector.numel()), dtype=torch.int8, device=dev)
+            for _ in range(2)]
+    delta = torch.empty(1, dtype=torch.float32, device=dev)
+    cur = vector
+    for k, m in enumerate(matrices):
+        out = bufs[k % 2]
+        m.decompress_and_mult(out, r32, cur, delta, stream)
+        cur = out[:m.rows]
+    return cur

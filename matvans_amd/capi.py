@@ -1,0 +1,94 @@
+"""ctypes binding of ``libmatvans.so`` (declared in ``include/matvans.h``).
+
+The library is loaded from ``matvans_amd/lib/libmatvans.so``.  There is no fallback:
+if the library is missing or fails to load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmatvans.so")
+
+P = ctypes.c_void_p
+u8, i8, u16, u32, u64, i32, f32 = (ctypes.c_uint8, ctypes.c_int8, ctypes.c_uint16,
+                                   ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_float)
+
+
+class MatvansError(RuntimeError):
+    """A matvans_* call returned a nonzero status."""
+
+    def __init__(self, fn, status, msg):
+        super().__init__(f"{fn} -> status {status}: {msg}")
+        self.status = status
+
+
+class DevMatrix(ctypes.Structure):
+    """matvans_dev_matrix."""
+    _fields_ = [("rows", u32), ("cols", u32), ("grid_spacing", f32), ("min_value", i8),
+                ("G", u8), ("reserved", u16), ("payload_size", u32), ("cursors", P),
+                ("cdf", P), ("ppf", P), ("payload", P)]
+
+
+class HostMatrix(ctypes.Structure):
+    """matvans_host_matrix."""
+    _fields_ = [("rows", u32), ("cols", u32), ("grid_spacing", f32), ("min_value", i8),
+                ("G", u8), ("reserved", u16), ("payload_size", u32), ("cursors", P),
+                ("cdf", P), ("ppf", P), ("payload", P)]
+
+
+# name -> (restype, argtypes); status-returning functions use i32
+SIGNATURES = {
+    "matvans_last_error": (ctypes.c_char_p, []),
+    "matvans_version": (i32, []),
+    "matvans_decompress_and_multiply": (i32, [P, P, u32, u32, f32, P, i8, u8, P, P, P, u32, P]),
+    "matvans_decompress_and_multiply_nvec": (i32, [P, P, u32, u32, u32, u32, P, i8, u8, P, P,
+                                                   P, u32, P]),
+    "matvans_decompress_and_mult": (i32, [ctypes.POINTER(DevMatrix), P, P, P, P, P]),
+    "matvans_absmax_requant": (i32, [P, u32, P, P, i32, P]),
+    "matvans_scale_output": (i32, [P, u32, u32, P, P, P, P]),
+    "matvans_int8_matvec": (i32, [P, P, P, u32, u32, u32, u32, P]),
+    "matvans_f32_gemv": (i32, [P, P, P, u32, u32, f32, P]),
+    "matvans_fxhash": (u32, [P, u64]),
+    "matvans_bin_read": (i32, [ctypes.c_char_p, ctypes.POINTER(P)]),
+    "matvans_bin_free": (None, [P]),
+    "matvans_bin_header": (i32, [P, ctypes.POINTER(u32), ctypes.POINTER(u32),
+                                 ctypes.POINTER(u32), ctypes.POINTER(P), ctypes.POINTER(u32)]),
+    "matvans_bin_matrix": (i32, [P, u32, ctypes.POINTER(HostMatrix)]),
+    "matvans_bin_write": (i32, [ctypes.c_char_p, u32, P, u32, u32,
+                                ctypes.POINTER(HostMatrix)]),
+    "matvans_build_model": (i32, [P, u64, ctypes.POINTER(i8), ctypes.POINTER(u32), P, P]),
+    "matvans_encode_bound": (u64, [u32, u32]),
+    "matvans_encode_rows": (i32, [P, u32, u32, i8, u32, P, u32, P, P, u64,
+                                  ctypes.POINTER(u64)]),
+}
+
+_LIB = None
+
+
+def lib():
+    """The loaded library; raises if it is not built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run python -m matvans_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def last_error() -> str:
+    return lib().matvans_last_error().decode()
+
+
+def call(name, *args):
+    """Call a status-returning entry point; raise MatvansError on failure."""
+    st = getattr(lib(), name)(*args)
+    if st != 0:
+        raise MatvansError(name, st, last_error())
+    return st

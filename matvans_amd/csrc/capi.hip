@@ -1,0 +1,125 @@
+// C ABI: argument validation and dispatch to the HIP kernels.  Every entry point
+// returns a matvans_status and never synchronises or allocates.
+#include <hip/hip_runtime.h>
+
+#include "launch.h"
+#include "matvans_internal.h"
+
+using mv::fail;
+
+namespace {
+inline bool aligned(const void *p, uintptr_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; }
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+int hip_status(hipError_t e, const char *what) {
+    if (e != hipSuccess) return fail(MATVANS_E_HIP, "%s: %s", what, hipGetErrorString(e));
+    mv::clear_error();
+    return MATVANS_OK;
+}
+
+int check_v0(const int32_t *dst, const int8_t *vec, uint32_t n_vec, uint32_t vec_stride,
+             uint32_t rows, uint32_t cols, const uint32_t *cursors, const uint8_t *cdf,
+             const uint8_t *ppf, const uint16_t *payload) {
+    if (!dst || !vec || !cursors || !cdf || !ppf || !payload)
+        return fail(MATVANS_E_INVALID, "null device pointer");
+    if (n_vec < 1 || n_vec > 8) return fail(MATVANS_E_INVALID, "n_vec %u outside 1..8", n_vec);
+    if (cols == 0 || cols % 512)
+        return fail(MATVANS_E_SHAPE, "cols %u is not a multiple of 512 (32 lanes x 16)", cols);
+    if (rows == 0) return fail(MATVANS_E_SHAPE, "rows == 0");
+    if (!aligned(payload, 16)) return fail(MATVANS_E_ALIGN, "payload not 16-byte aligned");
+    if (!aligned(vec, 16) || (n_vec > 1 && vec_stride % 16))
+        return fail(MATVANS_E_ALIGN, "vectors must be 16-byte aligned (stride %u)", vec_stride);
+    if (n_vec > 1 && vec_stride < cols)
+        return fail(MATVANS_E_INVALID, "vec_stride %u < cols %u", vec_stride, cols);
+    return MATVANS_OK;
+}
+}  // namespace
+
+extern "C" int matvans_decompress_and_multiply(int32_t *dst, const int8_t *vec, uint32_t rows,
+                                               uint32_t cols, float grid_spacing,
+                                               const uint32_t *cursors, int8_t min_value,
+                                               uint8_t G, const uint8_t *cdf_data,
+                                               const uint8_t *ppf_data,
+                                               const uint16_t *payload, uint32_t payload_size,
+                                               void *stream) {
+    (void)grid_spacing;   // unused by the reference kernel as well
+    return matvans_decompress_and_multiply_nvec(dst, vec, 1, cols, rows, cols, cursors,
+                                                min_value, G, cdf_data, ppf_data, payload,
+                                                payload_size, stream);
+}
+
+extern "C" int matvans_decompress_and_multiply_nvec(int32_t *dst, const int8_t *vecs,
+                                                    uint32_t n_vec, uint32_t vec_stride,
+                                                    uint32_t rows, uint32_t cols,
+                                                    const uint32_t *cursors, int8_t min_value,
+                                                    uint8_t G, const uint8_t *cdf_data,
+                                                    const uint8_t *ppf_data,
+                                                    const uint16_t *payload,
+                                                    uint32_t payload_size, void *stream) {
+    if (n_vec == 1 && vec_stride < cols) vec_stride = cols;
+    int st = check_v0(dst, vecs, n_vec, vec_stride, rows, cols, cursors, cdf_data, ppf_data,
+                      payload);
+    if (st) return st;
+    mv::V0Args a{dst, vecs, n_vec, vec_stride, rows, cols, cursors, min_value, G,
+                 cdf_data, ppf_data, payload, payload_size};
+    return hip_status(mv::launch_v0_decode_matvec(a, as_stream(stream)), "decode_matvec launch");
+}
+
+extern "C" int matvans_decompress_and_mult(const matvans_dev_matrix *m, int8_t *d_result8,
+                                           int32_t *d_result32, const int8_t *d_vector,
+                                           float *d_delta, void *stream) {
+    if (!m || !d_result8 || !d_result32 || !d_vector || !d_delta)
+        return fail(MATVANS_E_INVALID, "null argument");
+    int st = matvans_decompress_and_multiply(d_result32, d_vector, m->rows, m->cols,
+                                             m->grid_spacing, m->cursors, m->min_value, m->G,
+                                             m->cdf, m->ppf, m->payload, m->payload_size,
+                                             stream);
+    if (st) return st;
+    return matvans_absmax_requant(d_result32, m->rows, d_result8, d_delta, 0, stream);
+}
+
+extern "C" int matvans_absmax_requant(const int32_t *d_acc, uint32_t n, int8_t *d_out8,
+                                      float *d_delta, int clamp, void *stream) {
+    if (!d_acc || !d_out8 || !d_delta) return fail(MATVANS_E_INVALID, "null argument");
+    if (n == 0) return fail(MATVANS_E_SHAPE, "n == 0");
+    return hip_status(mv::launch_absmax_requant(d_acc, n, d_out8, d_delta, clamp,
+                                                as_stream(stream)),
+                      "absmax_requant launch");
+}
+
+extern "C" int matvans_scale_output(const int32_t *d_acc, uint32_t rows, uint32_t n_vec,
+                                    const float *d_s_w, const float *d_s_x, float *d_y,
+                                    void *stream) {
+    if (!d_acc || !d_s_w || !d_s_x || !d_y) return fail(MATVANS_E_INVALID, "null argument");
+    if (rows == 0 || n_vec == 0 || uint64_t(rows) * n_vec > 0xFFFFFFFFull)
+        return fail(MATVANS_E_SHAPE, "bad shape %u x %u", rows, n_vec);
+    return hip_status(mv::launch_scale_output(d_acc, rows, n_vec, d_s_w, d_s_x, d_y,
+                                              as_stream(stream)),
+                      "scale_output launch");
+}
+
+extern "C" int matvans_int8_matvec(int32_t *dst, const int8_t *mat, const int8_t *vecs,
+                                   uint32_t n_vec, uint32_t vec_stride, uint32_t rows,
+                                   uint32_t cols, void *stream) {
+    if (!dst || !mat || !vecs) return fail(MATVANS_E_INVALID, "null argument");
+    if (n_vec < 1 || n_vec > 8) return fail(MATVANS_E_INVALID, "n_vec %u outside 1..8", n_vec);
+    if (rows == 0 || cols == 0 || cols % 16)
+        return fail(MATVANS_E_SHAPE, "cols %u must be a positive multiple of 16", cols);
+    if (n_vec == 1 && vec_stride < cols) vec_stride = cols;
+    if (!aligned(mat, 16) || !aligned(vecs, 16) || vec_stride % 16)
+        return fail(MATVANS_E_ALIGN, "matrix and vectors must be 16-byte aligned");
+    return hip_status(mv::launch_int8_matvec(dst, mat, vecs, n_vec, vec_stride, rows, cols,
+                                             as_stream(stream)),
+                      "int8_matvec launch");
+}
+
+extern "C" int matvans_f32_gemv(float *y, const float *mat, const float *x, uint32_t rows,
+                                uint32_t cols, float alpha, void *stream) {
+    if (!y || !mat || !x) return fail(MATVANS_E_INVALID, "null argument");
+    if (rows == 0 || cols == 0 || cols % 4)
+        return fail(MATVANS_E_SHAPE, "cols %u must be a positive multiple of 4", cols);
+    if (!aligned(mat, 16) || !aligned(x, 16))
+        return fail(MATVANS_E_ALIGN, "matrix and x must be 16-byte aligned");
+    return hip_status(mv::launch_f32_gemv(y, mat, x, rows, cols, alpha, as_stream(stream)),
+                      "f32_gemv launch");
+}

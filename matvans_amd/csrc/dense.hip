@@ -1,0 +1,256 @@
+// Same-GPU comparators and epilogues.
+//
+//  int8_matvec   replaces vecMat_int8_warpRows (baselines/baseline_int8/
+//                baseline_matVec_custom.cu:99-155): uncompressed row-major int8 W, int32
+//                dot with v_dot4_i32_i8, 1..8 vectors per pass over W.
+//  f32_gemv      replaces cublasSgemv (baselines/cuBLAS_baseline_float/
+//                baseline_matVec_cuBLAS.cu:111-144) on the row-major fp32 W.
+//  absmax_requant  absMaxWithThrustDevice + normalizeAndRoundtoInt8
+//                (matvansWarpPerRow/matvansWarpPerRow.cu:51-72), in one stream-ordered
+//                launch that leaves delta on the device.
+//  scale_output  y = s_x * s_w[r] * float(acc).
+//
+// Both matvecs are HBM-bound: each wave streams whole rows with 16-byte nontemporal
+// loads (four in flight per lane) while the activation vectors sit in LDS.
+#include <hip/hip_runtime.h>
+
+#include "launch.h"
+
+namespace mv {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kRowsPerBlock = 8;   // 2 rows per wave, 4 waves
+constexpr uint32_t kMaxLdsBytes = 64 * 1024;
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// 16-byte streaming loads (the matrix is read exactly once)
+__device__ __forceinline__ uint4 ntload(const uint4 *p) {
+    u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
+    return make_uint4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ float4 ntload(const float4 *p) {
+    f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(p));
+    return make_float4(v[0], v[1], v[2], v[3]);
+}
+
+__device__ __forceinline__ int32_t wave_sum(int32_t a) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    return a;
+}
+__device__ __forceinline__ float wave_sumf(float a) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    return a;
+}
+
+template <int NV, bool LDS>
+__global__ __launch_bounds__(kThreads) void int8_matvec_kernel(
+    int32_t *__restrict__ dst, const int8_t *__restrict__ mat, const int8_t *__restrict__ vecs,
+    uint32_t vec_stride, uint32_t rows, uint32_t cols) {
+    extern __shared__ uint4 svec[];   // [NV][cols/16] when LDS
+    const uint32_t nch = cols / 16;
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    if (LDS) {
+        for (uint32_t v = 0; v < NV; v++)
+            for (uint32_t c = tid; c < nch; c += kThreads)
+                svec[v * nch + c] =
+                    *reinterpret_cast<const uint4 *>(vecs + size_t(v) * vec_stride + c * 16);
+        __syncthreads();
+    }
+    auto vchunk = [&](uint32_t v, uint32_t c) -> uint4 {
+        if (LDS) return svec[v * nch + c];
+        return *reinterpret_cast<const uint4 *>(vecs + size_t(v) * vec_stride + c * 16);
+    };
+    const uint32_t rbeg = blockIdx.x * kRowsPerBlock;
+    for (uint32_t row = rbeg + wave; row < rows && row < rbeg + kRowsPerBlock; row += 4) {
+        const uint4 *m = reinterpret_cast<const uint4 *>(mat + size_t(row) * cols);
+        int32_t acc[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) acc[v] = 0;
+        for (uint32_t c0 = lane; c0 < nch; c0 += 256) {
+            uint4 a[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                uint32_t c = c0 + 64 * u;
+                a[u] = c < nch ? ntload(m + c) : make_uint4(0, 0, 0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                uint32_t c = c0 + 64 * u;
+                if (c >= nch) break;
+#pragma unroll
+                for (int v = 0; v < NV; v++) {
+                    uint4 b = vchunk(v, c);
+                    acc[v] = __builtin_amdgcn_sdot4(int(a[u].x), int(b.x), acc[v], false);
+                    acc[v] = __builtin_amdgcn_sdot4(int(a[u].y), int(b.y), acc[v], false);
+                    acc[v] = __builtin_amdgcn_sdot4(int(a[u].z), int(b.z), acc[v], false);
+                    acc[v] = __builtin_amdgcn_sdot4(int(a[u].w), int(b.w), acc[v], false);
+                }
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            int32_t s = wave_sum(acc[v]);
+            if (lane == 0) dst[size_t(v) * rows + row] = s;
+        }
+    }
+}
+
+template <bool LDS>
+__global__ __launch_bounds__(kThreads) void f32_gemv_kernel(float *__restrict__ y,
+                                                            const float *__restrict__ mat,
+                                                            const float *__restrict__ x,
+                                                            uint32_t rows, uint32_t cols,
+                                                            float alpha) {
+    extern __shared__ float4 sx[];
+    const uint32_t nch = cols / 4;
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    if (LDS) {
+        for (uint32_t c = tid; c < nch; c += kThreads)
+            sx[c] = reinterpret_cast<const float4 *>(x)[c];
+        __syncthreads();
+    }
+    const uint32_t rbeg = blockIdx.x * kRowsPerBlock;
+    for (uint32_t row = rbeg + wave; row < rows && row < rbeg + kRowsPerBlock; row += 4) {
+        const float4 *m = reinterpret_cast<const float4 *>(mat + size_t(row) * cols);
+        float acc = 0.f;
+        for (uint32_t c0 = lane; c0 < nch; c0 += 256) {
+            float4 a[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                uint32_t c = c0 + 64 * u;
+                a[u] = c < nch ? ntload(m + c) : make_float4(0, 0, 0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                uint32_t c = c0 + 64 * u;
+                if (c >= nch) break;
+                float4 b = LDS ? sx[c] : reinterpret_cast<const float4 *>(x)[c];
+                acc = fmaf(a[u].x, b.x, acc);
+                acc = fmaf(a[u].y, b.y, acc);
+                acc = fmaf(a[u].z, b.z, acc);
+                acc = fmaf(a[u].w, b.w, acc);
+            }
+        }
+        float s = wave_sumf(acc);
+        if (lane == 0) y[row] = alpha * s;
+    }
+}
+
+// One block: block-wide max of |acc|, then the requantisation of every element.
+__global__ __launch_bounds__(1024) void absmax_requant_kernel(const int32_t *__restrict__ acc,
+                                                              uint32_t n, int8_t *__restrict__ out8,
+                                                              float *__restrict__ delta,
+                                                              int clamp) {
+    __shared__ uint32_t red[16];
+    uint32_t m = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        int32_t a = acc[i];
+        uint32_t u = a < 0 ? 0u - uint32_t(a) : uint32_t(a);
+        m = u > m ? u : m;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        uint32_t t = __shfl_xor(m, o, 64);
+        m = t > m ? t : m;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        uint32_t t = threadIdx.x < (blockDim.x >> 6) ? red[threadIdx.x] : 0u;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            uint32_t s = __shfl_xor(t, o, 64);
+            t = s > t ? s : t;
+        }
+        if (threadIdx.x == 0) red[0] = t;
+    }
+    __syncthreads();
+    // thrust::transform_reduce of float(abs(x)) with an integer max: float(max |acc|)
+    const float a = static_cast<float>(red[0]);
+    const float d = a / 127.0f;   // IEEE fp32 division (v_delta = abs_max / 127)
+    if (threadIdx.x == 0) *delta = d;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        float q = rintf(static_cast<float>(acc[i]) / d);   // __float2int_rn(afl / scalar)
+        int32_t v = (q == q) ? static_cast<int32_t>(fminf(fmaxf(q, -2147483648.0f),
+                                                          2147483520.0f))
+                             : 0;
+        if (clamp) v = v < -128 ? -128 : (v > 127 ? 127 : v);
+        out8[i] = static_cast<int8_t>(v);
+    }
+}
+
+__global__ void scale_output_kernel(const int32_t *__restrict__ acc, uint32_t rows,
+                                    uint32_t n_vec, const float *__restrict__ s_w,
+                                    const float *__restrict__ s_x, float *__restrict__ y) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows * n_vec) return;
+    uint32_t v = i / rows, r = i - v * rows;
+    y[i] = (s_x[v] * s_w[r]) * static_cast<float>(acc[i]);
+}
+
+template <int NV>
+hipError_t int8_nv(int32_t *dst, const int8_t *mat, const int8_t *vecs, uint32_t vec_stride,
+                   uint32_t rows, uint32_t cols, hipStream_t s) {
+    dim3 grid((rows + kRowsPerBlock - 1) / kRowsPerBlock), block(kThreads);
+    uint32_t lds = NV * cols;
+    if (lds <= kMaxLdsBytes)
+        hipLaunchKernelGGL((int8_matvec_kernel<NV, true>), grid, block, lds, s, dst, mat, vecs,
+                           vec_stride, rows, cols);
+    else
+        hipLaunchKernelGGL((int8_matvec_kernel<NV, false>), grid, block, 0, s, dst, mat, vecs,
+                           vec_stride, rows, cols);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_int8_matvec(int32_t *dst, const int8_t *mat, const int8_t *vecs,
+                              uint32_t n_vec, uint32_t vec_stride, uint32_t rows, uint32_t cols,
+                              hipStream_t s) {
+    switch (n_vec) {
+        case 1: return int8_nv<1>(dst, mat, vecs, vec_stride, rows, cols, s);
+        case 2: return int8_nv<2>(dst, mat, vecs, vec_stride, rows, cols, s);
+        case 3: return int8_nv<3>(dst, mat, vecs, vec_stride, rows, cols, s);
+        case 4: return int8_nv<4>(dst, mat, vecs, vec_stride, rows, cols, s);
+        case 5: return int8_nv<5>(dst, mat, vecs, vec_stride, rows, cols, s);
+        case 6: return int8_nv<6>(dst, mat, vecs, vec_stride, rows, cols, s);
+        case 7: return int8_nv<7>(dst, mat, vecs, vec_stride, rows, cols, s);
+        case 8: return int8_nv<8>(dst, mat, vecs, vec_stride, rows, cols, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_f32_gemv(float *y, const float *mat, const float *x, uint32_t rows,
+                           uint32_t cols, float alpha, hipStream_t s) {
+    dim3 grid((rows + kRowsPerBlock - 1) / kRowsPerBlock), block(kThreads);
+    uint32_t lds = cols * 4u;
+    if (lds <= kMaxLdsBytes)
+        hipLaunchKernelGGL((f32_gemv_kernel<true>), grid, block, lds, s, y, mat, x, rows, cols,
+                           alpha);
+    else
+        hipLaunchKernelGGL((f32_gemv_kernel<false>), grid, block, 0, s, y, mat, x, rows, cols,
+                           alpha);
+    return hipGetLastError();
+}
+
+hipError_t launch_absmax_requant(const int32_t *acc, uint32_t n, int8_t *out8, float *delta,
+                                 int clamp, hipStream_t s) {
+    hipLaunchKernelGGL(absmax_requant_kernel, dim3(1), dim3(1024), 0, s, acc, n, out8, delta,
+                       clamp);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale_output(const int32_t *acc, uint32_t rows, uint32_t n_vec,
+                               const float *s_w, const float *s_x, float *y, hipStream_t s) {
+    uint32_t n = rows * n_vec;
+    hipLaunchKernelGGL(scale_output_kernel, dim3((n + 255) / 256), dim3(256), 0, s, acc, rows,
+                       n_vec, s_w, s_x, y);
+    return hipGetLastError();
+}
+
+}  // namespace mv

@@ -1,0 +1,174 @@
+"""GPU parity of the fused decode-matvec against the oracle (bit-exact int32).
+
+Every launch goes through the C ABI of libmatvans.so; the oracle (oracle/) is only the
+checker.  Cases: the golden files written by the reference generator, the reference
+chain hash, small ragged shapes, 1..8 vectors, and the full LLM shapes."""
+import ctypes
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import matvans_amd as mv
+from matvans_amd.capi import MatvansError
+from oracle import oracle as O
+
+from .helpers import gaussian_int8, gaussian_vec, laplace_int8, uniform_vec
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = ["compressed_matrices_512_5bit_count2.bin", "compressed_matrices_1024_5bit_count1.bin"]
+
+
+def note(msg):
+    sys.stdout.write(f"[case] {msg}\n")
+    sys.stdout.flush()
+
+
+def oracle_rec(cm):
+    return O.Record(cm.rows, cm.cols, cm.grid_spacing, cm.cursors, cm.payload_size,
+                    cm.min_value, cm.G, cm.cdf_data, cm.ppf_data, cm.payload)
+
+
+def gpu_matvec(cm, vecs):
+    import torch
+    v = torch.from_numpy(np.ascontiguousarray(vecs)).cuda()
+    out = cm.multiply(v)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+@pytest.mark.parametrize("fx", FIXTURES)
+def test_golden_files(cuda, golden_dir, fx):
+    hdr, vec, mats = mv.read_bin(os.path.join(golden_dir, fx))
+    for k, cm in enumerate(mats):
+        note(f"{fx} matrix {k} rows={cm.rows} cols={cm.cols}")
+        cm.to_device()
+        dense = O.c_decode(oracle_rec(cm))
+        vecs = np.stack([vec[:cm.cols], uniform_vec(cm.cols, seed=k),
+                         gaussian_vec(cm.cols, seed=k)])
+        got = gpu_matvec(cm, vecs)
+        for i in range(3):
+            assert np.array_equal(got[i], O.matvec_i32(dense, vecs[i]))
+            assert np.array_equal(got[i], O.c_matvec(oracle_rec(cm), vecs[i]))
+
+
+def test_golden_chain_hash(cuda, golden_dir):
+    import torch
+    hdr, vec, mats = mv.read_bin(os.path.join(golden_dir, FIXTURES[0]))
+    note("chain over the 512 golden file")
+    for m in mats:
+        m.to_device()
+    out = mv.run_chain(mats, torch.from_numpy(vec).cuda())
+    torch.cuda.synchronize()
+    assert mv.hash_int8_array(out.cpu().numpy()) == hdr["result_hash"]
+    # per-step: int8 output and delta equal the fp32 oracle epilogue bit for bit
+    v = vec
+    d_vec = torch.from_numpy(vec).cuda()
+    for m in mats:
+        r8 = torch.empty(m.rows, dtype=torch.int8, device="cuda")
+        r32 = torch.empty(m.rows, dtype=torch.int32, device="cuda")
+        delta = m.decompressAndMult(r8, r32, d_vec)
+        ref8, refd = O.requant_f32(O.matvec_i32(O.c_decode(oracle_rec(m)), v))
+        assert np.float32(delta) == refd
+        assert np.array_equal(r8.cpu().numpy(), ref8)
+        v, d_vec = ref8, r8
+
+
+SMALL = [  # rows, cols, generator, parameter
+    (1, 512, "g", 6.0), (2, 512, "g", 6.0), (3, 1024, "g", 2.0), (7, 512, "g", 0.3),
+    (8, 1536, "l", 4.0), (9, 2048, "g", 16.0), (15, 512, "g", 60.0), (64, 4096, "l", 4.0),
+    (129, 3072, "g", 6.0), (1000, 1024, "g", 6.0), (333, 8192, "l", 4.0)]
+
+
+@pytest.mark.parametrize("rows,cols,kind,p", SMALL)
+def test_shapes_and_vectors(cuda, rows, cols, kind, p):
+    note(f"shape {rows}x{cols} {kind}{p}")
+    maker = gaussian_int8 if kind == "g" else laplace_int8
+    m, _ = maker(rows, cols, p, seed=rows * 7 + cols)
+    cm = mv.CompressedMatrix.encode(m).to_device()
+    for n_vec in (1, 3, 8):
+        vecs = np.stack([uniform_vec(cols, seed=100 + i) for i in range(n_vec)])
+        got = gpu_matvec(cm, vecs)
+        want = (m.astype(np.int64) @ vecs.astype(np.int64).T).T.astype(np.int32)
+        assert np.array_equal(got, want), (n_vec, np.argwhere(got != want)[:5])
+
+
+@pytest.mark.parametrize("val", [-127, 0, 127])
+def test_constant_matrix(cuda, val):
+    note(f"constant {val}")
+    m = np.full((6, 1024), val, np.int8)
+    cm = mv.CompressedMatrix.encode(m).to_device()
+    v = np.full(1024, -127, np.int8)
+    assert np.array_equal(gpu_matvec(cm, v[None])[0], O.matvec_i32(m, v))
+
+
+LLM = [(4096, 4096, "l", 4.0), (14336, 4096, "l", 4.0), (4096, 14336, "l", 4.0),
+       (4096, 14336, "g", 6.0)]
+
+
+@pytest.mark.parametrize("rows,cols,kind,p", LLM)
+def test_llm_shapes(cuda, rows, cols, kind, p):
+    note(f"LLM shape {rows}x{cols} {kind}{p}")
+    maker = gaussian_int8 if kind == "g" else laplace_int8
+    m, _ = maker(rows, cols, p, seed=rows + cols)
+    cm = mv.CompressedMatrix.encode(m, n_threads=16).to_device()
+    vecs = np.stack([gaussian_vec(cols, seed=3), uniform_vec(cols, seed=4)])
+    got = gpu_matvec(cm, vecs)
+    want = (m.astype(np.int64) @ vecs.astype(np.int64).T).T.astype(np.int32)
+    assert np.array_equal(got, want)
+    assert np.array_equal(got[0], O.c_matvec(oracle_rec(cm), vecs[0], 16))
+
+
+def test_reference_signature_entry(cuda):
+    import torch
+    note("decmpressAndMultiply argument list")
+    m, _ = gaussian_int8(40, 1024, 6.0, seed=2)
+    cm = mv.CompressedMatrix.encode(m).to_device()
+    v = uniform_vec(1024, seed=9)
+    dv = torch.from_numpy(v).cuda()
+    dst = torch.empty(40, dtype=torch.int32, device="cuda")
+    mv.decmpressAndMultiply(dst, dv, cm.rows, cm.cols, 1.0, cm.d_cursors, cm.min_value, cm.G,
+                            cm.d_cdf_data, cm.d_ppf_data, cm.d_payload, cm.payload_size)
+    torch.cuda.synchronize()
+    assert np.array_equal(dst.cpu().numpy(), O.matvec_i32(m, v))
+
+
+def test_side_stream(cuda):
+    import torch
+    note("launch on a side stream")
+    m, _ = gaussian_int8(100, 2048, 6.0, seed=4)
+    cm = mv.CompressedMatrix.encode(m).to_device()
+    v = uniform_vec(2048, seed=5)
+    dv = torch.from_numpy(v).cuda()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        out = cm.multiply(dv, stream=s)
+    s.synchronize()
+    assert np.array_equal(out.cpu().numpy(), O.matvec_i32(m, v))
+
+
+def test_argument_errors_do_not_launch(cuda):
+    import torch
+    note("argument validation")
+    m, _ = gaussian_int8(8, 512, 6.0)
+    cm = mv.CompressedMatrix.encode(m).to_device()
+    dv = torch.zeros(1024, dtype=torch.int8, device="cuda")
+    dst = torch.empty(8, dtype=torch.int32, device="cuda")
+    with pytest.raises(MatvansError) as e:   # payload not 16-byte aligned
+        mv.capi.call("matvans_decompress_and_multiply", ctypes.c_void_p(dst.data_ptr()),
+                     ctypes.c_void_p(dv.data_ptr()), 8, 512, 1.0,
+                     ctypes.c_void_p(cm.d_cursors.data_ptr()), cm.min_value, cm.G,
+                     ctypes.c_void_p(cm.d_cdf_data.data_ptr()),
+                     ctypes.c_void_p(cm.d_ppf_data.data_ptr()),
+                     ctypes.c_void_p(cm.d_payload.data_ptr() + 2), cm.payload_size, None)
+    assert e.value.status == 3
+    with pytest.raises(MatvansError) as e:   # cols not a multiple of 512
+        mv.decmpressAndMultiply(dst, dv, 8, 500, 1.0, cm.d_cursors, cm.min_value, cm.G,
+                                cm.d_cdf_data, cm.d_ppf_data, cm.d_payload, cm.payload_size)
+    assert e.value.status == 2
+    with pytest.raises(MatvansError) as e:   # 9 vectors
+        cm.multiply(torch.zeros((9, 512), dtype=torch.int8, device="cuda"))
+    assert e.value.status == 1
