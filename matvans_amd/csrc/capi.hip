@@ -2,6 +2,8 @@
 // returns a matvans_status and never synchronises or allocates.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "launch.h"
 #include "matvans_internal.h"
 
@@ -15,6 +17,16 @@ int hip_status(hipError_t e, const char *what) {
     if (e != hipSuccess) return fail(MATVANS_E_HIP, "%s: %s", what, hipGetErrorString(e));
     mv::clear_error();
     return MATVANS_OK;
+}
+
+// Decode-step variant: 1 (rotated state) unless MATVANS_DECODE_VARIANT=0 selects the
+// reference-form step, which is kept for A/B measurements.
+int decode_variant() {
+    static const int v = [] {
+        const char *e = std::getenv("MATVANS_DECODE_VARIANT");
+        return (e && e[0] == '0') ? 0 : 1;
+    }();
+    return v;
 }
 
 int check_v0(const int32_t *dst, const int8_t *vec, uint32_t n_vec, uint32_t vec_stride,
@@ -61,7 +73,7 @@ extern "C" int matvans_decompress_and_multiply_nvec(int32_t *dst, const int8_t *
                       payload);
     if (st) return st;
     mv::V0Args a{dst, vecs, n_vec, vec_stride, rows, cols, cursors, min_value, G,
-                 cdf_data, ppf_data, payload, payload_size};
+                 cdf_data, ppf_data, payload, payload_size, decode_variant()};
     return hip_status(mv::launch_v0_decode_matvec(a, as_stream(stream)), "decode_matvec launch");
 }
 
