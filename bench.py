@@ -117,6 +117,16 @@ def main():
     import matvans_amd as mv
 
     dev = torch.device("cuda:0")
+    if args.only == "copy":
+        # calibration for HBM byte counters: a plain device copy of known size
+        n = 64 << 20
+        a = torch.randint(0, 1 << 30, (n,), dtype=torch.int32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(max(1, args.steps)):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"kernel": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n}))
+        return
     rows, cols, nvec = args.rows, args.cols, args.nvec
     w, x = gen(rows, cols, args.dist)
     cm = mv.CompressedMatrix.encode(w, n_threads=0)

@@ -60,6 +60,14 @@ int matvans_decompress_and_multiply(int32_t *dst, const int8_t *vec, uint32_t ro
                                     const uint16_t *payload, uint32_t payload_size,
                                     void *stream);
 
+/* Decode kernel variant used by the launch entry points (all give the same bits):
+ * 0 = reference form of the step, 1 = rotated state with dword tables (default),
+ * 2 = rotated state with three conflict-free 256-byte tables; -1 restores the default
+ * (or MATVANS_DECODE_VARIANT).  No reference counterpart: a tuning control for
+ * measurements and the autotuner.  Process-wide. */
+int matvans_set_decode_variant(int variant);
+int matvans_get_decode_variant(void);
+
 /* A1 with 1..8 activation vectors decoded once: vecs[v*vec_stride + c] (int8),
  * dst[v*rows + r] (int32).  No reference counterpart (the reference multiplies one
  * vector per launch, matvansWarpPerRow.cu:161-164). */

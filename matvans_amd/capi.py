@@ -42,6 +42,8 @@ class HostMatrix(ctypes.Structure):
 SIGNATURES = {
     "matvans_last_error": (ctypes.c_char_p, []),
     "matvans_version": (i32, []),
+    "matvans_set_decode_variant": (i32, [i32]),
+    "matvans_get_decode_variant": (i32, []),
     "matvans_decompress_and_multiply": (i32, [P, P, u32, u32, f32, P, i8, u8, P, P, P, u32, P]),
     "matvans_decompress_and_multiply_nvec": (i32, [P, P, u32, u32, u32, u32, P, i8, u8, P, P,
                                                    P, u32, P]),

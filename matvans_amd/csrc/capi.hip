@@ -2,6 +2,7 @@
 // returns a matvans_status and never synchronises or allocates.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdlib>
 
 #include "launch.h"
@@ -19,13 +20,17 @@ int hip_status(hipError_t e, const char *what) {
     return MATVANS_OK;
 }
 
-// Decode-step variant: 1 (rotated state) unless MATVANS_DECODE_VARIANT=0 selects the
-// reference-form step, which is kept for A/B measurements.
+// Decode kernel variant, all computing the same bits: 0 = reference form of the step,
+// 1 = rotated state with dword tables (default), 2 = rotated state with three
+// conflict-free byte tables.  MATVANS_DECODE_VARIANT overrides the default for A/B runs.
+std::atomic<int> g_variant{-1};
+
 int decode_variant() {
-    static const int v = [] {
-        const char *e = std::getenv("MATVANS_DECODE_VARIANT");
-        return (e && e[0] == '0') ? 0 : 1;
-    }();
+    int v = g_variant.load(std::memory_order_relaxed);
+    if (v >= 0) return v;
+    const char *e = std::getenv("MATVANS_DECODE_VARIANT");
+    v = (e && e[0] >= '0' && e[0] <= '2' && e[1] == 0) ? e[0] - '0' : 1;
+    g_variant.store(v, std::memory_order_relaxed);
     return v;
 }
 
@@ -46,6 +51,15 @@ int check_v0(const int32_t *dst, const int8_t *vec, uint32_t n_vec, uint32_t vec
     return MATVANS_OK;
 }
 }  // namespace
+
+extern "C" int matvans_set_decode_variant(int variant) {
+    if (variant < -1 || variant > 2) return fail(MATVANS_E_INVALID, "variant %d outside -1..2", variant);
+    g_variant.store(variant, std::memory_order_relaxed);
+    mv::clear_error();
+    return MATVANS_OK;
+}
+
+extern "C" int matvans_get_decode_variant(void) { return decode_variant(); }
 
 extern "C" int matvans_decompress_and_multiply(int32_t *dst, const int8_t *vec, uint32_t rows,
                                                uint32_t cols, float grid_spacing,

@@ -63,7 +63,11 @@ __device__ __forceinline__ RowMajorSetup row_major_setup(uint32_t row0, uint32_t
     return s;
 }
 
-template <int NV>
+// TABLE 0: dword tables A[q] = f | sym << 24 and B[q] = bias, read with one
+// ds_read2st64_b32.  TABLE 2: three 256-byte tables f, bias, sym (64 dwords each, one per
+// LDS bank, so every read is conflict-free); same speed on the shapes measured, kept for
+// the per-row table format whose tables must be small.
+template <int NV, int TABLE>
 __global__ __launch_bounds__(256) void v0_fwd(int32_t *__restrict__ dst,
                                               const int8_t *__restrict__ vecs,
                                               uint32_t vec_stride, uint32_t rows, uint32_t cols,
@@ -75,23 +79,31 @@ __global__ __launch_bounds__(256) void v0_fwd(int32_t *__restrict__ dst,
                                               uint32_t payload_size) {
     __shared__ uint16_t ring[kWaves * kRingWords * 64];
     __shared__ uint32_t lut[512];
+    __shared__ uint16_t dummy[kDummyBytes / 2];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid >> 6, lane = tid & 63, sl = lane & 31;
     {
         uint32_t f, sym, bias;
         v0_table_entry(tid, min_value, G, cdf, ppf, f, sym, bias);
-        lut[tid] = f | (sym << 24);
-        lut[tid + 256] = bias;
+        if (TABLE == 0) {
+            lut[tid] = f | (sym << 24);
+            lut[tid + 256] = bias;
+        } else {
+            uint8_t *tab = reinterpret_cast<uint8_t *>(lut);
+            tab[tid] = static_cast<uint8_t>(f);
+            tab[256 + tid] = static_cast<uint8_t>(bias);
+            tab[512 + tid] = static_cast<uint8_t>(sym);
+        }
     }
     RowMajorSetup rs = row_major_setup(blockIdx.x * kRowsPerBlock + wave * 2, lane, rows,
                                        cursors, payload_size);
     V0Lane st;
-    st.init(payload, rs.win_beg, rs.win_end, rs.valid, rs.row_cur, rs.row_end, sl, ring, wave,
-            lane);
+    st.init(payload, rs.win_beg, rs.win_end, rs.valid, rs.row_cur, rs.row_end, sl, ring, dummy,
+            wave, lane);
     st.prologue();
 
-    // group t needs bytes [512t + 16*sl, +16) of each vector, loaded two groups ahead
+    // group t needs bytes [512t + 16*sl, +16) of each vector, loaded one group ahead
     const uint32_t groups = cols / 512u;
     auto vload = [&](uint32_t t, VecChunk<NV> &c) {
         uint32_t tt = t < groups ? t : groups - 1;
@@ -114,11 +126,19 @@ __global__ __launch_bounds__(256) void v0_fwd(int32_t *__restrict__ dst,
         uint32_t wbefore = st.waddr;
 #pragma unroll
         for (int d = 0; d < 4; d++) {
-            uint32_t e0, e1, e2, e3;
-            st.pair(lut, e0, e1);
-            st.pair(lut, e2, e3);
-            uint32_t packed = __builtin_amdgcn_perm(e1, e0, 0x0c0c0703u) |
-                              __builtin_amdgcn_perm(e3, e2, 0x07030c0cu);
+            uint32_t e0, e1, e2, e3, packed;
+            if (TABLE == 0) {
+                st.pair(lut, e0, e1);
+                st.pair(lut, e2, e3);
+                packed = __builtin_amdgcn_perm(e1, e0, 0x0c0c0703u) |
+                         __builtin_amdgcn_perm(e3, e2, 0x07030c0cu);
+            } else {
+                const uint8_t *tab = reinterpret_cast<const uint8_t *>(lut);
+                st.pair_u8(tab, e0, e1);
+                st.pair_u8(tab, e2, e3);
+                packed = __builtin_amdgcn_perm(e1, e0, 0x0c0c0400u) |
+                         __builtin_amdgcn_perm(e3, e2, 0x04000c0cu);
+            }
 #pragma unroll
             for (int v = 0; v < NV; v++)
                 acc[v] = __builtin_amdgcn_sdot4(static_cast<int>(packed),
@@ -128,20 +148,21 @@ __global__ __launch_bounds__(256) void v0_fwd(int32_t *__restrict__ dst,
         st.end_group(wbefore);
     };
 
-    for (uint32_t t = 0; t < groups; t += 2) {
+    // straight-line body (two groups, A/B loads) so that every wait counts exactly
+    uint32_t t = 0;
+    // (each vector chunk is reloaded in place right after its group has used it, so no
+    // register copy of a load in flight is needed at the back edge)
+    for (; t + 1 < groups; t += 2) {
         st.refill_a();
-        {
-            VecChunk<NV> cur = vc0;
-            vload(t + 2, vc0);
-            group(cur);
-        }
-        if (t + 1 >= groups) break;
+        group(vc0);
+        vload(t + 2, vc0);
         st.refill_b();
-        {
-            VecChunk<NV> cur = vc1;
-            vload(t + 3, vc1);
-            group(cur);
-        }
+        group(vc1);
+        vload(t + 3, vc1);
+    }
+    if (t < groups) {
+        st.refill_a();
+        group(vc0);
     }
 
 #pragma unroll
@@ -175,6 +196,7 @@ __global__ __launch_bounds__(256) void v0_fwd_ref(int32_t *__restrict__ dst,
                                                   uint32_t payload_size) {
     __shared__ uint16_t ring[kWaves * kRingWords * 64];
     __shared__ uint32_t lut[256];
+    __shared__ uint16_t dummy[kDummyBytes / 2];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid >> 6, lane = tid & 63, sl = lane & 31;
@@ -186,8 +208,8 @@ __global__ __launch_bounds__(256) void v0_fwd_ref(int32_t *__restrict__ dst,
     RowMajorSetup rs = row_major_setup(blockIdx.x * kRowsPerBlock + wave * 2, lane, rows,
                                        cursors, payload_size);
     V0Lane st;
-    st.init(payload, rs.win_beg, rs.win_end, rs.valid, rs.row_cur, rs.row_end, sl, ring, wave,
-            lane);
+    st.init(payload, rs.win_beg, rs.win_end, rs.valid, rs.row_cur, rs.row_end, sl, ring, dummy,
+            wave, lane);
     st.prologue();
     const uint32_t groups = cols / 512u;
     auto vload = [&](uint32_t t, VecChunk<NV> &c) {
@@ -231,20 +253,21 @@ __global__ __launch_bounds__(256) void v0_fwd_ref(int32_t *__restrict__ dst,
         }
     };
 
-    for (uint32_t t = 0; t < groups; t += 2) {
+    // straight-line body (two groups, A/B loads) so that every wait counts exactly
+    uint32_t t = 0;
+    // (each vector chunk is reloaded in place right after its group has used it, so no
+    // register copy of a load in flight is needed at the back edge)
+    for (; t + 1 < groups; t += 2) {
         st.refill_a();
-        {
-            VecChunk<NV> cur = vc0;
-            vload(t + 2, vc0);
-            group(cur);
-        }
-        if (t + 1 >= groups) break;
+        group(vc0);
+        vload(t + 2, vc0);
         st.refill_b();
-        {
-            VecChunk<NV> cur = vc1;
-            vload(t + 3, vc1);
-            group(cur);
-        }
+        group(vc1);
+        vload(t + 3, vc1);
+    }
+    if (t < groups) {
+        st.refill_a();
+        group(vc0);
     }
 #pragma unroll
     for (int v = 0; v < NV; v++) {
@@ -271,10 +294,14 @@ static hipError_t launch_nv(const V0Args &a, hipStream_t s) {
         hipLaunchKernelGGL((v0_fwd_ref<NV>), grid, block, 0, s, a.dst, a.vecs, a.vec_stride,
                            a.rows, a.cols, a.cursors, int(a.min_value), uint32_t(a.G), a.cdf,
                            a.ppf, a.payload, a.payload_size);
+    else if (a.variant == 1)
+        hipLaunchKernelGGL((v0_fwd<NV, 0>), grid, block, 0, s, a.dst, a.vecs, a.vec_stride,
+                           a.rows, a.cols, a.cursors, int(a.min_value), uint32_t(a.G), a.cdf,
+                           a.ppf, a.payload, a.payload_size);
     else
-        hipLaunchKernelGGL((v0_fwd<NV>), grid, block, 0, s, a.dst, a.vecs, a.vec_stride, a.rows,
-                           a.cols, a.cursors, int(a.min_value), uint32_t(a.G), a.cdf, a.ppf,
-                           a.payload, a.payload_size);
+        hipLaunchKernelGGL((v0_fwd<NV, 2>), grid, block, 0, s, a.dst, a.vecs, a.vec_stride,
+                           a.rows, a.cols, a.cursors, int(a.min_value), uint32_t(a.G), a.cdf,
+                           a.ppf, a.payload, a.payload_size);
     return hipGetLastError();
 }
 

@@ -17,7 +17,7 @@ struct V0Args {
     const uint8_t *cdf, *ppf;
     const uint16_t *payload;
     uint32_t payload_size;
-    int variant;             // decode-step variant (0 = reference form, 1 = rotated state)
+    int variant;             // decode kernel variant (0 reference step, 1 rotated, 2 byte tables)
 };
 
 hipError_t launch_v0_decode_matvec(const V0Args &a, hipStream_t s);

@@ -27,6 +27,7 @@ namespace mv {
 constexpr int kRingWords = 64;             // ring words per lane (power of two)
 constexpr uint32_t kRingWaveBytes = kRingWords * 64 * 2;   // 8 KiB per wave
 constexpr uint32_t kOOB = 0x80000000u;     // buffer offset past num_records: reads 0
+constexpr uint32_t kDummyBytes = 8 * 64 * 2;   // 8 rows x 64 lanes of u16 per block
 
 // Fused table entry {f, symbol, q - cdf[s]} for quantile q (reference kernel :120-136;
 // the symbol is (int8)(min_value + s) as in :126).  Reads cdf only inside [0, G].
@@ -64,13 +65,15 @@ struct V0Lane {
     uint4 pa, pb;        // granule loads in flight
     uint32_t ga, gb;     // their granule indices (~0 = nothing to write)
     uint16_t *ring;      // the block's ring array [waves][kRingWords][64]
+    uint16_t *dummy;     // the block's dummy rows [8][64] (written, never read)
     uint32_t lane;
 
     // win_beg/win_end: u16 window [first word, end) that holds every row of the wave
     // (wave-uniform); row_cur: this lane's row start (u16), row_end its end.
     __device__ void init(const uint16_t *payload, uint32_t win_beg, uint32_t win_end,
                          bool valid, uint32_t row_cur, uint32_t row_end, uint32_t slot,
-                         uint16_t *ring_array, uint32_t wave, uint32_t lane_in_wave) {
+                         uint16_t *ring_array, uint16_t *dummy_rows, uint32_t wave,
+                         uint32_t lane_in_wave) {
         rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(payload) + win_beg, 0,
                                                  static_cast<int>((win_end - win_beg) * 2u),
                                                  0x00020000);
@@ -85,6 +88,7 @@ struct V0Lane {
         }
         lane_off = roff + slot * 16u;
         ring = ring_array;
+        dummy = dummy_rows;
         lane = lane_in_wave;
         wave_base = wave * kRingWaveBytes;
         ga = gb = 0xFFFFFFFFu;
@@ -94,16 +98,21 @@ struct V0Lane {
         return ring[(wave_base >> 1) + word_slot * 64 + lane];
     }
 
+    // Writes granule g into its ring slots.  g == ~0 (nothing loaded) writes the zeros of
+    // the out-of-range load to the block's dummy rows instead, so that the write is not
+    // behind a branch: the loads in flight then stay visible to the compiler's counters
+    // and only the granule being written is waited for.
     __device__ __forceinline__ void write_granule(uint4 d, uint32_t g) {
         uint32_t s = (g * 8u) & (kRingWords - 1);
-        ring_at(s + 0) = static_cast<uint16_t>(d.x);
-        ring_at(s + 1) = static_cast<uint16_t>(d.x >> 16);
-        ring_at(s + 2) = static_cast<uint16_t>(d.y);
-        ring_at(s + 3) = static_cast<uint16_t>(d.y >> 16);
-        ring_at(s + 4) = static_cast<uint16_t>(d.z);
-        ring_at(s + 5) = static_cast<uint16_t>(d.z >> 16);
-        ring_at(s + 6) = static_cast<uint16_t>(d.w);
-        ring_at(s + 7) = static_cast<uint16_t>(d.w >> 16);
+        uint16_t *p = g != 0xFFFFFFFFu ? &ring_at(s) : dummy + lane;
+        p[0 * 64] = static_cast<uint16_t>(d.x);
+        p[1 * 64] = static_cast<uint16_t>(d.x >> 16);
+        p[2 * 64] = static_cast<uint16_t>(d.y);
+        p[3 * 64] = static_cast<uint16_t>(d.y >> 16);
+        p[4 * 64] = static_cast<uint16_t>(d.z);
+        p[5 * 64] = static_cast<uint16_t>(d.z >> 16);
+        p[6 * 64] = static_cast<uint16_t>(d.w);
+        p[7 * 64] = static_cast<uint16_t>(d.w >> 16);
     }
 
     // Load granule gl when it will fit into the ring at write time (two groups later;
@@ -125,8 +134,7 @@ struct V0Lane {
         for (uint32_t g = 0; g < kPre; g++)
             pre[g] = buffer_load16(rsrc, g < K ? lane_off + g * 512u : kOOB);
 #pragma unroll
-        for (uint32_t g = 0; g < kPre; g++)
-            if (g < K) write_granule(pre[g], g);
+        for (uint32_t g = 0; g < kPre; g++) write_granule(pre[g], g < K ? g : 0xFFFFFFFFu);
         gl = K < kPre ? K : kPre;
     }
 
@@ -143,11 +151,11 @@ struct V0Lane {
     // Group boundaries (every 16 steps), alternating A and B so that the registers of
     // the loads in flight stay fixed.
     __device__ __forceinline__ void refill_a() {
-        if (ga != 0xFFFFFFFFu) write_granule(pa, ga);
+        write_granule(pa, ga);
         issue(pa, ga);
     }
     __device__ __forceinline__ void refill_b() {
-        if (gb != 0xFFFFFFFFu) write_granule(pb, gb);
+        write_granule(pb, gb);
         issue(pb, gb);
     }
 
@@ -165,8 +173,31 @@ struct V0Lane {
             cc += c ? 1u : 0u;
             (h == 0 ? e0 : e1) = ea;
         }
-        // ring byte offset of word j: wave base | (j % 64) * 128 | lane * 2; the add may
-        // carry into the wave-base bits, the mask restores them
+        advance_word(cc);
+    }
+
+    // Two decode steps with three 256-byte tables f[q], bias[q], sym[q] (at +0, +256,
+    // +512): each table is 64 dwords, one per LDS bank, so every read is conflict-free.
+    __device__ __forceinline__ void pair_u8(const uint8_t *tab, uint32_t &s0, uint32_t &s1) {
+        uint32_t cc = 0;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            uint32_t q = xr >> 24;
+            uint32_t f = tab[q];
+            uint32_t b = tab[256 + q];
+            uint32_t sy = tab[512 + q];
+            uint32_t x1 = __umul24(xr, f) + b;
+            bool c = x1 < 0x10000u;
+            xr = __builtin_amdgcn_perm(x1, W, c ? kSelRenorm : kSelKeep);
+            cc += c ? 1u : 0u;
+            (h == 0 ? s0 : s1) = sy;
+        }
+        advance_word(cc);
+    }
+
+    // ring byte offset of word j: wave base | (j % 64) * 128 | lane * 2; the add may carry
+    // into the wave-base bits, the mask restores them
+    __device__ __forceinline__ void advance_word(uint32_t cc) {
         uint32_t t = waddr + (cc ? 128u : 0u);
         waddr = (t & (kRingWaveBytes - 1)) | (wave_base & ~(kRingWaveBytes - 1));
         W = *reinterpret_cast<const uint16_t *>(reinterpret_cast<const char *>(ring) + waddr);

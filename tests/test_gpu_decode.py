@@ -121,6 +121,26 @@ def test_llm_shapes(cuda, rows, cols, kind, p):
     assert np.array_equal(got[0], O.c_matvec(oracle_rec(cm), vecs[0], 16))
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_decode_variants_agree(cuda, golden_dir, variant):
+    """Every decode kernel variant gives the oracle's bits (golden file + a ragged shape)."""
+    note(f"decode variant {variant}")
+    lib = mv.lib()
+    assert lib.matvans_set_decode_variant(variant) == 0
+    try:
+        hdr, vec, mats = mv.read_bin(os.path.join(golden_dir, FIXTURES[1]))
+        cm = mats[0].to_device()
+        v = vec[:cm.cols]
+        assert np.array_equal(gpu_matvec(cm, v[None])[0], O.c_matvec(oracle_rec(cm), v))
+        m, _ = laplace_int8(77, 3072, 4.0, seed=variant)
+        cm = mv.CompressedMatrix.encode(m).to_device()
+        vecs = np.stack([uniform_vec(3072, seed=i) for i in range(5)])
+        want = (m.astype(np.int64) @ vecs.astype(np.int64).T).T.astype(np.int32)
+        assert np.array_equal(gpu_matvec(cm, vecs), want)
+    finally:
+        lib.matvans_set_decode_variant(-1)
+
+
 def test_reference_signature_entry(cuda):
     import torch
     note("decmpressAndMultiply argument list")
