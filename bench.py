@@ -159,6 +159,19 @@ def main():
     ms_fused = time_launches(fused, n_copies, args.steps, args.warmup)
 
     res = {}
+    if not args.no_compare:
+        # backward dx = W^T (s_w * g) from the same compressed copies (one vector)
+        g = torch.from_numpy(np.random.default_rng(5).standard_normal((nvec, rows))
+                             .astype(np.float32)).to(dev)
+        s_w = torch.full((rows,), 0.01, dtype=torch.float32, device=dev)
+        alpha = torch.ones(nvec, dtype=torch.float32, device=dev)
+        ws = torch.empty(int(mv.lib().matvans_backward_workspace_size(rows, cols, nvec)) // 4,
+                         dtype=torch.float32, device=dev)
+        dx = torch.empty((nvec, cols), dtype=torch.float32, device=dev)
+        res["bwd_us"] = 1e3 * time_launches(
+            lambda i: copies[i].backward(g, s_w, alpha, out=dx, workspace=ws),
+            n_copies, args.steps, args.warmup)
+        del ws
     if not args.no_compare or args.only in ("int8", "f32"):
         d_w8 = [torch.from_numpy(w).to(dev) for _ in range(max(2, -(-2 * MALL_BYTES // w.nbytes)))]
         o8 = torch.empty((nvec, rows), dtype=torch.int32, device=dev)
@@ -206,6 +219,8 @@ def main():
         "speedup_vs_int8": round(res["int8_us"] / us, 4) if "int8_us" in res else None,
         "int8_us": round(res["int8_us"], 3) if "int8_us" in res else None,
         "f32_us": round(res["f32_us"], 3) if "f32_us" in res else None,
+        "bwd_us": round(res["bwd_us"], 3) if "bwd_us" in res else None,
+        "fwd_plus_bwd_over_fwd": round((us + res["bwd_us"]) / us, 3) if "bwd_us" in res else None,
         "compressed_GBps": round(achieved, 1),
         "uncompressed_equiv_GBps": round(rows * cols * nvec / (us * 1e-6) / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,

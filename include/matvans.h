@@ -51,7 +51,7 @@ int matvans_version(void);
  *   interleaved rANS streams of each row (payload in u16 words, row r at cursors[r]).
  * grid_spacing is accepted and unused, as in the reference.  payload_size bounds every
  * read (the reference ignores it and may read past a row's end).
- * Preconditions: cols % 512 == 0; payload 16-byte aligned; vec 4-byte aligned.
+ * Preconditions: cols % 512 == 0; payload and vec 16-byte aligned.
  * ----------------------------------------------------------------------------*/
 int matvans_decompress_and_multiply(int32_t *dst, const int8_t *vec, uint32_t rows,
                                     uint32_t cols, float grid_spacing,
@@ -135,6 +135,28 @@ int matvans_int8_matvec(int32_t *dst, const int8_t *mat, const int8_t *vecs, uin
                         uint32_t vec_stride, uint32_t rows, uint32_t cols, void *stream);
 int matvans_f32_gemv(float *y, const float *mat, const float *x, uint32_t rows, uint32_t cols,
                      float alpha, void *stream);
+
+/* ------------------------------------------------------------------------------
+ * N3. Backward from the compressed streams (no reference counterpart):
+ *     dx[v*cols + c] = alpha[v] * sum_r W[r,c] * s_w[r] * g[v*g_stride + r]
+ * re-decoding the same row-ordered streams as the forward (no dense W is built).  The
+ * 64-row partial sums go to the caller's workspace (matvans_backward_workspace_size
+ * bytes, 16-byte aligned) and are reduced in a fixed order: results are bitwise
+ * reproducible.  1 <= n_vec <= 8, cols % 512 == 0.
+ * ----------------------------------------------------------------------------*/
+size_t matvans_backward_workspace_size(uint32_t rows, uint32_t cols, uint32_t n_vec);
+int matvans_decompress_backward(float *dx, const float *g, uint32_t n_vec, uint32_t g_stride,
+                                const float *s_w, const float *alpha, uint32_t rows,
+                                uint32_t cols, const uint32_t *cursors, int8_t min_value,
+                                uint8_t G, const uint8_t *cdf_data, const uint8_t *ppf_data,
+                                const uint16_t *payload, uint32_t payload_size,
+                                float *workspace, size_t workspace_bytes, void *stream);
+
+/* Activation quantiser of the compressed layer (no reference counterpart; the reference
+ * requantises the int32 result instead, A7): per vector s_x = max|x|/127 (1 for x == 0)
+ * and xq = clamp(rint(x / s_x), -127, 127), fp32 IEEE division. */
+int matvans_quantize_vectors(const float *x, uint32_t n_vec, uint32_t x_stride, uint32_t cols,
+                             int8_t *xq, uint32_t xq_stride, float *s_x, void *stream);
 
 /* ------------------------------------------------------------------------------
  * A10. FxHash of an int8 array (matvansWarpPerRow.cu:31-42,

@@ -24,7 +24,8 @@ from .capi import DevMatrix, HostMatrix, MatvansError, call, lib
 
 __all__ = ["CompressedMatrix", "MatvansError", "decmpressAndMultiply", "decompress_and_multiply",
            "int8_matvec", "f32_gemv", "absmax_requant", "scale_output", "hash_int8_array",
-           "build_model", "encode_rows", "read_bin", "write_bin", "run_chain", "lib"]
+           "build_model", "encode_rows", "read_bin", "write_bin", "run_chain", "lib",
+           "quantize_vectors", "compressed_linear"]
 
 
 def _torch():
@@ -166,6 +167,32 @@ class CompressedMatrix:
              self.payload_size, _stream(stream))
         return out[0] if squeeze else out
 
+    def backward(self, g, s_w, alpha=None, out=None, workspace=None, stream=None):
+        """dx[v] = alpha[v] * W^T (s_w * g[v]) re-decoded from the compressed streams,
+        g [rows] or [n_vec, rows] fp32 (n_vec <= 8); bitwise reproducible."""
+        torch = _torch()
+        if self.d_payload is None:
+            raise RuntimeError("matrix is not on the device; call to_device() first")
+        squeeze = g.dim() == 1
+        gg = (g.unsqueeze(0) if squeeze else g).contiguous()
+        if gg.dtype != torch.float32 or gg.shape[-1] != self.rows:
+            raise ValueError(f"expected fp32 gradients of length {self.rows}")
+        n = gg.shape[0]
+        dev = gg.device
+        if alpha is None:
+            alpha = torch.ones(n, dtype=torch.float32, device=dev)
+        need = int(lib().matvans_backward_workspace_size(self.rows, self.cols, n))
+        if workspace is None or workspace.numel() * workspace.element_size() < need:
+            workspace = torch.empty(max(1, need // 4), dtype=torch.float32, device=dev)
+        if out is None:
+            out = torch.empty((n, self.cols), dtype=torch.float32, device=dev)
+        call("matvans_decompress_backward", _ptr(out), _ptr(gg), n, gg.stride(0), _ptr(s_w),
+             _ptr(alpha), self.rows, self.cols, _ptr(self.d_cursors), self.min_value, self.G,
+             _ptr(self.d_cdf_data), _ptr(self.d_ppf_data), _ptr(self.d_payload),
+             self.payload_size, _ptr(workspace), workspace.numel() * workspace.element_size(),
+             _stream(stream))
+        return out[0] if squeeze else out
+
     def decompress_and_mult(self, d_result8, d_result32, vector, d_delta, stream=None):
         """Stream-ordered composite: decode-matvec -> absmax -> requant; delta stays on
         the device (matvansWarpPerRow.cu:150-176)."""
@@ -299,6 +326,64 @@ def scale_output(acc, s_w, s_x, out=None, stream=None):
     call("matvans_scale_output", _ptr(a), rows, n_vec, _ptr(s_w), _ptr(s_x), _ptr(out),
          _stream(stream))
     return out if acc.dim() == 2 else out[0]
+
+
+def quantize_vectors(x, stream=None):
+    """Per-vector activation quantiser: (int8 xq [n, cols], fp32 s_x [n]) with
+    s_x = max|x| / 127 and xq = clamp(rint(x / s_x), -127, 127)."""
+    torch = _torch()
+    xx = (x.unsqueeze(0) if x.dim() == 1 else x).contiguous()
+    n, cols = xx.shape
+    xq = torch.empty((n, cols), dtype=torch.int8, device=xx.device)
+    s_x = torch.empty(n, dtype=torch.float32, device=xx.device)
+    call("matvans_quantize_vectors", _ptr(xx), n, xx.stride(0), cols, _ptr(xq), xq.stride(0),
+         _ptr(s_x), _stream(stream))
+    return xq, s_x
+
+
+def _compressed_linear_function():
+    torch = _torch()
+
+    class CompressedLinearFunction(torch.autograd.Function):
+        """y = s_x * s_w * (Wq @ quantize(x)) with W frozen in its compressed form.
+
+        Backward uses the straight-through estimator for the activation quantiser
+        (d quantize(x) / dx = 1 / s_x, s_x treated as a constant), so
+        dL/dx = Wq^T (s_w * dL/dy), computed from the compressed streams."""
+
+        @staticmethod
+        def forward(ctx, x, matrix, s_w):
+            shape = x.shape
+            xx = x.reshape(-1, matrix.cols).float().contiguous()
+            if xx.shape[0] > 8:
+                raise ValueError("at most 8 vectors per call")
+            xq, s_x = quantize_vectors(xx)
+            acc = matrix.multiply(xq)
+            y = scale_output(acc, s_w, s_x)
+            ctx.matrix = matrix
+            ctx.in_shape = shape
+            ctx.save_for_backward(s_w)
+            return y.reshape(*shape[:-1], matrix.rows)
+
+        @staticmethod
+        def backward(ctx, gy):
+            (s_w,) = ctx.saved_tensors
+            g = gy.reshape(-1, ctx.matrix.rows).float().contiguous()
+            dx = ctx.matrix.backward(g, s_w)
+            return dx.reshape(ctx.in_shape), None, None
+
+    return CompressedLinearFunction
+
+
+_CLF = None
+
+
+def compressed_linear(x, matrix, s_w):
+    """Autograd-aware compressed linear layer (see CompressedLinearFunction)."""
+    global _CLF
+    if _CLF is None:
+        _CLF = _compressed_linear_function()
+    return _CLF.apply(x, matrix, s_w)
 
 
 def run_chain(matrices, vector, stream=None):

@@ -52,6 +52,10 @@ SIGNATURES = {
     "matvans_scale_output": (i32, [P, u32, u32, P, P, P, P]),
     "matvans_int8_matvec": (i32, [P, P, P, u32, u32, u32, u32, P]),
     "matvans_f32_gemv": (i32, [P, P, P, u32, u32, f32, P]),
+    "matvans_backward_workspace_size": (ctypes.c_size_t, [u32, u32, u32]),
+    "matvans_decompress_backward": (i32, [P, P, u32, u32, P, P, u32, u32, P, i8, u8, P, P, P,
+                                          u32, P, ctypes.c_size_t, P]),
+    "matvans_quantize_vectors": (i32, [P, u32, u32, u32, P, u32, P, P]),
     "matvans_fxhash": (u32, [P, u64]),
     "matvans_bin_read": (i32, [ctypes.c_char_p, ctypes.POINTER(P)]),
     "matvans_bin_free": (None, [P]),

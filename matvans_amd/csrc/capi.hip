@@ -149,3 +149,45 @@ extern "C" int matvans_f32_gemv(float *y, const float *mat, const float *x, uint
     return hip_status(mv::launch_f32_gemv(y, mat, x, rows, cols, alpha, as_stream(stream)),
                       "f32_gemv launch");
 }
+
+extern "C" size_t matvans_backward_workspace_size(uint32_t rows, uint32_t cols,
+                                                  uint32_t n_vec) {
+    return mv::v0_bwd_workspace_bytes(rows, cols, n_vec);
+}
+
+extern "C" int matvans_decompress_backward(float *dx, const float *g, uint32_t n_vec,
+                                           uint32_t g_stride, const float *s_w,
+                                           const float *alpha, uint32_t rows, uint32_t cols,
+                                           const uint32_t *cursors, int8_t min_value, uint8_t G,
+                                           const uint8_t *cdf_data, const uint8_t *ppf_data,
+                                           const uint16_t *payload, uint32_t payload_size,
+                                           float *workspace, size_t workspace_bytes,
+                                           void *stream) {
+    if (!dx || !g || !s_w || !alpha || !cursors || !cdf_data || !ppf_data || !payload ||
+        !workspace)
+        return fail(MATVANS_E_INVALID, "null device pointer");
+    if (n_vec < 1 || n_vec > 8) return fail(MATVANS_E_INVALID, "n_vec %u outside 1..8", n_vec);
+    if (rows == 0 || cols == 0 || cols % 512)
+        return fail(MATVANS_E_SHAPE, "shape %u x %u (cols must be a multiple of 512)", rows, cols);
+    if (n_vec > 1 && g_stride < rows) return fail(MATVANS_E_INVALID, "g_stride %u < rows", g_stride);
+    if (!aligned(payload, 16) || !aligned(workspace, 16) || !aligned(dx, 4))
+        return fail(MATVANS_E_ALIGN, "payload and workspace must be 16-byte aligned");
+    size_t need = mv::v0_bwd_workspace_bytes(rows, cols, n_vec);
+    if (workspace_bytes < need)
+        return fail(MATVANS_E_CAPACITY, "workspace %zu bytes < %zu", workspace_bytes, need);
+    mv::V0BwdArgs a{dx, workspace, g, n_vec, n_vec > 1 ? g_stride : rows, s_w, alpha, rows,
+                    cols, cursors, min_value, G, cdf_data, ppf_data, payload, payload_size};
+    return hip_status(mv::launch_v0_backward(a, as_stream(stream)), "backward launch");
+}
+
+extern "C" int matvans_quantize_vectors(const float *x, uint32_t n_vec, uint32_t x_stride,
+                                        uint32_t cols, int8_t *xq, uint32_t xq_stride,
+                                        float *s_x, void *stream) {
+    if (!x || !xq || !s_x) return fail(MATVANS_E_INVALID, "null device pointer");
+    if (n_vec < 1 || cols == 0) return fail(MATVANS_E_SHAPE, "empty input");
+    if (x_stride < cols || xq_stride < cols)
+        return fail(MATVANS_E_INVALID, "strides must be >= cols");
+    return hip_status(mv::launch_quantize_vectors(x, n_vec, x_stride, cols, xq, xq_stride, s_x,
+                                                  as_stream(stream)),
+                      "quantize launch");
+}

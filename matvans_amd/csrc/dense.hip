@@ -193,6 +193,34 @@ __global__ void scale_output_kernel(const int32_t *__restrict__ acc, uint32_t ro
     y[i] = (s_x[v] * s_w[r]) * static_cast<float>(acc[i]);
 }
 
+// One block per vector: s_x = max|x| / 127 (1 when x == 0), xq = clamp(rint(x / s_x)).
+__global__ __launch_bounds__(1024) void quantize_vectors_kernel(const float *__restrict__ x,
+                                                                uint32_t x_stride, uint32_t cols,
+                                                                int8_t *__restrict__ xq,
+                                                                uint32_t xq_stride,
+                                                                float *__restrict__ s_x) {
+    __shared__ float red[16];
+    const float *xv = x + size_t(blockIdx.x) * x_stride;
+    float m = 0.0f;
+    for (uint32_t i = threadIdx.x; i < cols; i += blockDim.x) m = fmaxf(m, fabsf(xv[i]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        float t = threadIdx.x < (blockDim.x >> 6) ? red[threadIdx.x] : 0.0f;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) t = fmaxf(t, __shfl_xor(t, o, 64));
+        if (threadIdx.x == 0) red[0] = t;
+    }
+    __syncthreads();
+    const float sx = red[0] > 0.0f ? red[0] / 127.0f : 1.0f;
+    if (threadIdx.x == 0) s_x[blockIdx.x] = sx;
+    int8_t *q = xq + size_t(blockIdx.x) * xq_stride;
+    for (uint32_t i = threadIdx.x; i < cols; i += blockDim.x)
+        q[i] = static_cast<int8_t>(fminf(fmaxf(rintf(xv[i] / sx), -127.0f), 127.0f));
+}
+
 template <int NV>
 hipError_t int8_nv(int32_t *dst, const int8_t *mat, const int8_t *vecs, uint32_t vec_stride,
                    uint32_t rows, uint32_t cols, hipStream_t s) {
@@ -242,6 +270,14 @@ hipError_t launch_absmax_requant(const int32_t *acc, uint32_t n, int8_t *out8, f
                                  int clamp, hipStream_t s) {
     hipLaunchKernelGGL(absmax_requant_kernel, dim3(1), dim3(1024), 0, s, acc, n, out8, delta,
                        clamp);
+    return hipGetLastError();
+}
+
+hipError_t launch_quantize_vectors(const float *x, uint32_t n_vec, uint32_t x_stride,
+                                   uint32_t cols, int8_t *xq, uint32_t xq_stride, float *s_x,
+                                   hipStream_t s) {
+    hipLaunchKernelGGL(quantize_vectors_kernel, dim3(n_vec), dim3(1024), 0, s, x, x_stride, cols,
+                       xq, xq_stride, s_x);
     return hipGetLastError();
 }
 

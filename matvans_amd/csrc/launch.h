@@ -22,6 +22,28 @@ struct V0Args {
 
 hipError_t launch_v0_decode_matvec(const V0Args &a, hipStream_t s);
 
+struct V0BwdArgs {
+    float *dx;               // [n_vec][cols]
+    float *partial;          // workspace [n_vec][ceil(rows/64)][cols]
+    const float *g;          // [n_vec][g_stride]
+    uint32_t n_vec, g_stride;
+    const float *s_w;        // [rows]
+    const float *alpha;      // [n_vec]
+    uint32_t rows, cols;
+    const uint32_t *cursors;
+    int8_t min_value;
+    uint8_t G;
+    const uint8_t *cdf, *ppf;
+    const uint16_t *payload;
+    uint32_t payload_size;
+};
+
+size_t v0_bwd_workspace_bytes(uint32_t rows, uint32_t cols, uint32_t n_vec);
+hipError_t launch_v0_backward(const V0BwdArgs &a, hipStream_t s);
+hipError_t launch_quantize_vectors(const float *x, uint32_t n_vec, uint32_t x_stride,
+                                   uint32_t cols, int8_t *xq, uint32_t xq_stride, float *s_x,
+                                   hipStream_t s);
+
 hipError_t launch_int8_matvec(int32_t *dst, const int8_t *mat, const int8_t *vecs,
                               uint32_t n_vec, uint32_t vec_stride, uint32_t rows, uint32_t cols,
                               hipStream_t s);
