@@ -41,7 +41,26 @@ def parse():
     ap.add_argument("--no-compare", action="store_true", help="skip the comparators")
     ap.add_argument("--only", default=None, choices=[None, "fused", "int8", "f32", "copy"],
                     help="run one kernel only (for profiling); prints its timing")
+    ap.add_argument("--formats", default=",".join(FORMATS),
+                    help="comma list of encodings to time; the fastest is the headline")
     return ap.parse_args()
+
+
+# encodings: (label, streams, precision, table_mode); "v0" is the reference .bin layout
+FORMATS = {
+    "v0": None,
+    "s64-p8": (64, 8, 0),
+    "s64-p12": (64, 12, 0),
+    "s64-p8-rowcdf": (64, 8, 1),
+    "s32-p12": (32, 12, 0),
+}
+FORMAT_TEXT = {
+    "v0": ".bin v0 (reference layout): 32 streams/row, global 8-bit CDF",
+    "s64-p8": "MVX1: 64 streams/row, global 8-bit CDF",
+    "s64-p12": "MVX1: 64 streams/row, global 12-bit CDF",
+    "s64-p8-rowcdf": "MVX1: 64 streams/row, per-row 8-bit CDF",
+    "s32-p12": "MVX1: 32 streams/row, global 12-bit CDF",
+}
 
 
 def gen(rows, cols, dist, seed=20250310):
@@ -127,36 +146,60 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"kernel": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n}))
         return
+    from matvans_amd.mvx import CompressedMatrixV1
     rows, cols, nvec = args.rows, args.cols, args.nvec
     w, x = gen(rows, cols, args.dist)
     cm = mv.CompressedMatrix.encode(w, n_threads=0)
     vecs = np.stack([np.roll(x, i) for i in range(nvec)])
     d_vecs = torch.from_numpy(vecs).to(dev)
-
-    # distinct device copies of the compressed matrix
-    comp_bytes = cm.payload_size * 2 + rows * 4 + 257 + 256
-    n_copies = max(2, -(-2 * MALL_BYTES // comp_bytes))
-    n_copies = min(n_copies, 64)
-    copies = [mv.CompressedMatrix(cm.rows, cm.cols, 1.0, cm.cursors, cm.min_value, cm.G,
-                                  cm.cdf_data, cm.ppf_data, cm.payload).to_device(dev)
-              for _ in range(n_copies)]
     out = torch.empty((nvec, rows), dtype=torch.int32, device=dev)
-
-    # correctness first: bit-exact int32 against the brute-force integer product
-    got = copies[0].multiply(d_vecs, out=out).cpu().numpy()
     want = (w.astype(np.int64) @ vecs.astype(np.int64).T).T.astype(np.int32)
-    if not np.array_equal(got, want):
-        print("fused decode-matvec result differs from the integer product", file=sys.stderr)
-        sys.exit(1)
 
-    def fused(i):
-        copies[i].multiply(d_vecs, out=out)
+    def encode(fmt):
+        """(host matrix, stored bytes, maker of one device copy)"""
+        if FORMATS[fmt] is None:
+            return cm, cm.payload_size * 2 + rows * 4 + (cm.G + 1) + 256, lambda: mv.CompressedMatrix(
+                cm.rows, cm.cols, 1.0, cm.cursors, cm.min_value, cm.G, cm.cdf_data, cm.ppf_data,
+                cm.payload).to_device(dev)
+        S, P, mode = FORMATS[fmt]
+        c = CompressedMatrixV1.encode(w, S, P, mode)
+        return c, c.stored_bytes, lambda: CompressedMatrixV1(
+            c.rows, c.cols, S, P, mode, c.min_value, c.G, c.cursors, c.payload, c.cdf,
+            c.freq).to_device(dev)
 
+    def time_format(fmt):
+        host, stored, make = encode(fmt)
+        # distinct device copies (more than twice the MALL) so repeats come from HBM
+        n_copies = min(64, max(2, -(-2 * MALL_BYTES // stored)))
+        copies = [make() for _ in range(n_copies)]
+        # correctness first: bit-exact int32 against the brute-force integer product
+        got = copies[0].multiply(d_vecs, out=out).cpu().numpy()
+        if not np.array_equal(got, want):
+            print(f"{fmt}: fused decode-matvec differs from the integer product", file=sys.stderr)
+            sys.exit(1)
+        ms = time_launches(lambda i: copies[i].multiply(d_vecs, out=out), n_copies, args.steps,
+                           args.warmup)
+        alg = stored + nvec * cols + 4 * rows * nvec
+        del copies
+        return {"us": round(ms * 1e3, 3), "bits_per_entry": round(8 * stored / (rows * cols), 4),
+                "alg_bytes": alg, "GBps": round(alg / (ms * 1e-3) / 1e9, 1),
+                "copies": n_copies}
+
+    fmts = [f for f in args.formats.split(",") if f]
+    for f in fmts:
+        if f not in FORMATS:
+            print(f"unknown format {f}; choose from {list(FORMATS)}", file=sys.stderr)
+            sys.exit(2)
     if args.only == "fused":
-        ms = time_launches(fused, n_copies, args.steps, args.warmup)
-        print(json.dumps({"kernel": "fused", "ms": ms}))
+        r = time_format(fmts[0])
+        print(json.dumps({"kernel": "fused", "format": fmts[0], "ms": r["us"] / 1e3}))
         return
-    ms_fused = time_launches(fused, n_copies, args.steps, args.warmup)
+    per_format = {f: time_format(f) for f in fmts}
+    best = min(per_format, key=lambda f: per_format[f]["us"])
+    ms_fused = per_format[best]["us"] / 1e3
+    n_copies = per_format[best]["copies"]
+    copies = [encode("v0")[2]() for _ in range(min(64, max(2, -(-2 * MALL_BYTES // (
+        cm.payload_size * 2)))))]
 
     res = {}
     if not args.no_compare:
@@ -170,7 +213,7 @@ def main():
         dx = torch.empty((nvec, cols), dtype=torch.float32, device=dev)
         res["bwd_us"] = 1e3 * time_launches(
             lambda i: copies[i].backward(g, s_w, alpha, out=dx, workspace=ws),
-            n_copies, args.steps, args.warmup)
+            len(copies), args.steps, args.warmup)
         del ws
     if not args.no_compare or args.only in ("int8", "f32"):
         d_w8 = [torch.from_numpy(w).to(dev) for _ in range(max(2, -(-2 * MALL_BYTES // w.nbytes)))]
@@ -191,11 +234,10 @@ def main():
         del d_wf, wf
 
     us = ms_fused * 1e3
-    alg_bytes = (2 * cm.payload_size + 4 * rows + (cm.G + 1) + 256 + nvec * cols +
-                 4 * rows * nvec)
+    alg_bytes = per_format[best]["alg_bytes"]
     achieved = alg_bytes / (us * 1e-6) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_fused_{rows}x{cols}_{args.dist}.json")
+    tpath = os.path.join(ROOT, "profiles", f"traffic_fused_{best}_{rows}x{cols}_{args.dist}.json")
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("bytes_per_launch")
     line = {
@@ -212,10 +254,11 @@ def main():
         "dtype": "int8 (int32 accumulate)",
         "data": "synthetic (seeded Laplace(0,4) int8 matrix, N(0,30^2) int8 vector)",
         "config": {"workload": f"fused rANS decode x int8 matvec, {rows}x{cols}, "
-                               f"{args.dist}, .bin v0 global CDF, 32 streams/row",
-                   "rows": rows, "cols": cols, "n_vec": nvec,
-                   "bits_per_entry": round(16 * cm.payload_size / (rows * cols), 4),
+                               f"{args.dist}, {FORMAT_TEXT[best]}",
+                   "format": best, "rows": rows, "cols": cols, "n_vec": nvec,
+                   "bits_per_entry": per_format[best]["bits_per_entry"],
                    "distinct_copies": n_copies},
+        "formats": per_format,
         "speedup_vs_int8": round(res["int8_us"] / us, 4) if "int8_us" in res else None,
         "int8_us": round(res["int8_us"], 3) if "int8_us" in res else None,
         "f32_us": round(res["f32_us"], 3) if "f32_us" in res else None,

@@ -212,6 +212,87 @@ int matvans_encode_rows(const int8_t *mat, uint32_t rows, uint32_t cols, int8_t 
                         uint32_t *cursors, uint16_t *payload, uint64_t payload_cap,
                         uint64_t *payload_size);
 
+/* ------------------------------------------------------------------------------
+ * N2. MVX1: format extensions of the WarpPerRow codec, written by this library's encoder.
+ * The reference has one variant (32 streams per row, 8-bit probabilities, one table per
+ * matrix: create_matrices_WarpPerRow.py:131-272,353-430); its per-matrix constant-memory
+ * CDF variant is matvansWarpPerRow_local_cdf.cu:18,196-363.  MVX1 adds
+ *   streams    32 or 64 interleaved streams per row.  Stream l decodes columns
+ *              16*S*(k/16) + 16*l + k%16.  A row is K granules of 512 bytes in which each
+ *              stream owns 256/S consecutive u16 words (8 for S = 32: the reference
+ *              layout; 4 for S = 64): word j of stream l is at
+ *              cursor + 256*(j/(256/S)) + (256/S)*l + j%(256/S).
+ *   precision  8 or 12 bit probabilities (decode q = x & (2^P-1);
+ *              x = (x >> P)*f + q - cdf[s]; renormalise with a 16-bit word below 2^16).
+ *   table_mode 0: one table per matrix, u16 cdf[G+1] (cdf[G] = 2^P);
+ *              1: one 8-bit table per row (P = 8 only), u8 freq[rows][G] summing to 256.
+ * S = 32, P = 8, table_mode 0 encodes exactly as the reference generator.
+ * ----------------------------------------------------------------------------*/
+typedef struct {
+    uint32_t rows, cols;
+    float grid_spacing;
+    uint32_t streams, precision, table_mode;
+    int32_t min_value;          /* symbols min_value .. min_value + G - 1 */
+    uint32_t G;
+    uint32_t payload_size;      /* u16 words, a multiple of 256 */
+    const uint32_t *cursors;    /* [rows] */
+    const uint16_t *cdf;        /* table_mode 0: [G+1] */
+    const uint8_t *freq;        /* table_mode 1: [rows][G] */
+    const uint16_t *payload;    /* [payload_size] */
+} matvans_v1_host;
+
+/* Model for the given precision and table mode.  table_mode 0 writes cdf[G+1] (room for
+ * 257 entries); table_mode 1 writes freq[rows][G] (room for rows*256 bytes).  Equal
+ * counts are ordered by descending symbol, as matvans_build_model. */
+int matvans_v1_model(const int8_t *mat, uint32_t rows, uint32_t cols, uint32_t precision,
+                     uint32_t table_mode, int32_t *min_value, uint32_t *G, uint16_t *cdf,
+                     uint8_t *freq);
+uint64_t matvans_v1_encode_bound(uint32_t rows, uint32_t cols, uint32_t streams,
+                                 uint32_t precision);
+/* Encode rows x cols (cols % (16*streams) == 0), threaded over rows. */
+int matvans_v1_encode(const int8_t *mat, uint32_t rows, uint32_t cols, uint32_t streams,
+                      uint32_t precision, uint32_t table_mode, int32_t min_value, uint32_t G,
+                      const uint16_t *cdf, const uint8_t *freq, uint32_t n_threads,
+                      uint32_t *cursors, uint16_t *payload, uint64_t payload_cap,
+                      uint64_t *payload_size);
+/* Decode table of a table_mode 0 matrix, 2 * 2^P entries: A[q] = f | (uint8)(min+s) << 24,
+ * then B[q] = q - cdf[s]. */
+int matvans_v1_lut(const uint16_t *cdf, uint32_t G, int32_t min_value, uint32_t precision,
+                   uint32_t *lut);
+
+typedef struct matvans_v1_file matvans_v1_file;
+/* MVX1 container: "MVX1", u32 version 1, u32 num_matrices, u32 result_hash, u32 len_v,
+ * i8 vector[len_v] (zero pad to 4 bytes), then per matrix the fields of matvans_v1_host
+ * in order (rows, cols, grid_spacing, streams, precision, table_mode, min_value, G,
+ * payload_size as 4-byte words), cursors, the table (zero pad to 4), the payload (zero
+ * pad to 4).  The reader validates lengths, parameters, tables and cursors. */
+int matvans_v1_read(const char *path, matvans_v1_file **out);
+void matvans_v1_free(matvans_v1_file *f);
+int matvans_v1_header(const matvans_v1_file *f, uint32_t *num_matrices, uint32_t *result_hash,
+                      const int8_t **vector, uint32_t *len_v);
+int matvans_v1_matrix(const matvans_v1_file *f, uint32_t k, matvans_v1_host *out);
+int matvans_v1_write(const char *path, uint32_t result_hash, const int8_t *vector,
+                     uint32_t len_v, uint32_t num_matrices, const matvans_v1_host *mats);
+
+/* Device view of an MVX1 matrix (all pointers device memory). */
+typedef struct {
+    uint32_t rows, cols, streams, precision, table_mode;
+    int32_t min_value;
+    uint32_t G;
+    uint32_t payload_size;      /* u16 words */
+    const uint32_t *cursors;    /* [rows] */
+    const uint32_t *lut;        /* table_mode 0: [2 * 2^P] from matvans_v1_lut */
+    const uint8_t *freq;        /* table_mode 1: [rows][G] */
+    const uint16_t *payload;    /* 16-byte aligned */
+} matvans_v1_dev_matrix;
+
+/* dst[v][r] = sum_c W[r,c] * vecs[v][c] for v < n_vec (1..8), W decoded from an MVX1
+ * matrix (int32, bit-exact).  vecs: n_vec vectors vec_stride bytes apart, 16-byte
+ * aligned.  The fused decode-matvec of A1 for the extended formats. */
+int matvans_v1_decompress_and_multiply(const matvans_v1_dev_matrix *m, int32_t *dst,
+                                       const int8_t *vecs, uint32_t n_vec,
+                                       uint32_t vec_stride, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -91,6 +91,37 @@ extern "C" int matvans_decompress_and_multiply_nvec(int32_t *dst, const int8_t *
     return hip_status(mv::launch_v0_decode_matvec(a, as_stream(stream)), "decode_matvec launch");
 }
 
+extern "C" int matvans_v1_decompress_and_multiply(const matvans_v1_dev_matrix *m, int32_t *dst,
+                                                  const int8_t *vecs, uint32_t n_vec,
+                                                  uint32_t vec_stride, void *stream) {
+    if (!m || !dst || !vecs || !m->cursors || !m->payload)
+        return fail(MATVANS_E_INVALID, "null argument or device pointer");
+    if (n_vec < 1 || n_vec > 8) return fail(MATVANS_E_INVALID, "n_vec %u outside 1..8", n_vec);
+    const bool s_ok = m->streams == 32 || m->streams == 64;
+    const bool p_ok = m->precision == 8 || m->precision == 12;
+    if (!s_ok || !p_ok || m->table_mode > 1)
+        return fail(MATVANS_E_INVALID, "streams %u, precision %u, table mode %u", m->streams,
+                    m->precision, m->table_mode);
+    if (m->table_mode == 1 && (m->streams != 64 || m->precision != 8))
+        return fail(MATVANS_E_UNSUPPORTED, "one table per row needs 64 streams and 8-bit tables");
+    if (m->table_mode == 0 && !m->lut) return fail(MATVANS_E_INVALID, "null decode table");
+    if (m->table_mode == 1 && (!m->freq || m->G < 2 || m->G > 256))
+        return fail(MATVANS_E_INVALID, "per-row tables need freq and 2 <= G <= 256");
+    if (m->rows == 0) return fail(MATVANS_E_SHAPE, "rows == 0");
+    if (m->cols == 0 || m->cols % (16 * m->streams))
+        return fail(MATVANS_E_SHAPE, "cols %u is not a multiple of %u", m->cols, 16 * m->streams);
+    if (!aligned(m->payload, 16) || (m->lut && !aligned(m->lut, 16)))
+        return fail(MATVANS_E_ALIGN, "payload and decode table must be 16-byte aligned");
+    if (n_vec == 1 && vec_stride < m->cols) vec_stride = m->cols;
+    if (!aligned(vecs, 16) || vec_stride % 16)
+        return fail(MATVANS_E_ALIGN, "vectors must be 16-byte aligned (stride %u)", vec_stride);
+    if (vec_stride < m->cols) return fail(MATVANS_E_INVALID, "vec_stride %u < cols %u", vec_stride, m->cols);
+    mv::V1Args a{dst, vecs, n_vec, vec_stride, m->rows, m->cols, m->streams, m->precision,
+                 m->table_mode, m->min_value, m->G, m->cursors, m->lut, m->freq, m->payload,
+                 m->payload_size};
+    return hip_status(mv::launch_v1_decode_matvec(a, as_stream(stream)), "v1 decode_matvec launch");
+}
+
 extern "C" int matvans_decompress_and_mult(const matvans_dev_matrix *m, int8_t *d_result8,
                                            int32_t *d_result32, const int8_t *d_vector,
                                            float *d_delta, void *stream) {

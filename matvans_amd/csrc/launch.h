@@ -22,6 +22,22 @@ struct V0Args {
 
 hipError_t launch_v0_decode_matvec(const V0Args &a, hipStream_t s);
 
+struct V1Args {
+    int32_t *dst;            // [n_vec][rows]
+    const int8_t *vecs;      // [n_vec][vec_stride]
+    uint32_t n_vec, vec_stride;
+    uint32_t rows, cols, streams, precision, table_mode;
+    int32_t min_value;
+    uint32_t G;
+    const uint32_t *cursors;
+    const uint32_t *lut;     // table_mode 0: 2 * 2^P dwords
+    const uint8_t *freq;     // table_mode 1: [rows][G]
+    const uint16_t *payload;
+    uint32_t payload_size;
+};
+
+hipError_t launch_v1_decode_matvec(const V1Args &a, hipStream_t s);
+
 struct V0BwdArgs {
     float *dx;               // [n_vec][cols]
     float *partial;          // workspace [n_vec][ceil(rows/64)][cols]

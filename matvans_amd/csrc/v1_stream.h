@@ -1,0 +1,233 @@
+// One lane's rANS stream of the MVX1 layouts on gfx950: LDS word ring fed by buffer
+// loads, and the decode steps for 8- and 12-bit probabilities.
+//
+// Layout: a row is K granules of 512 bytes; each of the row's S streams owns GW = 256/S
+// consecutive u16 words of every granule (GW = 8 for S = 32, the reference layout of
+// create_matrices_WarpPerRow.py:405-422; GW = 4 for S = 64).  Word j of stream l is at
+// u16 index cursor + 256*(j/GW) + GW*l + j%GW; words 0 and 1 are the initial state.
+//
+// Decode (P-bit probabilities, 32-bit state, 16-bit words):
+//   q = x & (2^P - 1); x = (x >> P) * f[q] + bias[q]; if x < 2^16: x = x << 16 | word
+// P = 8 keeps the state rotated (xr = rotr(x, 8), so x >> 8 is the low 24 bits that
+// v_mad_u32_u24 reads and q is byte 3) and reads the ring once per two steps, because a
+// renormalised state is >= 2^24 and the next step leaves it >= 2^16.  P = 12 keeps x
+// plain and reads the ring after every step (two renormalisations can follow each other).
+//
+// Ring: 64 words per lane, lane-interleaved u16 (word slot w of lane i at
+// (w*64 + i)*2 bytes of the wave's ring), conflict-free for reads and writes.  Every
+// 16-step group consumes at most M words (8 for P = 8, 16 for P = 12), and NG = M/GW
+// granules are written per group, loaded two groups ahead.  A granule is loaded only when
+// it will fit (the read index only grows); otherwise an out-of-range buffer offset
+// returns zeros without touching memory and the write goes to dummy rows, so the number
+// of loads in flight is fixed and the compiler's wait counters stay exact.  With 64 ring
+// words, a blocked load implies at least 64 - GW - 2M > M + 1 words are already in the
+// ring, so the group that follows never reads an unwritten slot.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mv {
+namespace v1 {
+
+constexpr int kRingWords = 64;
+constexpr uint32_t kRingWaveBytes = kRingWords * 64 * 2;   // 8 KiB per wave
+constexpr uint32_t kOOB = 0x80000000u;
+constexpr uint32_t kDummyWords = 16;   // dummy rows (u16 x 64 lanes each) per wave
+constexpr uint32_t kSelRenorm8 = 0x00050401u;   // rotr(x1 << 16 | word, 8)
+constexpr uint32_t kSelKeep8 = 0x04070605u;     // rotr(x1, 8)
+constexpr uint32_t kSelRenorm12 = 0x05040100u;  // x1 << 16 | word
+constexpr uint32_t kSelKeep12 = 0x07060504u;    // x1
+
+template <int GW>
+struct Chunk;
+template <>
+struct Chunk<8> {
+    uint32_t d[4];
+};
+template <>
+struct Chunk<4> {
+    uint32_t d[2];
+};
+
+template <int GW>
+__device__ __forceinline__ Chunk<GW> load_chunk(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    Chunk<GW> c;
+    if constexpr (GW == 8) {
+        auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+        c.d[0] = v[0]; c.d[1] = v[1]; c.d[2] = v[2]; c.d[3] = v[3];
+    } else {
+        auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+        c.d[0] = v[0]; c.d[1] = v[1];
+    }
+    return c;
+}
+
+template <int GW, int P>
+struct Lane {
+    static constexpr int M = (P == 8) ? 8 : 16;     // max words per 16-step group
+    static constexpr int NG = M / GW;               // granules written per group
+    static constexpr int kPre = 48 / GW;            // granules loaded by the prologue
+
+    __amdgpu_buffer_rsrc_t rsrc;
+    uint32_t lane_off;   // byte offset of this lane's slice in granule 0
+    uint32_t K;          // granules of this lane's row (0 = inactive lane)
+    uint32_t gl;         // next granule to load
+    uint32_t x;          // state (rotated for P = 8)
+    uint32_t j;          // index of the next unread word
+    uint32_t W;          // that word
+    uint32_t waddr;      // LDS byte offset of word j within the ring array
+    uint32_t wave_base;
+    Chunk<GW> pa[NG], pb[NG];
+    uint32_t ga[NG], gb[NG];
+    uint16_t *ring, *dummy;
+    uint32_t lane;
+
+    // win_beg/win_end: wave-uniform u16 window holding every row of the wave;
+    // row_cur/row_end: this lane's row; slot: stream index within the row.
+    __device__ void init(const uint16_t *payload, uint32_t win_beg, uint32_t win_end, bool valid,
+                         uint32_t row_cur, uint32_t row_end, uint32_t slot, uint16_t *ring_array,
+                         uint16_t *dummy_rows, uint32_t wave, uint32_t lane_in_wave) {
+        rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(payload) + win_beg, 0,
+                                                 static_cast<int>((win_end - win_beg) * 2u),
+                                                 0x00020000);
+        K = 0;
+        uint32_t roff = 0;
+        if (valid) {
+            if (row_end > win_end) row_end = win_end;
+            if ((row_cur & 255u) == 0 && row_cur >= win_beg && row_end > row_cur) {
+                K = (row_end - row_cur) >> 8;
+                roff = (row_cur - win_beg) * 2u;
+            }
+        }
+        lane_off = roff + slot * (GW * 2u);
+        ring = ring_array;
+        dummy = dummy_rows;
+        lane = lane_in_wave;
+        wave_base = wave * kRingWaveBytes;
+#pragma unroll
+        for (int i = 0; i < NG; i++) ga[i] = gb[i] = 0xFFFFFFFFu;
+    }
+
+    __device__ __forceinline__ uint16_t &ring_at(uint32_t w) {
+        return ring[(wave_base >> 1) + w * 64 + lane];
+    }
+
+    // Granule g into its ring slots; g == ~0 writes the zeros of an out-of-range load to
+    // the wave's dummy rows instead, so the write is not behind a branch.
+    __device__ __forceinline__ void write_granule(const Chunk<GW> &d, uint32_t g) {
+        uint32_t s = (g * GW) & (kRingWords - 1);
+        uint16_t *p = g != 0xFFFFFFFFu ? &ring_at(s) : dummy + lane;
+#pragma unroll
+        for (int i = 0; i < GW / 2; i++) {
+            p[(2 * i) * 64] = static_cast<uint16_t>(d.d[i]);
+            p[(2 * i + 1) * 64] = static_cast<uint16_t>(d.d[i] >> 16);
+        }
+    }
+
+    __device__ __forceinline__ void issue(Chunk<GW> &p, uint32_t &gp) {
+        bool ok = gl < K && (gl + 1) * GW - j <= uint32_t(kRingWords);
+        p = load_chunk<GW>(rsrc, ok ? lane_off + gl * 512u : kOOB);
+        gp = ok ? gl : 0xFFFFFFFFu;
+        gl += ok ? 1u : 0u;
+    }
+
+    __device__ void prologue() {
+        Chunk<GW> pre[kPre];
+#pragma unroll
+        for (uint32_t g = 0; g < uint32_t(kPre); g++)
+            pre[g] = load_chunk<GW>(rsrc, g < K ? lane_off + g * 512u : kOOB);
+#pragma unroll
+        for (uint32_t g = 0; g < uint32_t(kPre); g++) write_granule(pre[g], g < K ? g : 0xFFFFFFFFu);
+        gl = K < uint32_t(kPre) ? K : uint32_t(kPre);
+    }
+
+    __device__ void start() {
+        uint32_t x0 = (uint32_t(ring_at(0)) << 16) | ring_at(1);
+        x = (P == 8) ? ((x0 >> 8) | (x0 << 24)) : x0;
+        j = 2;
+        W = ring_at(2);
+        waddr = wave_base | (2u * 128u) | (lane * 2u);
+#pragma unroll
+        for (int i = 0; i < NG; i++) issue(pa[i], ga[i]);
+#pragma unroll
+        for (int i = 0; i < NG; i++) issue(pb[i], gb[i]);
+    }
+
+    __device__ __forceinline__ void refill_a() {
+#pragma unroll
+        for (int i = 0; i < NG; i++) write_granule(pa[i], ga[i]);
+#pragma unroll
+        for (int i = 0; i < NG; i++) issue(pa[i], ga[i]);
+    }
+    __device__ __forceinline__ void refill_b() {
+#pragma unroll
+        for (int i = 0; i < NG; i++) write_granule(pb[i], gb[i]);
+#pragma unroll
+        for (int i = 0; i < NG; i++) issue(pb[i], gb[i]);
+    }
+
+    // ring byte offset of the next word; the add may carry into the wave-base bits, the
+    // mask restores them
+    __device__ __forceinline__ void advance_word(uint32_t cc) {
+        uint32_t t = waddr + (cc ? 128u : 0u);
+        waddr = (t & (kRingWaveBytes - 1)) | (wave_base & ~(kRingWaveBytes - 1));
+        W = *reinterpret_cast<const uint16_t *>(reinterpret_cast<const char *>(ring) + waddr);
+    }
+
+    __device__ __forceinline__ void end_group(uint32_t waddr_before) {
+        j += ((waddr - waddr_before) >> 7) & uint32_t(kRingWords - 1);
+    }
+
+    // P = 8, two steps, dword tables A[q] = f | sym << 24 at lut[q], B[q] at lut[256 + q];
+    // returns the two A entries (symbol in byte 3).
+    __device__ __forceinline__ void pair8(const uint32_t *lut, uint32_t &e0, uint32_t &e1) {
+        uint32_t cc = 0;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            uint32_t q = x >> 24;
+            uint32_t ea = lut[q];
+            uint32_t eb = lut[q + 256];
+            uint32_t x1 = __umul24(x, ea) + eb;
+            bool c = x1 < 0x10000u;
+            x = __builtin_amdgcn_perm(x1, W, c ? kSelRenorm8 : kSelKeep8);
+            cc += c ? 1u : 0u;
+            (h == 0 ? e0 : e1) = ea;
+        }
+        advance_word(cc);
+    }
+
+    // P = 8, two steps, byte tables f[q], bias[q], sym[q] at tab + 0 / 256 / 512 (tab may
+    // differ per wave); returns the two symbols.
+    __device__ __forceinline__ void pair8_u8(const uint8_t *tab, uint32_t &s0, uint32_t &s1) {
+        uint32_t cc = 0;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            uint32_t q = x >> 24;
+            uint32_t f = tab[q];
+            uint32_t b = tab[256 + q];
+            uint32_t sy = tab[512 + q];
+            uint32_t x1 = __umul24(x, f) + b;
+            bool c = x1 < 0x10000u;
+            x = __builtin_amdgcn_perm(x1, W, c ? kSelRenorm8 : kSelKeep8);
+            cc += c ? 1u : 0u;
+            (h == 0 ? s0 : s1) = sy;
+        }
+        advance_word(cc);
+    }
+
+    // P = 12, one step, dword tables A[q] at lut[q], B[q] at lut[4096 + q]; returns A.
+    __device__ __forceinline__ uint32_t step12(const uint32_t *lut) {
+        uint32_t q = x & 0xFFFu;
+        uint32_t ea = lut[q];
+        uint32_t eb = lut[q + 4096];
+        uint32_t x1 = __umul24(x >> 12, ea) + eb;
+        bool c = x1 < 0x10000u;
+        x = __builtin_amdgcn_perm(x1, W, c ? kSelRenorm12 : kSelKeep12);
+        advance_word(c ? 1u : 0u);
+        return ea;
+    }
+};
+
+}  // namespace v1
+}  // namespace mv

@@ -1,0 +1,172 @@
+"""MVX1 matrices: the format extensions of the WarpPerRow codec (64 streams per row,
+12-bit probabilities, one table per row), over the C ABI of ``lib/libmatvans.so``.
+
+  CompressedMatrixV1.encode(W, streams, precision, table_mode)   library encoder
+  CompressedMatrixV1.to_device() / .multiply(vecs)               fused decode-matvec
+  read_mvx1(path) / write_mvx1(path, ...)                        MVX1 container
+
+``streams=32, precision=8, table_mode=0`` is the reference's layout and model
+(create_matrices_WarpPerRow.py:131-272,353-430); the others are extensions.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from .capi import V1DevMatrix, V1Host, call, lib
+
+__all__ = ["CompressedMatrixV1", "read_mvx1", "write_mvx1"]
+
+
+def _np_ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+class CompressedMatrixV1:
+    """One MVX1 matrix.  Host arrays: cursors uint32[rows], cdf uint16[G+1] (table_mode 0)
+    or freq uint8[rows, G] (table_mode 1), payload uint16[payload_size]."""
+
+    def __init__(self, rows, cols, streams, precision, table_mode, min_value, G, cursors,
+                 payload, cdf=None, freq=None, grid_spacing=1.0):
+        self.rows, self.cols = int(rows), int(cols)
+        self.streams, self.precision, self.table_mode = int(streams), int(precision), int(table_mode)
+        self.min_value, self.G = int(min_value), int(G)
+        self.cursors = np.ascontiguousarray(cursors, np.uint32)
+        self.payload = np.ascontiguousarray(payload, np.uint16)
+        self.payload_size = len(self.payload)
+        self.cdf = None if cdf is None else np.ascontiguousarray(cdf, np.uint16)
+        self.freq = None if freq is None else np.ascontiguousarray(freq, np.uint8).reshape(
+            self.rows, self.G)
+        self.grid_spacing = float(grid_spacing)
+        self.d_cursors = self.d_lut = self.d_freq = self.d_payload = None
+
+    @property
+    def stored_bytes(self):
+        """Payload + cursors + tables, as stored."""
+        tab = 2 * (self.G + 1) if self.table_mode == 0 else self.rows * self.G
+        return 2 * self.payload_size + 4 * self.rows + tab
+
+    @classmethod
+    def encode(cls, matrix, streams=64, precision=8, table_mode=0, n_threads=0):
+        m = np.ascontiguousarray(np.asarray(matrix), dtype=np.int8)
+        rows, cols = m.shape
+        mn, G = ctypes.c_int32(), ctypes.c_uint32()
+        cdf = np.zeros(257, np.uint16)
+        freq = np.zeros(rows * 256 if table_mode == 1 else 1, np.uint8)
+        call("matvans_v1_model", _np_ptr(m), rows, cols, precision, table_mode, ctypes.byref(mn),
+             ctypes.byref(G), _np_ptr(cdf), _np_ptr(freq))
+        g = G.value
+        cap = int(lib().matvans_v1_encode_bound(rows, cols, streams, precision))
+        cur = np.empty(rows, np.uint32)
+        pay = np.empty(cap, np.uint16)
+        size = ctypes.c_uint64()
+        call("matvans_v1_encode", _np_ptr(m), rows, cols, streams, precision, table_mode,
+             mn.value, g, _np_ptr(cdf), _np_ptr(freq), int(n_threads), _np_ptr(cur),
+             _np_ptr(pay), cap, ctypes.byref(size))
+        return cls(rows, cols, streams, precision, table_mode, mn.value, g, cur,
+                   pay[:size.value].copy(),
+                   cdf=cdf[:g + 1].copy() if table_mode == 0 else None,
+                   freq=freq[:rows * g].copy() if table_mode == 1 else None)
+
+    def lut(self):
+        """Decode table of a one-table matrix (2 * 2^P uint32)."""
+        out = np.zeros(2 << self.precision, np.uint32)
+        call("matvans_v1_lut", _np_ptr(self.cdf), self.G, self.min_value, self.precision,
+             _np_ptr(out))
+        return out
+
+    def to_device(self, device="cuda"):
+        import torch
+        dev = torch.device(device)
+        self.d_cursors = torch.from_numpy(self.cursors.view(np.int32)).to(dev)
+        self.d_payload = torch.from_numpy(self.payload.view(np.int16)).to(dev)
+        if self.table_mode == 0:
+            self.d_lut = torch.from_numpy(self.lut().view(np.int32)).to(dev)
+        else:
+            self.d_freq = torch.from_numpy(self.freq.reshape(-1)).to(dev)
+        return self
+
+    def dev_struct(self) -> V1DevMatrix:
+        if self.d_payload is None:
+            raise RuntimeError("matrix is not on the device; call to_device() first")
+        return V1DevMatrix(self.rows, self.cols, self.streams, self.precision, self.table_mode,
+                           self.min_value, self.G, self.payload_size, self.d_cursors.data_ptr(),
+                           self.d_lut.data_ptr() if self.d_lut is not None else 0,
+                           self.d_freq.data_ptr() if self.d_freq is not None else 0,
+                           self.d_payload.data_ptr())
+
+    def multiply(self, vec, out=None, stream=None):
+        """int32 W @ vec for one vector [cols] or a batch [n_vec, cols] (n_vec <= 8)."""
+        import torch
+        s = self.dev_struct()
+        squeeze = vec.dim() == 1
+        v = vec.unsqueeze(0) if squeeze else vec
+        if v.dtype != torch.int8 or v.shape[-1] != self.cols:
+            raise ValueError(f"expected int8 vectors of length {self.cols}")
+        n = v.shape[0]
+        if out is None:
+            out = torch.empty((n, self.rows), dtype=torch.int32, device=v.device)
+        call("matvans_v1_decompress_and_multiply", ctypes.byref(s), _ptr(out), _ptr(v), n,
+             v.stride(0), _stream(stream))
+        return out[0] if squeeze else out
+
+    def host_struct(self, keep) -> V1Host:
+        keep += [self.cursors, self.payload, self.cdf, self.freq]
+        return V1Host(self.rows, self.cols, self.grid_spacing, self.streams, self.precision,
+                      self.table_mode, self.min_value, self.G, self.payload_size,
+                      self.cursors.ctypes.data,
+                      self.cdf.ctypes.data if self.cdf is not None else 0,
+                      self.freq.ctypes.data if self.freq is not None else 0,
+                      self.payload.ctypes.data)
+
+
+def read_mvx1(path):
+    """Returns (header dict, vector int8, [CompressedMatrixV1])."""
+    h = ctypes.c_void_p()
+    call("matvans_v1_read", str(path).encode(), ctypes.byref(h))
+    try:
+        num, rh, lv = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        vp = ctypes.c_void_p()
+        call("matvans_v1_header", h, ctypes.byref(num), ctypes.byref(rh), ctypes.byref(vp),
+             ctypes.byref(lv))
+
+        def arr(p, ct, n):
+            if n == 0:
+                return np.zeros(0, np.dtype(ct))
+            return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ct)), (n,)).copy()
+        vec = arr(vp, ctypes.c_int8, lv.value)
+        mats = []
+        for k in range(num.value):
+            m = V1Host()
+            call("matvans_v1_matrix", h, k, ctypes.byref(m))
+            mats.append(CompressedMatrixV1(
+                m.rows, m.cols, m.streams, m.precision, m.table_mode, m.min_value, m.G,
+                arr(m.cursors, ctypes.c_uint32, m.rows),
+                arr(m.payload, ctypes.c_uint16, m.payload_size),
+                cdf=arr(m.cdf, ctypes.c_uint16, m.G + 1) if m.table_mode == 0 else None,
+                freq=arr(m.freq, ctypes.c_uint8, m.rows * m.G) if m.table_mode == 1 else None,
+                grid_spacing=m.grid_spacing))
+        return {"num_matrices": num.value, "result_hash": rh.value}, vec, mats
+    finally:
+        lib().matvans_v1_free(h)
+
+
+def write_mvx1(path, matrices, vector=None, result_hash=0):
+    vec = np.ascontiguousarray(np.zeros(0, np.int8) if vector is None else vector, np.int8)
+    keep = []
+    arr = (V1Host * max(1, len(matrices)))()
+    for i, m in enumerate(matrices):
+        arr[i] = m.host_struct(keep)
+    call("matvans_v1_write", str(path).encode(), result_hash, _np_ptr(vec), len(vec),
+         len(matrices), arr)

@@ -1,0 +1,339 @@
+"""CPU oracle for the MVX1 format extensions: S interleaved streams per row (32 or 64),
+P-bit probabilities (8 or 12) and per-row frequency tables.
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import it, and only as the checker.
+
+The extensions have no reference implementation; they generalise the reference's
+WarpPerRow codec (create_matrices_WarpPerRow.py:131-272,353-430 and
+matvansWarpPerRow-kernel.cu:95-174), which is the special case S = 32, P = 8, one table
+per matrix.  Pinned by (``tests/test_oracle_ext.py``):
+  * the special case: ``encode_rows(S=32, P=8, cdf)`` reproduces ``oracle.encode_matrix``
+    (itself pinned to the reference's golden payloads) word for word, and
+    ``decode_rows`` decodes the golden files;
+  * round trips: decode(encode(W)) == W for every (S, P, table mode), on Gaussian,
+    Laplace, near-constant and wide-alphabet matrices, constant rows included;
+  * closed forms: the model sums to 2^P with every present symbol >= 1; a textbook rANS
+    decode of one stream (independent of the vectorised decoder); brute-force matvec;
+  * the container: write(read(x)) is byte-identical, and ``decode_rows`` of a record
+    written by the C++ library equals its input.
+
+Codec, generalised (32-bit state x, 16-bit words, P-bit quantiles):
+  encode push(s): if (x >> (32 - P)) >= f: emit x & 0xFFFF; x >>= 16
+                  x = ((x // f) << P) | (x % f + c[s])
+  decode:         q = x & (2^P - 1); s = ppf[q]; x = (x >> P) * f[s] + q - c[s]
+                  if x < 2^16: x = (x << 16) | next word
+Stream l of a row decodes columns 16*S*(k // 16) + 16*l + k % 16 (k = 0 .. cols/S - 1).
+A row's payload is K granules of 512 bytes; in each granule every stream owns
+GW = 256 / S consecutive words (8 for S = 32, as in the reference layout; 4 for S = 64),
+so stream l's word j sits at u16 index cursor + 256*(j // GW) + GW*l + j % GW.
+
+MVX1 container (little-endian):
+  "MVX1", u32 version = 1, u32 num_matrices, u32 result_hash, u32 len_v,
+  i8 vector[len_v], zero pad to 4 bytes; then per matrix:
+  u32 rows, u32 cols, f32 grid_spacing, u32 streams, u32 precision, u32 table_mode,
+  i32 min_value, u32 G, u32 payload_size, u32 cursors[rows],
+  table_mode 0: u16 cdf[G + 1] | table_mode 1: u8 freq[rows][G], zero pad to 4 bytes,
+  u16 payload[payload_size], zero pad to 4 bytes.
+"""
+from __future__ import annotations
+
+import struct
+
+import numpy as np
+
+MAGIC = b"MVX1"
+VERSION = 1
+
+
+# ------------------------------------------------------------------------ models
+def quantize_counts(counts, total):
+    """Quantise symbol counts to integer frequencies summing to ``total``.
+
+    The reference's steps (create_matrices_WarpPerRow.py:145-170) with a deterministic
+    order: entries sorted by count descending, equal counts by symbol index descending;
+    ``max(1, round_half_even(count * total / n))`` for every present symbol; an excess is
+    shaved one unit at a time from the leading entries that are above 1, a deficit is
+    spread over the trailing entries.  Absent symbols get 0.  Returns int64[len(counts)]
+    or raises ValueError if the counts cannot be normalised."""
+    counts = np.asarray(counts, np.int64)
+    idx = np.nonzero(counts)[0]
+    n = int(counts.sum())
+    if n == 0:
+        raise ValueError("no symbols")
+    order = idx[np.lexsort((-idx, -counts[idx]))]
+    q = np.maximum(1, np.round(counts[order] * (total / n)).astype(np.int64))
+    excess = int(q.sum()) - total
+    while excess > 0:
+        nred = int((q > 1).sum())
+        nr = min(excess, nred)
+        if nr <= 0:
+            raise ValueError("model cannot be normalised")
+        q[:nr] -= 1
+        excess -= nr
+    if excess < 0:
+        if -excess > len(q):
+            raise ValueError("model cannot be normalised")
+        q[excess:] += 1
+    f = np.zeros(len(counts), np.int64)
+    f[order] = q
+    assert int(f.sum()) == total
+    return f
+
+
+def global_model(matrix, precision):
+    """One table for the matrix: (min_value, G, cdf uint16[G + 1]).  A constant matrix
+    gets a dummy second symbol of frequency 1 (as create_matrices_WarpPerRow.py:135-143)."""
+    m = np.asarray(matrix, np.int8)
+    mn, mx = int(m.min()), int(m.max())
+    total = 1 << precision
+    if mn == mx:
+        return mn, 2, np.array([0, total - 1, total], np.uint16)
+    G = mx - mn + 1
+    counts = np.bincount((m.astype(np.int64) - mn).ravel(), minlength=G)
+    f = quantize_counts(counts, total)
+    cdf = np.concatenate([[0], np.cumsum(f)]).astype(np.uint16)
+    return mn, G, cdf
+
+
+def row_freqs(row_symbols, G):
+    """Per-row 8-bit frequencies over the matrix alphabet (G >= 2).  A row with a single
+    distinct symbol s gets f[s] = 255 and a dummy neighbour (s + 1, or s - 1 at the top)
+    with f = 1, so that every frequency fits a byte."""
+    counts = np.bincount(np.asarray(row_symbols, np.int64), minlength=G)
+    present = np.nonzero(counts)[0]
+    if len(present) == 1:
+        s = int(present[0])
+        f = np.zeros(G, np.int64)
+        f[s] = 255
+        f[s + 1 if s + 1 < G else s - 1] = 1
+        return f.astype(np.uint8)
+    return quantize_counts(counts, 256).astype(np.uint8)
+
+
+def per_row_model(matrix):
+    """(min_value, G, freq uint8[rows][G]) with one 8-bit table per row."""
+    m = np.asarray(matrix, np.int8)
+    mn, mx = int(m.min()), int(m.max())
+    G = max(2, mx - mn + 1)
+    sym = m.astype(np.int64) - mn
+    return mn, G, np.stack([row_freqs(sym[r], G) for r in range(m.shape[0])])
+
+
+def cdf_of(freq):
+    return np.concatenate([[0], np.cumsum(np.asarray(freq, np.int64))])
+
+
+def ppf_of(cdf, precision):
+    """ppf[q] = the symbol s with cdf[s] <= q < cdf[s + 1] (never a zero-frequency one)."""
+    cdf = np.asarray(cdf, np.int64)
+    return (np.searchsorted(cdf, np.arange(1 << precision), side="right") - 1).astype(np.int64)
+
+
+def lut_of(cdf, min_value, precision):
+    """Fused decode table over quantiles: A[q] = f | (uint8)(min + s) << 24,
+    B[q] = q - cdf[s]; returns uint32[2 * 2^P] = A then B."""
+    cdf = np.asarray(cdf, np.int64)
+    ppf = ppf_of(cdf, precision)
+    q = np.arange(1 << precision)
+    f = cdf[ppf + 1] - cdf[ppf]
+    sym = (min_value + ppf) & 0xFF
+    return np.concatenate([f | (sym << 24), q - cdf[ppf]]).astype(np.uint32)
+
+
+# ------------------------------------------------------------------------- coder
+def stream_columns(cols, streams, lane):
+    k = np.arange(cols // streams)
+    return 16 * streams * (k // 16) + 16 * lane + (k % 16)
+
+
+def encode_stream(symbols, cdf, precision):
+    """Encode one stream (symbols in decode order); returns its words in decode order:
+    [x_hi, x_lo, last flushed, ..., first flushed] (AnsCoder push/interrupt,
+    create_matrices_WarpPerRow.py:226-272, with P-bit probabilities)."""
+    cdf = [int(c) for c in cdf]
+    x = 1 << 16
+    bulk = []
+    for s in reversed([int(v) for v in symbols]):
+        f = cdf[s + 1] - cdf[s]
+        if (x >> (32 - precision)) >= f:
+            bulk.append(x & 0xFFFF)
+            x >>= 16
+        x = ((x // f) << precision) | (x % f + cdf[s])
+    bulk.append(x & 0xFFFF)
+    bulk.append(x >> 16)
+    return bulk[::-1]
+
+
+def encode_rows(matrix, streams, precision, min_value, cdf=None, freq=None):
+    """Encode a matrix with ``streams`` streams per row, one table (``cdf``) or one per
+    row (``freq``).  Returns (cursors uint32[rows], payload uint16)."""
+    m = np.asarray(matrix, np.int8)
+    rows, cols = m.shape
+    assert streams in (32, 64) and cols % (16 * streams) == 0
+    GW = 256 // streams
+    out, cursors, pos = [], [], 0
+    for r in range(rows):
+        c = cdf if freq is None else cdf_of(freq[r])
+        sym = m[r].astype(np.int64) - min_value
+        lanes = [encode_stream(sym[stream_columns(cols, streams, l)], c, precision)
+                 for l in range(streams)]
+        K = (max(len(s) for s in lanes) + GW - 1) // GW
+        row = np.zeros(K * 256, np.uint16)
+        for l, s in enumerate(lanes):
+            j = np.arange(len(s))
+            row[256 * (j // GW) + GW * l + j % GW] = s
+        cursors.append(pos)
+        pos += len(row)
+        out.append(row)
+    return np.asarray(cursors, np.uint32), np.concatenate(out)
+
+
+def decode_rows(rows, cols, streams, precision, cursors, payload, min_value, cdf=None,
+                freq=None, payload_size=None):
+    """Decode every row, vectorised over the rows x streams streams.  Raises IndexError
+    if a stream would read outside the payload."""
+    payload = np.asarray(payload, np.uint16)
+    size = len(payload) if payload_size is None else int(payload_size)
+    S, P = streams, precision
+    mask = (1 << P) - 1
+    if freq is None:
+        cdfs = np.broadcast_to(np.asarray(cdf, np.int64), (rows, len(cdf)))
+    else:
+        cdfs = np.stack([cdf_of(freq[r]) for r in range(rows)])
+    ppfs = np.stack([ppf_of(cdfs[r], P) for r in range(rows)])
+    n = rows * S
+    row = np.repeat(np.arange(rows), S)
+    lane = np.tile(np.arange(S), rows)
+    base = np.repeat(np.asarray(cursors, np.int64)[:rows], S) + (256 // S) * lane
+
+    x = ((_words(payload, size, base, S, np.zeros(n, np.int64)) << 16)
+         | _words(payload, size, base, S, np.ones(n, np.int64)))
+    ptr = np.full(n, 2, np.int64)
+    out = np.empty((rows, cols), np.int8)
+    for k in range(cols // S):
+        q = x & mask
+        s = ppfs[row, q]
+        col = 16 * S * (k // 16) + 16 * lane + k % 16
+        out[row, col] = ((min_value + s) & 0xFF).astype(np.uint8).view(np.int8)
+        c0 = cdfs[row, s]
+        f = cdfs[row, s + 1] - c0
+        x = (x >> P) * f + q - c0
+        need = x < (1 << 16)
+        if need.any():
+            i = np.nonzero(need)[0]
+            x[i] = (x[i] << 16) | _words(payload, size, base[i], S, ptr[i])
+            ptr[i] += 1
+    return out
+
+
+def _words(payload, size, base, S, j):
+    GW = 256 // S
+    p = base + 256 * (j // GW) + j % GW
+    if p.size and p.max() >= size:
+        raise IndexError("stream reads past the payload")
+    return payload[p].astype(np.int64)
+
+
+def decode_stream_textbook(words, n, cdf, precision):
+    """Textbook rANS decode of one stream given as a list of words in decode order
+    (state = w0 << 16 | w1, then pop); independent of the layout and the vectorised
+    decoder.  Returns the n symbol indices."""
+    cdf = [int(c) for c in cdf]
+    x = (words[0] << 16) | words[1]
+    p = 2
+    out = []
+    M = 1 << precision
+    for _ in range(n):
+        q = x % M
+        s = max(i for i in range(len(cdf) - 1) if cdf[i] <= q and cdf[i + 1] > q)
+        out.append(s)
+        x = (cdf[s + 1] - cdf[s]) * (x // M) + q - cdf[s]
+        if x < 1 << 16:
+            x = x * 65536 + words[p]
+            p += 1
+    return out
+
+
+# --------------------------------------------------------------------- container
+class RecordV1:
+    def __init__(self, rows, cols, streams, precision, table_mode, min_value, G, cursors,
+                 payload, cdf=None, freq=None, grid_spacing=1.0):
+        self.rows, self.cols, self.streams, self.precision = rows, cols, streams, precision
+        self.table_mode, self.min_value, self.G = table_mode, min_value, G
+        self.cursors = np.asarray(cursors, np.uint32)
+        self.payload = np.asarray(payload, np.uint16)
+        self.cdf = None if cdf is None else np.asarray(cdf, np.uint16)
+        self.freq = None if freq is None else np.asarray(freq, np.uint8).reshape(rows, G)
+        self.grid_spacing = grid_spacing
+
+    def decode(self):
+        return decode_rows(self.rows, self.cols, self.streams, self.precision, self.cursors,
+                           self.payload, self.min_value, self.cdf, self.freq)
+
+
+def encode_record(matrix, streams=64, precision=8, table_mode=0):
+    m = np.asarray(matrix, np.int8)
+    if table_mode == 0:
+        mn, G, cdf = global_model(m, precision)
+        cur, pay = encode_rows(m, streams, precision, mn, cdf=cdf)
+        return RecordV1(m.shape[0], m.shape[1], streams, precision, 0, mn, G, cur, pay, cdf=cdf)
+    assert precision == 8
+    mn, G, freq = per_row_model(m)
+    cur, pay = encode_rows(m, streams, precision, mn, freq=freq)
+    return RecordV1(m.shape[0], m.shape[1], streams, 8, 1, mn, G, cur, pay, freq=freq)
+
+
+def _pad4(f, n):
+    f.write(b"\0" * ((4 - n % 4) % 4))
+
+
+def write_mvx1(path, records, vector=np.zeros(0, np.int8), result_hash=0):
+    with open(path, "wb") as f:
+        vector = np.asarray(vector, np.int8)
+        f.write(MAGIC + struct.pack("<LLLL", VERSION, len(records), result_hash, len(vector)))
+        f.write(vector.tobytes())
+        _pad4(f, len(vector))
+        for r in records:
+            f.write(struct.pack("<LLfLLLlLL", r.rows, r.cols, r.grid_spacing, r.streams,
+                                r.precision, r.table_mode, r.min_value, r.G, len(r.payload)))
+            f.write(np.asarray(r.cursors, "<u4").tobytes())
+            tab = (np.asarray(r.cdf, "<u2").tobytes() if r.table_mode == 0
+                   else np.asarray(r.freq, np.uint8).tobytes())
+            f.write(tab)
+            _pad4(f, len(tab))
+            f.write(np.asarray(r.payload, "<u2").tobytes())
+            _pad4(f, 2 * len(r.payload))
+
+
+def read_mvx1(path):
+    """Returns (header dict, vector, [RecordV1])."""
+    with open(path, "rb") as f:
+        data = f.read()
+    if data[:4] != MAGIC:
+        raise ValueError("not an MVX1 file")
+    ver, num, rhash, lv = struct.unpack_from("<LLLL", data, 4)
+    if ver != VERSION:
+        raise ValueError("unsupported MVX1 version")
+    o = 20
+    vec = np.frombuffer(data, np.int8, lv, o).copy()
+    o += lv + (4 - lv % 4) % 4
+    recs = []
+    for _ in range(num):
+        rows, cols, gs, S, P, mode, mn, G, psize = struct.unpack_from("<LLfLLLlLL", data, o)
+        o += 36
+        cur = np.frombuffer(data, "<u4", rows, o).copy()
+        o += 4 * rows
+        if mode == 0:
+            cdf = np.frombuffer(data, "<u2", G + 1, o).copy()
+            n, freq = 2 * (G + 1), None
+        else:
+            freq = np.frombuffer(data, np.uint8, rows * G, o).copy().reshape(rows, G)
+            n, cdf = rows * G, None
+        o += n + (4 - n % 4) % 4
+        pay = np.frombuffer(data, "<u2", psize, o).copy()
+        o += 2 * psize + (4 - (2 * psize) % 4) % 4
+        recs.append(RecordV1(rows, cols, S, P, mode, mn, G, cur, pay, cdf, freq, gs))
+    if o != len(data):
+        raise ValueError("trailing bytes")
+    return {"num_matrices": num, "result_hash": rhash}, vec, recs

@@ -1,0 +1,89 @@
+"""Host side of the MVX1 extensions in the product library (no GPU): the C++ model,
+encoder, decode table and container against the oracle, and their error paths."""
+import numpy as np
+import pytest
+
+from matvans_amd.capi import MatvansError
+from matvans_amd.mvx import CompressedMatrixV1, read_mvx1, write_mvx1
+from oracle import ext as E
+
+from .helpers import gaussian_int8, laplace_int8
+
+VARIANTS = [(32, 8, 0), (64, 8, 0), (32, 12, 0), (64, 12, 0), (64, 8, 1)]
+
+
+def _mats():
+    near = np.zeros((5, 2048), np.int8)
+    near[2, 100] = -4
+    near[3] = 127
+    return [gaussian_int8(6, 2048, 6.0, seed=11)[0], laplace_int8(5, 4096, 4.0, seed=12)[0],
+            near, np.random.default_rng(5).integers(-128, 128, (3, 2048)).astype(np.int8)]
+
+
+@pytest.mark.parametrize("S,P,mode", VARIANTS)
+def test_encoder_matches_oracle(S, P, mode):
+    for m in _mats():
+        c = CompressedMatrixV1.encode(m, S, P, mode, n_threads=3)
+        r = E.encode_record(m, S, P, mode)
+        assert (c.min_value, c.G) == (r.min_value, r.G)
+        assert np.array_equal(c.cursors, r.cursors)
+        assert np.array_equal(c.payload, r.payload)
+        if mode == 0:
+            assert np.array_equal(c.cdf, r.cdf)
+            assert np.array_equal(c.lut(), E.lut_of(r.cdf, r.min_value, P))
+        else:
+            assert np.array_equal(c.freq, r.freq)
+        assert np.array_equal(r.decode(), m)
+
+
+def test_reference_special_case_matches_v0_library():
+    import matvans_amd as mv
+    m = laplace_int8(9, 3072, 4.0, seed=4)[0]
+    v0 = mv.CompressedMatrix.encode(m)
+    v1 = CompressedMatrixV1.encode(m, 32, 8, 0)
+    assert np.array_equal(v0.cursors, v1.cursors) and np.array_equal(v0.payload, v1.payload)
+
+
+def test_container_round_trip(tmp_path):
+    ms = _mats()
+    cs = [CompressedMatrixV1.encode(ms[0], 64, 12, 0), CompressedMatrixV1.encode(ms[2], 64, 8, 1),
+          CompressedMatrixV1.encode(ms[1], 32, 8, 0)]
+    vec = np.arange(7, dtype=np.int8)
+    p = tmp_path / "a.mvx"
+    write_mvx1(p, cs, vec, 77)
+    hdr, v, back = read_mvx1(p)
+    assert hdr == {"num_matrices": 3, "result_hash": 77} and np.array_equal(v, vec)
+    ohdr, ov, orecs = E.read_mvx1(p)
+    for c, b, o in zip(cs, back, orecs):
+        assert np.array_equal(c.payload, b.payload) and np.array_equal(c.cursors, b.cursors)
+        assert np.array_equal(o.decode(), E.decode_rows(c.rows, c.cols, c.streams, c.precision,
+                                                        c.cursors, c.payload, c.min_value,
+                                                        c.cdf, c.freq))
+    q = tmp_path / "b.mvx"
+    E.write_mvx1(q, orecs, ov, ohdr["result_hash"])
+    assert open(p, "rb").read() == open(q, "rb").read()
+
+
+def test_reader_rejects_bad_files(tmp_path):
+    c = CompressedMatrixV1.encode(_mats()[0], 64, 8, 0)
+    p = tmp_path / "a.mvx"
+    write_mvx1(p, [c])
+    data = open(p, "rb").read()
+    for name, blob in [("magic", b"MVX2" + data[4:]), ("truncated", data[:-6]),
+                       ("trailing", data + b"\0\0\0\0"),
+                       ("streams", data[:32] + (48).to_bytes(4, "little") + data[36:])]:
+        q = tmp_path / f"{name}.mvx"
+        open(q, "wb").write(blob)
+        with pytest.raises(MatvansError) as e:
+            read_mvx1(q)
+        assert e.value.status == 4, name
+
+
+def test_encoder_argument_errors():
+    m = _mats()[0]
+    with pytest.raises(MatvansError) as e:       # 12-bit per-row tables
+        CompressedMatrixV1.encode(m, 64, 12, 1)
+    assert e.value.status == 8
+    with pytest.raises(MatvansError) as e:       # cols not a multiple of 16 * 64
+        CompressedMatrixV1.encode(m[:, :512], 64, 8, 0)
+    assert e.value.status == 2

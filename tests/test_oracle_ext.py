@@ -1,0 +1,156 @@
+"""Pins the MVX1 extension oracle (oracle/ext.py): the reference special case against the
+golden-pinned v0 oracle and the golden files, round trips for every (streams, precision,
+table mode), a textbook decode of single streams, the models' invariants and the
+container's byte round trip."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import ext as E
+from oracle import oracle as O
+
+from .helpers import gaussian_int8, laplace_int8
+
+VARIANTS = [(32, 8, 0), (64, 8, 0), (32, 12, 0), (64, 12, 0), (64, 8, 1)]
+
+
+def _cases():
+    rng = np.random.default_rng(7)
+    wide = rng.integers(-127, 128, (5, 2048)).astype(np.int8)
+    near = np.zeros((6, 1024), np.int8)
+    near[1, 17] = 3
+    near[4, :] = -127          # a constant row at the alphabet edge
+    near[5, ::7] = 1
+    return {
+        "gauss6": gaussian_int8(9, 2048, 6.0, seed=1)[0],
+        "laplace4": laplace_int8(7, 3072, 4.0, seed=2)[0],
+        "gauss0.4": gaussian_int8(4, 1024, 0.4, seed=3)[0],
+        "near_const": near,
+        "wide": wide,
+        "const": np.full((3, 1024), 9, np.int8),
+    }
+
+
+CASES = _cases()
+
+
+def test_special_case_is_reference_encoder():
+    """S = 32, P = 8, one table: word for word the v0 encoder (pinned to the golden
+    payloads by tests/test_oracle.py)."""
+    for name, m in CASES.items():
+        mn, G, cdf, _, _ = O.create_entropy_model(m)
+        c0, p0, _ = O.encode_matrix(m, mn, cdf)
+        c1, p1 = E.encode_rows(m, 32, 8, mn, cdf=cdf)
+        assert np.array_equal(c0, c1) and np.array_equal(p0, p1), name
+
+
+@pytest.mark.parametrize("fx", ["compressed_matrices_512_5bit_count2.bin",
+                                "compressed_matrices_1024_5bit_count1.bin"])
+def test_decoder_reads_golden_files(golden_dir, fx):
+    hdr, vec, recs = O.read_bin(os.path.join(golden_dir, fx))
+    for r in recs:
+        cdf = r.cdf.astype(np.int64)
+        cdf[-1] = 256
+        got = E.decode_rows(r.rows, r.cols, 32, 8, r.cursors, r.payload, r.min_value, cdf=cdf)
+        assert np.array_equal(got, O.decode_matrix(r.rows, r.cols, r.cursors, r.payload,
+                                                   r.min_value, r.cdf, r.ppf))
+
+
+@pytest.mark.parametrize("S,P,mode", VARIANTS)
+@pytest.mark.parametrize("name", list(CASES))
+def test_round_trip(S, P, mode, name):
+    m = CASES[name]
+    if m.shape[1] % (16 * S):
+        pytest.skip("cols not a multiple of 16 * streams")
+    r = E.encode_record(m, S, P, mode)
+    assert r.payload.size % 256 == 0 and np.all(r.cursors % 256 == 0)
+    assert np.array_equal(r.decode(), m)
+    v = np.random.default_rng(3).integers(-127, 128, m.shape[1]).astype(np.int8)
+    assert np.array_equal(O.matvec_i32(r.decode(), v),
+                          (m.astype(np.int64) @ v.astype(np.int64)).astype(np.int32))
+
+
+@pytest.mark.parametrize("S,P,mode", VARIANTS)
+def test_textbook_decode_of_single_streams(S, P, mode):
+    """Gather one stream's words with the layout formula and decode them with a plain
+    pop loop: the symbols are the row's columns 16*S*(k//16) + 16*l + k%16."""
+    m = CASES["laplace4"][:, :2048] if S == 64 else CASES["laplace4"]
+    r = E.encode_record(m, S, P, mode)
+    GW = 256 // S
+    for row in (0, m.shape[0] - 1):
+        cdf = r.cdf if mode == 0 else E.cdf_of(r.freq[row])
+        for lane in (0, 5, S - 1):
+            n = m.shape[1] // S
+            words = []
+            j = 0
+            end = r.cursors[row + 1] if row + 1 < m.shape[0] else len(r.payload)
+            while True:
+                p = int(r.cursors[row]) + 256 * (j // GW) + GW * lane + j % GW
+                if p >= end:
+                    break
+                words.append(int(r.payload[p]))
+                j += 1
+            syms = E.decode_stream_textbook(words, n, cdf, P)
+            want = m[row, E.stream_columns(m.shape[1], S, lane)].astype(np.int64) - r.min_value
+            assert syms == want.tolist()
+
+
+def test_models():
+    m = CASES["gauss6"]
+    for P in (8, 12):
+        mn, G, cdf = E.global_model(m, P)
+        f = np.diff(cdf.astype(np.int64))
+        present = np.bincount(m.astype(np.int64).ravel() - mn, minlength=G) > 0
+        assert cdf[0] == 0 and cdf[-1] == 1 << P
+        assert np.all(f[present] >= 1) and np.all(f[~present] == 0)
+    mn, G, freq = E.per_row_model(CASES["near_const"])
+    assert np.all(freq.astype(np.int64).sum(axis=1) == 256)
+    row4 = freq[4]   # constant row at the bottom of the alphabet: 255 + dummy neighbour
+    assert row4[0] == 255 and row4[1] == 1
+    mn, G, cdf = E.global_model(CASES["const"], 12)
+    assert G == 2 and cdf.tolist() == [0, 4095, 4096]
+    # quantize_counts: closed forms
+    assert E.quantize_counts([1, 0, 1], 256).tolist() == [128, 0, 128]
+    assert E.quantize_counts([3, 1], 4).tolist() == [3, 1]
+
+
+def test_ppf_and_lut_closed_form():
+    cdf = np.array([0, 0, 10, 10, 200, 256])
+    ppf = E.ppf_of(cdf, 8)
+    want = [next(s for s in range(5) if cdf[s] <= q < cdf[s + 1]) for q in range(256)]
+    assert ppf.tolist() == want
+    lut = E.lut_of(cdf, -3, 8)
+    q = 57
+    s = want[q]
+    assert lut[q] == (cdf[s + 1] - cdf[s]) | (((-3 + s) & 0xFF) << 24)
+    assert lut[256 + q] == q - cdf[s]
+
+
+def test_planted_column_mistake_is_caught():
+    """A decoder that maps stream l to columns 16*32*(k//16) + ... for S = 64 (the
+    reference's 512-column grouping) does not reproduce the matrix."""
+    m = CASES["gauss6"]
+    r = E.encode_record(m, 64, 8, 0)
+    good = r.decode()
+    bad = np.empty_like(good)
+    for lane in range(64):
+        k = np.arange(m.shape[1] // 64)
+        col_bad = (512 * (k // 16) + 16 * lane + k % 16) % m.shape[1]
+        bad[:, col_bad] = good[:, E.stream_columns(m.shape[1], 64, lane)]
+    assert not np.array_equal(bad, m)
+
+
+def test_container_round_trip(tmp_path):
+    recs = [E.encode_record(CASES["gauss6"], 64, 12, 0), E.encode_record(CASES["near_const"], 64, 8, 1),
+            E.encode_record(CASES["laplace4"], 32, 8, 0)]
+    vec = np.arange(-5, 6, dtype=np.int8)
+    p = tmp_path / "a.mvx"
+    E.write_mvx1(p, recs, vec, 0xDEADBEEF)
+    hdr, v, back = E.read_mvx1(p)
+    assert hdr == {"num_matrices": 3, "result_hash": 0xDEADBEEF} and np.array_equal(v, vec)
+    for a, b in zip(recs, back):
+        assert np.array_equal(a.decode(), b.decode())
+    q = tmp_path / "b.mvx"
+    E.write_mvx1(q, back, v, hdr["result_hash"])
+    assert open(p, "rb").read() == open(q, "rb").read()
