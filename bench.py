@@ -49,6 +49,7 @@ def parse():
 # encodings: (label, streams, precision, table_mode); "v0" is the reference .bin layout
 FORMATS = {
     "v0": None,
+    "s128-p8": (128, 8, 0),
     "s64-p8": (64, 8, 0),
     "s64-p12": (64, 12, 0),
     "s64-p8-rowcdf": (64, 8, 1),
@@ -56,6 +57,7 @@ FORMATS = {
 }
 FORMAT_TEXT = {
     "v0": ".bin v0 (reference layout): 32 streams/row, global 8-bit CDF",
+    "s128-p8": "MVX1: 128 streams/row, global 8-bit CDF",
     "s64-p8": "MVX1: 64 streams/row, global 8-bit CDF",
     "s64-p12": "MVX1: 64 streams/row, global 12-bit CDF",
     "s64-p8-rowcdf": "MVX1: 64 streams/row, per-row 8-bit CDF",

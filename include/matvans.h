@@ -217,10 +217,10 @@ int matvans_encode_rows(const int8_t *mat, uint32_t rows, uint32_t cols, int8_t 
  * The reference has one variant (32 streams per row, 8-bit probabilities, one table per
  * matrix: create_matrices_WarpPerRow.py:131-272,353-430); its per-matrix constant-memory
  * CDF variant is matvansWarpPerRow_local_cdf.cu:18,196-363.  MVX1 adds
- *   streams    32 or 64 interleaved streams per row.  Stream l decodes columns
+ *   streams    32, 64 or 128 interleaved streams per row.  Stream l decodes columns
  *              16*S*(k/16) + 16*l + k%16.  A row is K granules of 512 bytes in which each
  *              stream owns 256/S consecutive u16 words (8 for S = 32: the reference
- *              layout; 4 for S = 64): word j of stream l is at
+ *              layout; 4 for S = 64; 2 for S = 128): word j of stream l is at
  *              cursor + 256*(j/(256/S)) + (256/S)*l + j%(256/S).
  *   precision  8 or 12 bit probabilities (decode q = x & (2^P-1);
  *              x = (x >> P)*f + q - cdf[s]; renormalise with a 16-bit word below 2^16).

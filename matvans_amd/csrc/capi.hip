@@ -97,7 +97,7 @@ extern "C" int matvans_v1_decompress_and_multiply(const matvans_v1_dev_matrix *m
     if (!m || !dst || !vecs || !m->cursors || !m->payload)
         return fail(MATVANS_E_INVALID, "null argument or device pointer");
     if (n_vec < 1 || n_vec > 8) return fail(MATVANS_E_INVALID, "n_vec %u outside 1..8", n_vec);
-    const bool s_ok = m->streams == 32 || m->streams == 64;
+    const bool s_ok = m->streams == 32 || m->streams == 64 || m->streams == 128;
     const bool p_ok = m->precision == 8 || m->precision == 12;
     if (!s_ok || !p_ok || m->table_mode > 1)
         return fail(MATVANS_E_INVALID, "streams %u, precision %u, table mode %u", m->streams,

@@ -1,10 +1,15 @@
-// Fused rANS decode x int8 matvec for the MVX1 layouts (S = 32 or 64 streams per row,
-// P = 8 or 12 bit probabilities, one table per matrix or one 8-bit table per row), gfx950.
+// Fused rANS decode x int8 matvec for the MVX1 layouts (S = 32, 64 or 128 streams per
+// row, P = 8 or 12 bit probabilities, one table per matrix or one 8-bit table per row),
+// gfx950.
 //
-// Mapping: every lane owns one stream.  S = 64: a wave decodes one row; S = 32: a wave
-// decodes two rows, one per 32-lane half (the reference layout,
-// matvansWarpPerRow-kernel.cu:95-174).  Stream l's 16-symbol group t covers columns
-// 16*S*t + 16*l .. +15, so each lane multiplies its own 16 bytes of each vector per group:
+// Mapping: every lane owns one stream.  S = 128: two waves of a block decode one row and
+// add their sums through LDS; S = 64: a wave decodes one row; S = 32: a wave decodes two
+// rows, one per 32-lane half (the reference layout, matvansWarpPerRow-kernel.cu:95-174).
+// The word ring holds 32 words per lane for P = 8 and 64 for P = 12 (v1_stream.h), so a
+// CU keeps up to 32 waves (P = 8) resident: the decode is a chain of dependent LDS
+// lookups per stream, and throughput comes from the number of chains in flight.
+// Stream l's 16-symbol group t covers columns 16*S*t + 16*l .. +15, so each lane
+// multiplies its own 16 bytes of each vector per group:
 // four symbols per v_dot4_i32_i8, one int32 accumulator per vector, then a DPP/shuffle
 // reduction over the row's lanes.
 //
@@ -61,18 +66,24 @@ __global__ __launch_bounds__(256) void v1_fwd(int32_t *__restrict__ dst,
                                               const uint8_t *__restrict__ freq,
                                               const uint16_t *__restrict__ payload,
                                               uint32_t payload_size) {
-    constexpr int RPW = 64 / S;           // rows per wave
-    constexpr int GW = 256 / S;           // words per lane per granule
+    constexpr int RPW = S < 64 ? 64 / S : 1;   // rows per wave
+    constexpr int WPR = S > 64 ? S / 64 : 1;   // waves per row
+    constexpr int GW = 256 / S;                // words per lane per granule
+    constexpr int R = P == 8 ? 32 : 64;        // ring words per lane
     constexpr int kLut = MODE == 1 ? 1 : 2 << P;   // dwords of the shared table
-    __shared__ uint16_t ring[kWaves * v1::kRingWords * 64];
+    __shared__ uint16_t ring[kWaves * R * 64];
     __shared__ uint32_t lut[kLut];
     __shared__ uint32_t rowtab[MODE == 1 ? kWaves * 384 : 1];   // per wave: f|bias|sym + scratch
     __shared__ uint16_t dummy[8 * 64];
+    __shared__ int32_t part[WPR > 1 ? kWaves * NV : 1];
 
     const uint32_t tid = threadIdx.x;
-    const uint32_t wave = tid >> 6, lane = tid & 63, sl = lane & (S - 1);
-    const uint32_t row0 = (blockIdx.x * kWaves + wave) * RPW;
-    const uint32_t row = row0 + lane / S;
+    const uint32_t wave = tid >> 6, lane = tid & 63;
+    // stream slot within the row, and the rows of this wave
+    const uint32_t sl = S >= 64 ? (wave % WPR) * 64 + lane : lane & (S - 1);
+    const uint32_t row0 = S >= 64 ? blockIdx.x * (kWaves / WPR) + wave / WPR
+                                  : (blockIdx.x * kWaves + wave) * RPW;
+    const uint32_t row = row0 + (S >= 64 ? 0u : lane / S);
     const bool valid = row < rows;
 
     if constexpr (MODE == 0) {
@@ -139,7 +150,7 @@ __global__ __launch_bounds__(256) void v1_fwd(int32_t *__restrict__ dst,
     const uint32_t rcur = valid ? cursors[row] : 0u;
     const uint32_t rend = valid ? (row + 1 < rows ? cursors[row + 1] : payload_size) : 0u;
 
-    Lane<GW, P> st;
+    Lane<GW, P, R> st;
     st.init(payload, wbeg, wend, valid, rcur, rend, sl, ring, dummy, wave, lane);
     st.prologue();
 
@@ -211,7 +222,7 @@ __global__ __launch_bounds__(256) void v1_fwd(int32_t *__restrict__ dst,
 #pragma unroll
     for (int v = 0; v < NV; v++) {
         int32_t a = acc[v];
-        if constexpr (S == 64) a += __shfl_xor(a, 32, 64);
+        if constexpr (S >= 64) a += __shfl_xor(a, 32, 64);
         a += __shfl_xor(a, 16, 64);
         a += __shfl_xor(a, 8, 64);
         a += __shfl_xor(a, 4, 64);
@@ -219,7 +230,23 @@ __global__ __launch_bounds__(256) void v1_fwd(int32_t *__restrict__ dst,
         a += __shfl_xor(a, 1, 64);
         acc[v] = a;
     }
-    if (sl == 0 && valid) {
+    if constexpr (WPR > 1) {
+        // the row's waves are consecutive in the block: add their totals in wave order
+        if (lane == 0) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) part[wave * NV + v] = acc[v];
+        }
+        __syncthreads();
+        if (lane == 0 && wave % WPR == 0 && valid) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                int32_t a = 0;
+#pragma unroll
+                for (int w = 0; w < WPR; w++) a += part[(wave + w) * NV + v];
+                dst[size_t(v) * rows + row] = a;
+            }
+        }
+    } else if ((lane & (S - 1)) == 0 && valid) {
 #pragma unroll
         for (int v = 0; v < NV; v++) dst[size_t(v) * rows + row] = acc[v];
     }
@@ -227,7 +254,7 @@ __global__ __launch_bounds__(256) void v1_fwd(int32_t *__restrict__ dst,
 
 template <int NV, int S, int P, int MODE>
 hipError_t launch_cfg(const V1Args &a, hipStream_t s) {
-    constexpr uint32_t rows_per_block = kWaves * (64 / S);
+    constexpr uint32_t rows_per_block = S > 64 ? kWaves / (S / 64) : kWaves * (64 / S);
     dim3 grid((a.rows + rows_per_block - 1) / rows_per_block), block(256);
     hipLaunchKernelGGL((v1_fwd<NV, S, P, MODE>), grid, block, 0, s, a.dst, a.vecs, a.vec_stride,
                        a.rows, a.cols, a.cursors, int(a.min_value), a.G, a.lut, a.freq,
@@ -241,6 +268,8 @@ hipError_t launch_v1_nv(const V1Args &a, hipStream_t s) {
         if (a.streams == 64 && a.precision == 8) return launch_cfg<NV, 64, 8, 1>(a, s);
         return hipErrorInvalidValue;
     }
+    if (a.streams == 128)
+        return a.precision == 8 ? launch_cfg<NV, 128, 8, 0>(a, s) : launch_cfg<NV, 128, 12, 0>(a, s);
     if (a.streams == 64)
         return a.precision == 8 ? launch_cfg<NV, 64, 8, 0>(a, s) : launch_cfg<NV, 64, 12, 0>(a, s);
     return a.precision == 8 ? launch_cfg<NV, 32, 8, 0>(a, s) : launch_cfg<NV, 32, 12, 0>(a, s);

@@ -1,4 +1,4 @@
-// Host side of the MVX1 format extensions: S = 32 or 64 interleaved streams per row,
+// Host side of the MVX1 format extensions: S = 32, 64 or 128 interleaved streams per row,
 // P = 8 or 12 bit probabilities, one table per matrix or one 8-bit table per row.
 //
 // These extend the reference's WarpPerRow codec (create_matrices_WarpPerRow.py:131-272,
@@ -62,7 +62,7 @@ int quantize_counts(const uint64_t *counts, uint32_t n_sym, uint32_t total, uint
 }
 
 bool valid_params(uint32_t streams, uint32_t precision, uint32_t table_mode) {
-    return (streams == 32 || streams == 64) && (precision == 8 || precision == 12) &&
+    return (streams == 32 || streams == 64 || streams == 128) && (precision == 8 || precision == 12) &&
            (table_mode == 0 || (table_mode == 1 && precision == 8));
 }
 

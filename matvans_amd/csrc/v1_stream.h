@@ -3,8 +3,9 @@
 //
 // Layout: a row is K granules of 512 bytes; each of the row's S streams owns GW = 256/S
 // consecutive u16 words of every granule (GW = 8 for S = 32, the reference layout of
-// create_matrices_WarpPerRow.py:405-422; GW = 4 for S = 64).  Word j of stream l is at
-// u16 index cursor + 256*(j/GW) + GW*l + j%GW; words 0 and 1 are the initial state.
+// create_matrices_WarpPerRow.py:405-422; GW = 4 for S = 64; GW = 2 for S = 128).
+// Word j of stream l is at u16 index cursor + 256*(j/GW) + GW*l + j%GW; words 0 and 1 are
+// the initial state.
 //
 // Decode (P-bit probabilities, 32-bit state, 16-bit words):
 //   q = x & (2^P - 1); x = (x >> P) * f[q] + bias[q]; if x < 2^16: x = x << 16 | word
@@ -13,15 +14,17 @@
 // renormalised state is >= 2^24 and the next step leaves it >= 2^16.  P = 12 keeps x
 // plain and reads the ring after every step (two renormalisations can follow each other).
 //
-// Ring: 64 words per lane, lane-interleaved u16 (word slot w of lane i at
-// (w*64 + i)*2 bytes of the wave's ring), conflict-free for reads and writes.  Every
+// Ring: R words per lane (a power of two), lane-interleaved u16 (word slot w of lane i
+// at (w*64 + i)*2 bytes of the wave's ring), conflict-free for reads and writes.  Every
 // 16-step group consumes at most M words (8 for P = 8, 16 for P = 12), and NG = M/GW
 // granules are written per group, loaded two groups ahead.  A granule is loaded only when
 // it will fit (the read index only grows); otherwise an out-of-range buffer offset
 // returns zeros without touching memory and the write goes to dummy rows, so the number
-// of loads in flight is fixed and the compiler's wait counters stay exact.  With 64 ring
-// words, a blocked load implies at least 64 - GW - 2M > M + 1 words are already in the
-// ring, so the group that follows never reads an unwritten slot.
+// of loads in flight is fixed and the compiler's wait counters stay exact.
+// No unwritten slot is ever read: a load blocked at group t-2 means the words written by
+// group t exceed j(t-2) + R - GW >= j(t) - 2M + R - GW, i.e. reach j(t) + M + 1 whenever
+// R >= 3M + GW (R = 32 for P = 8, R = 64 for P = 12); a group whose loads all went
+// through adds M words, at least what the previous group consumed.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -30,10 +33,7 @@
 namespace mv {
 namespace v1 {
 
-constexpr int kRingWords = 64;
-constexpr uint32_t kRingWaveBytes = kRingWords * 64 * 2;   // 8 KiB per wave
 constexpr uint32_t kOOB = 0x80000000u;
-constexpr uint32_t kDummyWords = 16;   // dummy rows (u16 x 64 lanes each) per wave
 constexpr uint32_t kSelRenorm8 = 0x00050401u;   // rotr(x1 << 16 | word, 8)
 constexpr uint32_t kSelKeep8 = 0x04070605u;     // rotr(x1, 8)
 constexpr uint32_t kSelRenorm12 = 0x05040100u;  // x1 << 16 | word
@@ -49,6 +49,10 @@ template <>
 struct Chunk<4> {
     uint32_t d[2];
 };
+template <>
+struct Chunk<2> {
+    uint32_t d[1];
+};
 
 template <int GW>
 __device__ __forceinline__ Chunk<GW> load_chunk(__amdgpu_buffer_rsrc_t r, uint32_t off) {
@@ -56,18 +60,23 @@ __device__ __forceinline__ Chunk<GW> load_chunk(__amdgpu_buffer_rsrc_t r, uint32
     if constexpr (GW == 8) {
         auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
         c.d[0] = v[0]; c.d[1] = v[1]; c.d[2] = v[2]; c.d[3] = v[3];
-    } else {
+    } else if constexpr (GW == 4) {
         auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
         c.d[0] = v[0]; c.d[1] = v[1];
+    } else {
+        c.d[0] = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
     }
     return c;
 }
 
-template <int GW, int P>
+template <int GW, int P, int R>
 struct Lane {
+    static constexpr int kRingWords = R;
+    static constexpr uint32_t kRingWaveBytes = R * 64 * 2;
     static constexpr int M = (P == 8) ? 8 : 16;     // max words per 16-step group
+    static_assert(R >= 3 * M + GW && (R & (R - 1)) == 0, "ring too small");
     static constexpr int NG = M / GW;               // granules written per group
-    static constexpr int kPre = 48 / GW;            // granules loaded by the prologue
+    static constexpr int kPre = (R * 3 / 4) / GW;   // granules loaded by the prologue
 
     __amdgpu_buffer_rsrc_t rsrc;
     uint32_t lane_off;   // byte offset of this lane's slice in granule 0

@@ -1,4 +1,4 @@
-"""CPU oracle for the MVX1 format extensions: S interleaved streams per row (32 or 64),
+"""CPU oracle for the MVX1 format extensions: S interleaved streams per row (32, 64, 128),
 P-bit probabilities (8 or 12) and per-row frequency tables.
 
 TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
@@ -25,7 +25,8 @@ Codec, generalised (32-bit state x, 16-bit words, P-bit quantiles):
                   if x < 2^16: x = (x << 16) | next word
 Stream l of a row decodes columns 16*S*(k // 16) + 16*l + k % 16 (k = 0 .. cols/S - 1).
 A row's payload is K granules of 512 bytes; in each granule every stream owns
-GW = 256 / S consecutive words (8 for S = 32, as in the reference layout; 4 for S = 64),
+GW = 256 / S consecutive words (8 for S = 32, as in the reference layout; 4 for S = 64;
+2 for S = 128),
 so stream l's word j sits at u16 index cursor + 256*(j // GW) + GW*l + j % GW.
 
 MVX1 container (little-endian):
@@ -170,7 +171,7 @@ def encode_rows(matrix, streams, precision, min_value, cdf=None, freq=None):
     row (``freq``).  Returns (cursors uint32[rows], payload uint16)."""
     m = np.asarray(matrix, np.int8)
     rows, cols = m.shape
-    assert streams in (32, 64) and cols % (16 * streams) == 0
+    assert streams in (32, 64, 128) and cols % (16 * streams) == 0
     GW = 256 // streams
     out, cursors, pos = [], [], 0
     for r in range(rows):

@@ -15,7 +15,8 @@ from .helpers import gaussian_int8, gaussian_vec, laplace_int8, uniform_vec
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = [(32, 8, 0), (64, 8, 0), (32, 12, 0), (64, 12, 0), (64, 8, 1)]
+VARIANTS = [(32, 8, 0), (64, 8, 0), (128, 8, 0), (32, 12, 0), (64, 12, 0), (128, 12, 0),
+            (64, 8, 1)]
 
 
 def note(msg):
@@ -76,7 +77,8 @@ def test_edge_alphabets(cuda, S, P, mode):
 LLM = [(4096, 14336, "l", 4.0), (14336, 4096, "l", 4.0), (4096, 4096, "g", 6.0)]
 
 
-@pytest.mark.parametrize("S,P,mode", [(64, 8, 0), (64, 12, 0), (64, 8, 1), (32, 12, 0)])
+@pytest.mark.parametrize("S,P,mode", [(128, 8, 0), (64, 8, 0), (64, 12, 0), (64, 8, 1),
+                                      (32, 12, 0)])
 @pytest.mark.parametrize("rows,cols,kind,p", LLM)
 def test_llm_shapes(cuda, S, P, mode, rows, cols, kind, p):
     note(f"v1 LLM S={S} P={P} mode={mode} {rows}x{cols} {kind}{p}")

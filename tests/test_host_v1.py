@@ -9,7 +9,8 @@ from oracle import ext as E
 
 from .helpers import gaussian_int8, laplace_int8
 
-VARIANTS = [(32, 8, 0), (64, 8, 0), (32, 12, 0), (64, 12, 0), (64, 8, 1)]
+VARIANTS = [(32, 8, 0), (64, 8, 0), (128, 8, 0), (32, 12, 0), (64, 12, 0), (128, 12, 0),
+            (64, 8, 1)]
 
 
 def _mats():

@@ -12,7 +12,8 @@ from oracle import oracle as O
 
 from .helpers import gaussian_int8, laplace_int8
 
-VARIANTS = [(32, 8, 0), (64, 8, 0), (32, 12, 0), (64, 12, 0), (64, 8, 1)]
+VARIANTS = [(32, 8, 0), (64, 8, 0), (128, 8, 0), (32, 12, 0), (64, 12, 0), (128, 12, 0),
+            (64, 8, 1)]
 
 
 def _cases():
@@ -75,7 +76,7 @@ def test_round_trip(S, P, mode, name):
 def test_textbook_decode_of_single_streams(S, P, mode):
     """Gather one stream's words with the layout formula and decode them with a plain
     pop loop: the symbols are the row's columns 16*S*(k//16) + 16*l + k%16."""
-    m = CASES["laplace4"][:, :2048] if S == 64 else CASES["laplace4"]
+    m = CASES["laplace4"][:, :2048] if S >= 64 else CASES["laplace4"]
     r = E.encode_record(m, S, P, mode)
     GW = 256 // S
     for row in (0, m.shape[0] - 1):
