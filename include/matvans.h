@@ -293,6 +293,70 @@ int matvans_v1_decompress_and_multiply(const matvans_v1_dev_matrix *m, int32_t *
                                        const int8_t *vecs, uint32_t n_vec,
                                        uint32_t vec_stride, void *stream);
 
+/* ------------------------------------------------------------------------------
+ * A13. The reference's family-B codec ("buffered"): quad rANS with 7-bit probabilities,
+ * 64-bit coder states and 32-bit words, 32 coders per warp sharing one word stream.
+ * Layouts ans-warp-rows (a warp is one row; compress_ans_warp_rows, buffered/encoder.py:
+ * 381-430) and ans-warp-stripes (a warp is 32 consecutive rows, one coder per row;
+ * compress_ans_warp_stripes, encoder.py:323-378).  Files are safetensors containers
+ * (compress_all_matrices, encoder.py:474-510).
+ * ----------------------------------------------------------------------------*/
+
+/* Fused decode x int8 matvec, the drop-in for the kernels
+ *   mat_vec_ans_warp_per_row(uint4* out_vec, const uint4* compressed,
+ *       const uint32_t* warp_offsets, const uint32_t* ppf, const uint32_t* ppf_cum,
+ *       const uint32_t* ppf_prob, const uint4* in_vec, int in_dim, int out_dim,
+ *       float scale)                       (buffered/ans-warp-rows-kernel.cu:7-17)
+ *   mat_vec_ans_thread_per_row(same)      (buffered/ans-warp-stripes-kernel.cu:21-31)
+ * Same arguments and meaning (device pointers; warp_offsets in 16-byte units; the three
+ * tables 32 words of 4 packed bytes), plus:
+ *   compressed_words  size of `compressed` in u32 words (every read stays inside),
+ *   end_offset        end of this matrix's data in 16-byte units (the next matrix's first
+ *                     offset, or compressed_words / 4), bounding its last warp,
+ *   n_vec, vec_stride 1..8 vectors of in_dim int8, vec_stride bytes apart (4-aligned),
+ *   out8              out8[v*out_dim + r] = clamp(rint(float(acc) * scale), -128, 127)
+ *                     as the reference's epilogue (:212-220), or NULL,
+ *   acc32             the int32 sums themselves, or NULL (at least one of out8/acc32).
+ * Preconditions: rows layout in_dim % 128 == 0; stripes out_dim % 32 == 0, in_dim % 4 == 0. */
+int matvans_b_rows_matvec(const uint32_t *compressed, uint64_t compressed_words,
+                          const uint32_t *warp_offsets, uint32_t end_offset,
+                          const uint32_t *ppf, const uint32_t *ppf_cum, const uint32_t *ppf_prob,
+                          const int8_t *vecs, uint32_t n_vec, uint32_t vec_stride,
+                          uint32_t in_dim, uint32_t out_dim, float scale, int8_t *out8,
+                          int32_t *acc32, void *stream);
+int matvans_b_stripes_matvec(const uint32_t *compressed, uint64_t compressed_words,
+                             const uint32_t *warp_offsets, uint32_t end_offset,
+                             const uint32_t *ppf, const uint32_t *ppf_cum,
+                             const uint32_t *ppf_prob, const int8_t *vecs, uint32_t n_vec,
+                             uint32_t vec_stride, uint32_t in_dim, uint32_t out_dim,
+                             float scale, int8_t *out8, int32_t *acc32, void *stream);
+
+/* 7-bit model (fit_128bit_encoder_model, encoder.py:268-306, float32 rounding and
+ * first-maximum adjustment as numpy) and the decoder tables
+ * (decoder_model_from_128bit_encoder_model, :309-320) packed 4 bytes per word
+ * (pack_4bytes, :465-471).  cdf receives n_symbols + 1 <= 129 bytes.  More than 128
+ * distinct values return MATVANS_E_RANGE (the reference divides by zero there). */
+int matvans_b_model(const int8_t *data, uint64_t n, int32_t *min_value, uint32_t *n_symbols,
+                    uint8_t *cdf, uint32_t *ppf, uint32_t *ppf_cum, uint32_t *ppf_prob);
+/* Upper bound on the words matvans_b_encode writes; layout 0 rows, 1 stripes. */
+uint64_t matvans_b_encode_bound(uint32_t rows, uint32_t cols, int layout);
+/* Encode one matrix, word for word as encoder.py (the 32 coders carry their states from
+ * warp to warp).  warp_offsets[rows or rows/32] = offset_base + start in 16-byte units. */
+int matvans_b_encode(const int8_t *mat, uint32_t rows, uint32_t cols, int layout,
+                     int32_t min_value, const uint8_t *cdf, uint32_t n_symbols,
+                     uint32_t offset_base, uint32_t *compressed, uint64_t cap_words,
+                     uint64_t *n_words, uint32_t *warp_offsets);
+
+/* Minimal safetensors reader (the container of buffered/encoder.py:498-510, read by
+ * buffered/sane_safetensors.hpp:17-68): a u64 header length, a JSON header of
+ * {name: {dtype, shape, data_offsets}}, then the data.  matvans_st_tensor returns a view
+ * (dtype string such as "U32", up to 4 dimensions, data pointer and byte size). */
+typedef struct matvans_st matvans_st;
+int matvans_st_read(const char *path, matvans_st **out);
+void matvans_st_free(matvans_st *st);
+int matvans_st_tensor(const matvans_st *st, const char *name, const char **dtype,
+                      uint32_t *ndim, uint64_t *shape, const void **data, uint64_t *nbytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -95,6 +95,17 @@ SIGNATURES = {
     "matvans_v1_matrix": (i32, [P, u32, ctypes.POINTER(V1Host)]),
     "matvans_v1_write": (i32, [ctypes.c_char_p, u32, P, u32, u32, ctypes.POINTER(V1Host)]),
     "matvans_v1_decompress_and_multiply": (i32, [ctypes.POINTER(V1DevMatrix), P, P, u32, u32, P]),
+    "matvans_b_rows_matvec": (i32, [P, u64, P, u32, P, P, P, P, u32, u32, u32, u32, f32, P, P, P]),
+    "matvans_b_stripes_matvec": (i32, [P, u64, P, u32, P, P, P, P, u32, u32, u32, u32, f32, P, P,
+                                       P]),
+    "matvans_b_model": (i32, [P, u64, ctypes.POINTER(i32), ctypes.POINTER(u32), P, P, P, P]),
+    "matvans_b_encode_bound": (u64, [u32, u32, i32]),
+    "matvans_b_encode": (i32, [P, u32, u32, i32, i32, P, u32, u32, P, u64, ctypes.POINTER(u64),
+                               P]),
+    "matvans_st_read": (i32, [ctypes.c_char_p, ctypes.POINTER(P)]),
+    "matvans_st_free": (None, [P]),
+    "matvans_st_tensor": (i32, [P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p),
+                                ctypes.POINTER(u32), P, ctypes.POINTER(P), ctypes.POINTER(u64)]),
 }
 
 _LIB = None

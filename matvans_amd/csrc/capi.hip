@@ -122,6 +122,54 @@ extern "C" int matvans_v1_decompress_and_multiply(const matvans_v1_dev_matrix *m
     return hip_status(mv::launch_v1_decode_matvec(a, as_stream(stream)), "v1 decode_matvec launch");
 }
 
+namespace {
+int b_matvec(int layout, const uint32_t *compressed, uint64_t compressed_words,
+             const uint32_t *warp_offsets, uint32_t end_offset, const uint32_t *ppf,
+             const uint32_t *ppf_cum, const uint32_t *ppf_prob, const int8_t *vecs,
+             uint32_t n_vec, uint32_t vec_stride, uint32_t in_dim, uint32_t out_dim, float scale,
+             int8_t *out8, int32_t *acc32, void *stream) {
+    if (!compressed || !warp_offsets || !ppf || !ppf_cum || !ppf_prob || !vecs || (!out8 && !acc32))
+        return fail(MATVANS_E_INVALID, "null device pointer");
+    if (n_vec < 1 || n_vec > 8) return fail(MATVANS_E_INVALID, "n_vec %u outside 1..8", n_vec);
+    if (out_dim == 0 || in_dim == 0) return fail(MATVANS_E_SHAPE, "empty matrix");
+    if (layout == 0 && in_dim % 128)
+        return fail(MATVANS_E_SHAPE, "ans-warp-rows needs in_dim %% 128 == 0 (in_dim = %u)", in_dim);
+    if (layout == 1 && (out_dim % 32 || in_dim % 4))
+        return fail(MATVANS_E_SHAPE, "ans-warp-stripes needs out_dim %% 32 == 0, in_dim %% 4 == 0");
+    if (compressed_words > 0xFFFFFFF0ull) return fail(MATVANS_E_SHAPE, "compressed data over 2^32 words");
+    if (!aligned(compressed, 16)) return fail(MATVANS_E_ALIGN, "compressed data not 16-byte aligned");
+    if (n_vec == 1 && vec_stride < in_dim) vec_stride = in_dim;
+    // the kernels read the vectors as 32-bit words
+    if (!aligned(vecs, 4) || vec_stride % 4)
+        return fail(MATVANS_E_ALIGN, "vectors must be 4-byte aligned (stride %u)", vec_stride);
+    if (vec_stride < in_dim) return fail(MATVANS_E_INVALID, "vec_stride %u < in_dim %u", vec_stride, in_dim);
+    mv::BArgs a{layout, acc32, out8, scale, compressed, uint32_t(compressed_words), warp_offsets,
+                end_offset, ppf, ppf_cum, ppf_prob, vecs, n_vec, vec_stride, in_dim, out_dim};
+    return hip_status(mv::launch_b_decode_matvec(a, as_stream(stream)), "family-B decode_matvec launch");
+}
+}  // namespace
+
+extern "C" int matvans_b_rows_matvec(const uint32_t *compressed, uint64_t compressed_words,
+                                     const uint32_t *warp_offsets, uint32_t end_offset,
+                                     const uint32_t *ppf, const uint32_t *ppf_cum,
+                                     const uint32_t *ppf_prob, const int8_t *vecs, uint32_t n_vec,
+                                     uint32_t vec_stride, uint32_t in_dim, uint32_t out_dim,
+                                     float scale, int8_t *out8, int32_t *acc32, void *stream) {
+    return b_matvec(0, compressed, compressed_words, warp_offsets, end_offset, ppf, ppf_cum,
+                    ppf_prob, vecs, n_vec, vec_stride, in_dim, out_dim, scale, out8, acc32, stream);
+}
+
+extern "C" int matvans_b_stripes_matvec(const uint32_t *compressed, uint64_t compressed_words,
+                                        const uint32_t *warp_offsets, uint32_t end_offset,
+                                        const uint32_t *ppf, const uint32_t *ppf_cum,
+                                        const uint32_t *ppf_prob, const int8_t *vecs,
+                                        uint32_t n_vec, uint32_t vec_stride, uint32_t in_dim,
+                                        uint32_t out_dim, float scale, int8_t *out8,
+                                        int32_t *acc32, void *stream) {
+    return b_matvec(1, compressed, compressed_words, warp_offsets, end_offset, ppf, ppf_cum,
+                    ppf_prob, vecs, n_vec, vec_stride, in_dim, out_dim, scale, out8, acc32, stream);
+}
+
 extern "C" int matvans_decompress_and_mult(const matvans_dev_matrix *m, int8_t *d_result8,
                                            int32_t *d_result32, const int8_t *d_vector,
                                            float *d_delta, void *stream) {

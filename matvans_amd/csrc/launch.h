@@ -38,6 +38,22 @@ struct V1Args {
 
 hipError_t launch_v1_decode_matvec(const V1Args &a, hipStream_t s);
 
+struct BArgs {
+    int layout;              // 0 ans-warp-rows, 1 ans-warp-stripes
+    int32_t *acc32;          // [n_vec][out_dim] or null
+    int8_t *out8;            // [n_vec][out_dim] or null: clamp(rint(float(acc) * scale))
+    float scale;
+    const uint32_t *compressed;
+    uint32_t compressed_words;
+    const uint32_t *warp_offsets;   // uint4 units
+    uint32_t end_offset;            // end of this matrix's data, uint4 units
+    const uint32_t *ppf, *ppf_cum, *ppf_prob;   // 32 words each (4 entries per word)
+    const int8_t *vecs;
+    uint32_t n_vec, vec_stride, in_dim, out_dim;
+};
+
+hipError_t launch_b_decode_matvec(const BArgs &a, hipStream_t s);
+
 struct V0BwdArgs {
     float *dx;               // [n_vec][cols]
     float *partial;          // workspace [n_vec][ceil(rows/64)][cols]
