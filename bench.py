@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--no-compare", action="store_true", help="skip the comparators")
     ap.add_argument("--only", default=None, choices=[None, "fused", "int8", "f32", "copy"],
                     help="run one kernel only (for profiling); prints its timing")
-    ap.add_argument("--formats", default=",".join(FORMATS),
+    ap.add_argument("--formats", default=",".join(DEFAULT_FORMATS),
                     help="comma list of encodings to time; the fastest is the headline")
     return ap.parse_args()
 
@@ -54,6 +54,11 @@ FORMATS = {
     "s64-p12": (64, 12, 0),
     "s64-p8-rowcdf": (64, 8, 1),
     "s32-p12": (32, 12, 0),
+    "q32": (32, 8, 0, 1),
+    "q64": (64, 8, 0, 1),
+    "q128": (128, 8, 0, 1),
+    "b-rows": "rows",
+    "b-stripes": "stripes",
 }
 FORMAT_TEXT = {
     "v0": ".bin v0 (reference layout): 32 streams/row, global 8-bit CDF",
@@ -62,7 +67,14 @@ FORMAT_TEXT = {
     "s64-p12": "MVX1: 64 streams/row, global 12-bit CDF",
     "s64-p8-rowcdf": "MVX1: 64 streams/row, per-row 8-bit CDF",
     "s32-p12": "MVX1: 32 streams/row, global 12-bit CDF",
+    "q32": "MVX1 quad coder: 32 streams/row, 64-bit state, global 8-bit CDF",
+    "q64": "MVX1 quad coder: 64 streams/row, 64-bit state, global 8-bit CDF",
+    "q128": "MVX1 quad coder: 128 streams/row, 64-bit state, global 8-bit CDF",
+    "b-rows": "family B (buffered) ans-warp-rows: quad rANS, 7-bit CDF, 32 coders/row",
+    "b-stripes": "family B (buffered) ans-warp-stripes: quad rANS, 7-bit CDF, 32 rows/warp",
 }
+DEFAULT_FORMATS = ["v0", "q32", "q64", "q128", "s128-p8", "s64-p8", "s64-p12", "s64-p8-rowcdf",
+                   "s32-p12", "b-rows"]
 
 
 def gen(rows, cols, dist, seed=20250310):
@@ -148,6 +160,7 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"kernel": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n}))
         return
+    from matvans_amd.buffered import BufferedMatrices
     from matvans_amd.mvx import CompressedMatrixV1
     rows, cols, nvec = args.rows, args.cols, args.nvec
     w, x = gen(rows, cols, args.dist)
@@ -157,20 +170,38 @@ def main():
     out = torch.empty((nvec, rows), dtype=torch.int32, device=dev)
     want = (w.astype(np.int64) @ vecs.astype(np.int64).T).T.astype(np.int32)
 
+    class BMat:
+        """one family-B matrix on the device, with the multiply() of the other formats"""
+
+        def __init__(self, bm):
+            self.bm = bm.to_device(dev)
+
+        def multiply(self, v, out):
+            return self.bm.mat_vec(0, v, acc32=out)
+
     def encode(fmt):
         """(host matrix, stored bytes, maker of one device copy)"""
+        if isinstance(FORMATS[fmt], str):
+            bm = BufferedMatrices.encode([w], FORMATS[fmt])
+            stored = bm.compressed.nbytes + bm.offsets[0].nbytes + 3 * 128
+            return bm, stored, lambda: BMat(BufferedMatrices(
+                bm.layout, bm.compressed, bm.offsets, bm.ppfs, bm.cums, bm.probs, bm.scales,
+                shapes=bm.shapes))
         if FORMATS[fmt] is None:
             return cm, cm.payload_size * 2 + rows * 4 + (cm.G + 1) + 256, lambda: mv.CompressedMatrix(
                 cm.rows, cm.cols, 1.0, cm.cursors, cm.min_value, cm.G, cm.cdf_data, cm.ppf_data,
                 cm.payload).to_device(dev)
-        S, P, mode = FORMATS[fmt]
-        c = CompressedMatrixV1.encode(w, S, P, mode)
+        S, P, mode, coder = (FORMATS[fmt] + (0,))[:4]
+        c = CompressedMatrixV1.encode(w, S, P, mode, coder=coder)
         return c, c.stored_bytes, lambda: CompressedMatrixV1(
             c.rows, c.cols, S, P, mode, c.min_value, c.G, c.cursors, c.payload, c.cdf,
-            c.freq).to_device(dev)
+            c.freq, coder=coder).to_device(dev)
 
     def time_format(fmt):
-        host, stored, make = encode(fmt)
+        try:
+            host, stored, make = encode(fmt)
+        except mv.MatvansError as e:
+            return {"unsupported": str(e)}
         # distinct device copies (more than twice the MALL) so repeats come from HBM
         n_copies = min(64, max(2, -(-2 * MALL_BYTES // stored)))
         copies = [make() for _ in range(n_copies)]
@@ -194,10 +225,11 @@ def main():
             sys.exit(2)
     if args.only == "fused":
         r = time_format(fmts[0])
-        print(json.dumps({"kernel": "fused", "format": fmts[0], "ms": r["us"] / 1e3}))
+        print(json.dumps({"kernel": "fused", "format": fmts[0], "ms": r.get("us", 0) / 1e3,
+                          "result": r}))
         return
     per_format = {f: time_format(f) for f in fmts}
-    best = min(per_format, key=lambda f: per_format[f]["us"])
+    best = min((f for f in per_format if "us" in per_format[f]), key=lambda f: per_format[f]["us"])
     ms_fused = per_format[best]["us"] / 1e3
     n_copies = per_format[best]["copies"]
     copies = [encode("v0")[2]() for _ in range(min(64, max(2, -(-2 * MALL_BYTES // (

@@ -226,12 +226,20 @@ int matvans_encode_rows(const int8_t *mat, uint32_t rows, uint32_t cols, int8_t 
  *              x = (x >> P)*f + q - cdf[s]; renormalise with a 16-bit word below 2^16).
  *   table_mode 0: one table per matrix, u16 cdf[G+1] (cdf[G] = 2^P);
  *              1: one 8-bit table per row (P = 8 only), u8 freq[rows][G] summing to 256.
- * S = 32, P = 8, table_mode 0 encodes exactly as the reference generator.
+ *   coder      0: one symbol per step (32-bit state, 16-bit words, as the reference);
+ *              1: quad coder (P = 8, table_mode 0): 64-bit state x, 32-bit words, a step
+ *              decodes the four bytes of the low word of x (if x < 2^32 first
+ *              x = x << 32 | next word; then x = (x >> 32)*f0 f1 f2 f3 + r0 + f0(r1 +
+ *              f1(r2 + f2 r3)), r_i = q_i - cdf[s_i]) -- the family-B quad coder of
+ *              buffered/encoder.py:198-237 with 8-bit quantiles and one stream per coder.
+ *              Words are u32 (two u16 of the payload): word j of stream l at u16 index
+ *              cursor + 256*(j/(128/S)) + 2*((128/S)*l + j%(128/S)).
+ * S = 32, P = 8, table_mode 0, coder 0 encodes exactly as the reference generator.
  * ----------------------------------------------------------------------------*/
 typedef struct {
     uint32_t rows, cols;
     float grid_spacing;
-    uint32_t streams, precision, table_mode;
+    uint32_t streams, precision, table_mode, coder;
     int32_t min_value;          /* symbols min_value .. min_value + G - 1 */
     uint32_t G;
     uint32_t payload_size;      /* u16 words, a multiple of 256 */
@@ -251,10 +259,10 @@ uint64_t matvans_v1_encode_bound(uint32_t rows, uint32_t cols, uint32_t streams,
                                  uint32_t precision);
 /* Encode rows x cols (cols % (16*streams) == 0), threaded over rows. */
 int matvans_v1_encode(const int8_t *mat, uint32_t rows, uint32_t cols, uint32_t streams,
-                      uint32_t precision, uint32_t table_mode, int32_t min_value, uint32_t G,
-                      const uint16_t *cdf, const uint8_t *freq, uint32_t n_threads,
-                      uint32_t *cursors, uint16_t *payload, uint64_t payload_cap,
-                      uint64_t *payload_size);
+                      uint32_t precision, uint32_t table_mode, uint32_t coder,
+                      int32_t min_value, uint32_t G, const uint16_t *cdf, const uint8_t *freq,
+                      uint32_t n_threads, uint32_t *cursors, uint16_t *payload,
+                      uint64_t payload_cap, uint64_t *payload_size);
 /* Decode table of a table_mode 0 matrix, 2 * 2^P entries: A[q] = f | (uint8)(min+s) << 24,
  * then B[q] = q - cdf[s]. */
 int matvans_v1_lut(const uint16_t *cdf, uint32_t G, int32_t min_value, uint32_t precision,
@@ -263,9 +271,10 @@ int matvans_v1_lut(const uint16_t *cdf, uint32_t G, int32_t min_value, uint32_t 
 typedef struct matvans_v1_file matvans_v1_file;
 /* MVX1 container: "MVX1", u32 version 1, u32 num_matrices, u32 result_hash, u32 len_v,
  * i8 vector[len_v] (zero pad to 4 bytes), then per matrix the fields of matvans_v1_host
- * in order (rows, cols, grid_spacing, streams, precision, table_mode, min_value, G,
- * payload_size as 4-byte words), cursors, the table (zero pad to 4), the payload (zero
- * pad to 4).  The reader validates lengths, parameters, tables and cursors. */
+ * as 4-byte words, cursors, the table (zero pad to 4), the payload (zero
+ * pad to 4); the record header is rows, cols, grid_spacing, streams, precision,
+ * table_mode, coder, min_value, G, payload_size.  The reader validates lengths,
+ * parameters, tables and cursors. */
 int matvans_v1_read(const char *path, matvans_v1_file **out);
 void matvans_v1_free(matvans_v1_file *f);
 int matvans_v1_header(const matvans_v1_file *f, uint32_t *num_matrices, uint32_t *result_hash,
@@ -276,7 +285,7 @@ int matvans_v1_write(const char *path, uint32_t result_hash, const int8_t *vecto
 
 /* Device view of an MVX1 matrix (all pointers device memory). */
 typedef struct {
-    uint32_t rows, cols, streams, precision, table_mode;
+    uint32_t rows, cols, streams, precision, table_mode, coder;
     int32_t min_value;
     uint32_t G;
     uint32_t payload_size;      /* u16 words */

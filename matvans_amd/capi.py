@@ -41,15 +41,16 @@ class HostMatrix(ctypes.Structure):
 class V1Host(ctypes.Structure):
     """matvans_v1_host."""
     _fields_ = [("rows", u32), ("cols", u32), ("grid_spacing", f32), ("streams", u32),
-                ("precision", u32), ("table_mode", u32), ("min_value", i32), ("G", u32),
-                ("payload_size", u32), ("cursors", P), ("cdf", P), ("freq", P), ("payload", P)]
+                ("precision", u32), ("table_mode", u32), ("coder", u32), ("min_value", i32),
+                ("G", u32), ("payload_size", u32), ("cursors", P), ("cdf", P), ("freq", P),
+                ("payload", P)]
 
 
 class V1DevMatrix(ctypes.Structure):
     """matvans_v1_dev_matrix."""
     _fields_ = [("rows", u32), ("cols", u32), ("streams", u32), ("precision", u32),
-                ("table_mode", u32), ("min_value", i32), ("G", u32), ("payload_size", u32),
-                ("cursors", P), ("lut", P), ("freq", P), ("payload", P)]
+                ("table_mode", u32), ("coder", u32), ("min_value", i32), ("G", u32),
+                ("payload_size", u32), ("cursors", P), ("lut", P), ("freq", P), ("payload", P)]
 
 
 # name -> (restype, argtypes); status-returning functions use i32
@@ -85,8 +86,8 @@ SIGNATURES = {
     "matvans_v1_model": (i32, [P, u32, u32, u32, u32, ctypes.POINTER(i32), ctypes.POINTER(u32),
                                P, P]),
     "matvans_v1_encode_bound": (u64, [u32, u32, u32, u32]),
-    "matvans_v1_encode": (i32, [P, u32, u32, u32, u32, u32, i32, u32, P, P, u32, P, P, u64,
-                                ctypes.POINTER(u64)]),
+    "matvans_v1_encode": (i32, [P, u32, u32, u32, u32, u32, u32, i32, u32, P, P, u32, P, P,
+                                u64, ctypes.POINTER(u64)]),
     "matvans_v1_lut": (i32, [P, u32, i32, u32, P]),
     "matvans_v1_read": (i32, [ctypes.c_char_p, ctypes.POINTER(P)]),
     "matvans_v1_free": (None, [P]),

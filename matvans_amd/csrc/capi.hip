@@ -104,6 +104,9 @@ extern "C" int matvans_v1_decompress_and_multiply(const matvans_v1_dev_matrix *m
                     m->precision, m->table_mode);
     if (m->table_mode == 1 && (m->streams != 64 || m->precision != 8))
         return fail(MATVANS_E_UNSUPPORTED, "one table per row needs 64 streams and 8-bit tables");
+    if (m->coder > 1) return fail(MATVANS_E_INVALID, "coder %u", m->coder);
+    if (m->coder == 1 && (m->precision != 8 || m->table_mode != 0))
+        return fail(MATVANS_E_UNSUPPORTED, "the quad coder needs 8-bit probabilities and one table");
     if (m->table_mode == 0 && !m->lut) return fail(MATVANS_E_INVALID, "null decode table");
     if (m->table_mode == 1 && (!m->freq || m->G < 2 || m->G > 256))
         return fail(MATVANS_E_INVALID, "per-row tables need freq and 2 <= G <= 256");
@@ -117,7 +120,7 @@ extern "C" int matvans_v1_decompress_and_multiply(const matvans_v1_dev_matrix *m
         return fail(MATVANS_E_ALIGN, "vectors must be 16-byte aligned (stride %u)", vec_stride);
     if (vec_stride < m->cols) return fail(MATVANS_E_INVALID, "vec_stride %u < cols %u", vec_stride, m->cols);
     mv::V1Args a{dst, vecs, n_vec, vec_stride, m->rows, m->cols, m->streams, m->precision,
-                 m->table_mode, m->min_value, m->G, m->cursors, m->lut, m->freq, m->payload,
+                 m->table_mode, m->coder, m->min_value, m->G, m->cursors, m->lut, m->freq, m->payload,
                  m->payload_size};
     return hip_status(mv::launch_v1_decode_matvec(a, as_stream(stream)), "v1 decode_matvec launch");
 }

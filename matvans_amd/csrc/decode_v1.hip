@@ -252,6 +252,139 @@ __global__ __launch_bounds__(256) void v1_fwd(int32_t *__restrict__ dst,
     }
 }
 
+// Quad coder (coder 1, P = 8, one table per matrix): same mapping, per-lane 64-bit
+// state; a 16-symbol group is four quads, each one v_dot4 per vector.
+template <int NV, int S>
+__global__ __launch_bounds__(256) void v1q_fwd(int32_t *__restrict__ dst,
+                                               const int8_t *__restrict__ vecs,
+                                               uint32_t vec_stride, uint32_t rows, uint32_t cols,
+                                               const uint32_t *__restrict__ cursors,
+                                               const uint32_t *__restrict__ glut,
+                                               const uint16_t *__restrict__ payload,
+                                               uint32_t payload_size) {
+    constexpr int RPW = S < 64 ? 64 / S : 1;
+    constexpr int WPR = S > 64 ? S / 64 : 1;
+    constexpr int G4 = 128 / S;
+    using QL = v1::QuadLane<G4>;
+    __shared__ uint32_t ring[kWaves * QL::R4 * 64];
+    __shared__ uint32_t lut[512];
+    __shared__ uint32_t dummy[4 * 64];
+    __shared__ int32_t part[WPR > 1 ? kWaves * NV : 1];
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wave = tid >> 6, lane = tid & 63;
+    const uint32_t sl = S >= 64 ? (wave % WPR) * 64 + lane : lane & (S - 1);
+    const uint32_t row0 = S >= 64 ? blockIdx.x * (kWaves / WPR) + wave / WPR
+                                  : (blockIdx.x * kWaves + wave) * RPW;
+    const uint32_t row = row0 + (S >= 64 ? 0u : lane / S);
+    const bool valid = row < rows;
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(glut);
+        uint4 *d = reinterpret_cast<uint4 *>(lut);
+        if (tid < 128) d[tid] = src[tid];
+    }
+    uint32_t wbeg = 0, wend = 0;
+    if (row0 < rows) {
+        wbeg = cursors[row0];
+        uint32_t last = row0 + RPW - 1 < rows ? row0 + RPW - 1 : rows - 1;
+        wend = last + 1 < rows ? cursors[last + 1] : payload_size;
+        if (wend > payload_size) wend = payload_size;
+        if (wend < wbeg) wend = wbeg;
+    }
+    wbeg = __builtin_amdgcn_readfirstlane(wbeg);
+    wend = __builtin_amdgcn_readfirstlane(wend);
+    const uint32_t rcur = valid ? cursors[row] : 0u;
+    const uint32_t rend = valid ? (row + 1 < rows ? cursors[row + 1] : payload_size) : 0u;
+
+    QL st;
+    st.init(payload, wbeg, wend, valid, rcur, rend, sl, ring, dummy, wave, lane);
+    st.prologue();
+
+    const uint32_t groups = cols / (16u * S);
+    auto vload = [&](uint32_t t, uint4 *c) {
+        uint32_t tt = t < groups ? t : groups - 1;
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+            c[v] = *reinterpret_cast<const uint4 *>(vecs + size_t(v) * vec_stride +
+                                                    tt * (16u * S) + sl * 16u);
+    };
+    uint4 vc0[NV], vc1[NV];
+    vload(0, vc0);
+    vload(1, vc1);
+    __syncthreads();
+    st.start();
+
+    int32_t acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) acc[v] = 0;
+    auto group = [&](const uint4 *vc) {
+        uint32_t wbefore = st.waddr;
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            uint32_t packed = st.quad(lut);
+#pragma unroll
+            for (int v = 0; v < NV; v++)
+                acc[v] = __builtin_amdgcn_sdot4(static_cast<int>(packed),
+                                                static_cast<int>(dword_of(vc[v], d)), acc[v],
+                                                false);
+        }
+        st.end_group(wbefore);
+    };
+    uint32_t t = 0;
+    for (; t + 1 < groups; t += 2) {
+        st.refill_a();
+        group(vc0);
+        vload(t + 2, vc0);
+        st.refill_b();
+        group(vc1);
+        vload(t + 3, vc1);
+    }
+    if (t < groups) {
+        st.refill_a();
+        group(vc0);
+    }
+
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        int32_t a = acc[v];
+        if constexpr (S >= 64) a += __shfl_xor(a, 32, 64);
+        a += __shfl_xor(a, 16, 64);
+        a += __shfl_xor(a, 8, 64);
+        a += __shfl_xor(a, 4, 64);
+        a += __shfl_xor(a, 2, 64);
+        a += __shfl_xor(a, 1, 64);
+        acc[v] = a;
+    }
+    if constexpr (WPR > 1) {
+        if (lane == 0) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) part[wave * NV + v] = acc[v];
+        }
+        __syncthreads();
+        if (lane == 0 && wave % WPR == 0 && valid) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                int32_t a = 0;
+#pragma unroll
+                for (int w = 0; w < WPR; w++) a += part[(wave + w) * NV + v];
+                dst[size_t(v) * rows + row] = a;
+            }
+        }
+    } else if ((lane & (S - 1)) == 0 && valid) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) dst[size_t(v) * rows + row] = acc[v];
+    }
+}
+
+template <int NV, int S>
+hipError_t launch_quad(const V1Args &a, hipStream_t s) {
+    constexpr uint32_t rows_per_block = S > 64 ? kWaves / (S / 64) : kWaves * (64 / S);
+    dim3 grid((a.rows + rows_per_block - 1) / rows_per_block), block(256);
+    hipLaunchKernelGGL((v1q_fwd<NV, S>), grid, block, 0, s, a.dst, a.vecs, a.vec_stride, a.rows,
+                       a.cols, a.cursors, a.lut, a.payload, a.payload_size);
+    return hipGetLastError();
+}
+
 template <int NV, int S, int P, int MODE>
 hipError_t launch_cfg(const V1Args &a, hipStream_t s) {
     constexpr uint32_t rows_per_block = S > 64 ? kWaves / (S / 64) : kWaves * (64 / S);
@@ -264,6 +397,12 @@ hipError_t launch_cfg(const V1Args &a, hipStream_t s) {
 
 template <int NV>
 hipError_t launch_v1_nv(const V1Args &a, hipStream_t s) {
+    if (a.coder == 1) {
+        if (a.precision != 8 || a.table_mode != 0) return hipErrorInvalidValue;
+        if (a.streams == 128) return launch_quad<NV, 128>(a, s);
+        if (a.streams == 64) return launch_quad<NV, 64>(a, s);
+        return launch_quad<NV, 32>(a, s);
+    }
     if (a.table_mode == 1) {
         if (a.streams == 64 && a.precision == 8) return launch_cfg<NV, 64, 8, 1>(a, s);
         return hipErrorInvalidValue;

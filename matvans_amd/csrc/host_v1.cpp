@@ -61,9 +61,47 @@ int quantize_counts(const uint64_t *counts, uint32_t n_sym, uint32_t total, uint
     return MATVANS_OK;
 }
 
-bool valid_params(uint32_t streams, uint32_t precision, uint32_t table_mode) {
+bool valid_params(uint32_t streams, uint32_t precision, uint32_t table_mode, uint32_t coder = 0) {
     return (streams == 32 || streams == 64 || streams == 128) && (precision == 8 || precision == 12) &&
-           (table_mode == 0 || (table_mode == 1 && precision == 8));
+           (table_mode == 0 || (table_mode == 1 && precision == 8)) &&
+           (coder == 0 || (coder == 1 && precision == 8 && table_mode == 0));
+}
+
+// Quad coder (coder 1): 64-bit state, 32-bit words, four 8-bit quantiles per step (the
+// family-B quad coder of buffered/encoder.py:198-237 with P = 8 and one stream per
+// coder).  Symbols in reverse decode order, starting from x = 2^32; words out in decode
+// order [x_lo, x_hi, last flushed, ..., first flushed].
+int encode_stream_quad(const int8_t *row, uint32_t cols, uint32_t S, uint32_t lane, int mn,
+                       uint32_t G, const uint32_t *cdf, std::vector<uint32_t> &tmp,
+                       std::vector<uint32_t> &out) {
+    const uint32_t steps = cols / S;
+    tmp.clear();
+    uint64_t x = 1ull << 32;
+    for (int64_t j = int64_t(steps / 4) - 1; j >= 0; j--) {
+        uint32_t f[4], c[4];
+        uint64_t F = 1;
+        for (int i = 0; i < 4; i++) {
+            uint32_t k = uint32_t(4 * j + i);
+            uint32_t col = 16u * S * (k / 16) + 16u * lane + (k % 16);
+            int s = int(row[col]) - mn;
+            if (s < 0 || uint32_t(s) >= G) return -1;
+            c[i] = cdf[s];
+            f[i] = cdf[s + 1] - c[i];
+            if (f[i] == 0 || f[i] > 255) return -1;
+            F *= f[i];
+        }
+        if ((x >> 32) >= F) { tmp.push_back(uint32_t(x)); x >>= 32; }
+        uint64_t concat = 0;
+        for (int i = 0; i < 4; i++) {
+            concat |= (uint64_t(c[i]) + x % f[i]) << (8 * i);
+            x /= f[i];
+        }
+        x = (x << 32) | concat;
+    }
+    tmp.push_back(uint32_t(x >> 32));
+    tmp.push_back(uint32_t(x));
+    out.assign(tmp.rbegin(), tmp.rend());
+    return 0;
 }
 
 // One stream, symbols in reverse decode order; output words in decode order.
@@ -149,21 +187,23 @@ extern "C" int matvans_v1_model(const int8_t *mat, uint32_t rows, uint32_t cols,
 extern "C" uint64_t matvans_v1_encode_bound(uint32_t rows, uint32_t cols, uint32_t streams,
                                             uint32_t precision) {
     if (!valid_params(streams, precision, 0) || !cols) return 0;
-    // a stream's state gains at most P bits per symbol and loses 16 per word
+    // a stream's state gains at most P bits per symbol and loses 16 per word (the quad
+    // coder's 32-bit words count as two)
     const uint64_t gw = 256 / streams;
-    const uint64_t per_lane = (uint64_t(cols / streams) * precision + 15) / 16 + 4;
-    return uint64_t(rows) * 256 * ((per_lane + gw - 1) / gw);
+    const uint64_t per_lane = (uint64_t(cols / streams) * precision + 15) / 16 + 8;
+    return uint64_t(rows) * 256 * ((per_lane + gw - 1) / gw + 1);
 }
 
 extern "C" int matvans_v1_encode(const int8_t *mat, uint32_t rows, uint32_t cols,
                                  uint32_t streams, uint32_t precision, uint32_t table_mode,
-                                 int32_t min_value, uint32_t G, const uint16_t *cdf,
-                                 const uint8_t *freq, uint32_t n_threads, uint32_t *cursors,
-                                 uint16_t *payload, uint64_t payload_cap, uint64_t *payload_size) {
+                                 uint32_t coder, int32_t min_value, uint32_t G,
+                                 const uint16_t *cdf, const uint8_t *freq, uint32_t n_threads,
+                                 uint32_t *cursors, uint16_t *payload, uint64_t payload_cap,
+                                 uint64_t *payload_size) {
     if (!mat || !cursors || !payload || !payload_size) return fail(MATVANS_E_INVALID, "null argument");
-    if (!valid_params(streams, precision, table_mode))
-        return fail(MATVANS_E_UNSUPPORTED, "streams %u, precision %u, table mode %u", streams,
-                    precision, table_mode);
+    if (!valid_params(streams, precision, table_mode, coder))
+        return fail(MATVANS_E_UNSUPPORTED, "streams %u, precision %u, table mode %u, coder %u",
+                    streams, precision, table_mode, coder);
     if (!cols || cols % (16 * streams))
         return fail(MATVANS_E_SHAPE, "cols %u not a multiple of %u", cols, 16 * streams);
     if (G < 2 || G > 256) return fail(MATVANS_E_INVALID, "G = %u", G);
@@ -183,12 +223,14 @@ extern "C" int matvans_v1_encode(const int8_t *mat, uint32_t rows, uint32_t cols
     }
     if (n_threads == 0) n_threads = std::max(1u, std::thread::hardware_concurrency());
     n_threads = std::min(n_threads, std::max(1u, rows));
-    const uint32_t S = streams, GW = 256 / S;
+    const uint32_t S = streams, GW = 256 / S, G4 = 128 / S;
     std::vector<uint32_t> kgr(rows);
-    std::vector<std::vector<uint16_t>> st(size_t(rows) * S);
+    std::vector<std::vector<uint16_t>> st(coder == 0 ? size_t(rows) * S : 0);
+    std::vector<std::vector<uint32_t>> st4(coder == 1 ? size_t(rows) * S : 0);
     std::atomic<int> bad{-1};
     auto pass1 = [&](uint32_t t) {
         std::vector<uint16_t> tmp;
+        std::vector<uint32_t> tmp4;
         std::vector<uint32_t> c(G + 1);
         if (table_mode == 0) for (uint32_t s = 0; s <= G; s++) c[s] = cdf[s];
         for (uint32_t r = t; r < rows; r += n_threads) {
@@ -198,15 +240,26 @@ extern "C" int matvans_v1_encode(const int8_t *mat, uint32_t rows, uint32_t cols
             }
             uint32_t mx = 0;
             for (uint32_t l = 0; l < S; l++) {
-                auto &s = st[size_t(r) * S + l];
-                if (encode_stream(mat + size_t(r) * cols, cols, S, l, precision, min_value, G,
-                                  c.data(), tmp, s)) {
+                int e;
+                size_t len;
+                if (coder == 0) {
+                    auto &s = st[size_t(r) * S + l];
+                    e = encode_stream(mat + size_t(r) * cols, cols, S, l, precision, min_value, G,
+                                      c.data(), tmp, s);
+                    len = s.size();
+                } else {
+                    auto &s = st4[size_t(r) * S + l];
+                    e = encode_stream_quad(mat + size_t(r) * cols, cols, S, l, min_value, G,
+                                           c.data(), tmp4, s);
+                    len = s.size();
+                }
+                if (e) {
                     bad = int(r);
                     return;
                 }
-                mx = std::max<uint32_t>(mx, uint32_t(s.size()));
+                mx = std::max<uint32_t>(mx, uint32_t(len));
             }
-            kgr[r] = (mx + GW - 1) / GW;
+            kgr[r] = coder == 0 ? (mx + GW - 1) / GW : (mx + G4 - 1) / G4;
         }
     };
     std::vector<std::thread> th;
@@ -229,8 +282,17 @@ extern "C" int matvans_v1_encode(const int8_t *mat, uint32_t rows, uint32_t cols
             uint16_t *dst = payload + cursors[r];
             std::memset(dst, 0, size_t(kgr[r]) * 512);
             for (uint32_t l = 0; l < S; l++) {
-                const auto &s = st[size_t(r) * S + l];
-                for (size_t j = 0; j < s.size(); j++) dst[256 * (j / GW) + GW * l + j % GW] = s[j];
+                if (coder == 0) {
+                    const auto &s = st[size_t(r) * S + l];
+                    for (size_t j = 0; j < s.size(); j++) dst[256 * (j / GW) + GW * l + j % GW] = s[j];
+                } else {
+                    const auto &s = st4[size_t(r) * S + l];
+                    for (size_t j = 0; j < s.size(); j++) {
+                        size_t w = 256 * (j / G4) + 2 * (G4 * l + j % G4);
+                        dst[w] = uint16_t(s[j]);
+                        dst[w + 1] = uint16_t(s[j] >> 16);
+                    }
+                }
             }
         }
     };
@@ -315,9 +377,10 @@ extern "C" int matvans_v1_read(const char *path, matvans_v1_file **out) {
         matvans_v1_host m{};
         if (!rd.take(&m.rows, 4) || !rd.take(&m.cols, 4) || !rd.take(&m.grid_spacing, 4) ||
             !rd.take(&m.streams, 4) || !rd.take(&m.precision, 4) || !rd.take(&m.table_mode, 4) ||
-            !rd.take(&m.min_value, 4) || !rd.take(&m.G, 4) || !rd.take(&m.payload_size, 4))
+            !rd.take(&m.coder, 4) || !rd.take(&m.min_value, 4) || !rd.take(&m.G, 4) ||
+            !rd.take(&m.payload_size, 4))
             return bad("truncated record header", k);
-        if (!valid_params(m.streams, m.precision, m.table_mode)) return bad("bad parameters", k);
+        if (!valid_params(m.streams, m.precision, m.table_mode, m.coder)) return bad("bad parameters", k);
         if (m.G < 2 || m.G > 256 || m.min_value < -128 || m.min_value + int(m.G) - 1 > 127)
             return bad("bad alphabet", k);
         if (!m.cols || m.cols % (16 * m.streams)) return bad("cols do not fit the streams", k);
@@ -392,7 +455,7 @@ extern "C" int matvans_v1_write(const char *path, uint32_t result_hash, const in
     if (!path || (len_v && !vector) || (num_matrices && !mats)) return fail(MATVANS_E_INVALID, "null argument");
     for (uint32_t k = 0; k < num_matrices; k++) {
         const auto &m = mats[k];
-        if (!valid_params(m.streams, m.precision, m.table_mode))
+        if (!valid_params(m.streams, m.precision, m.table_mode, m.coder))
             return fail(MATVANS_E_UNSUPPORTED, "matrix %u: bad parameters", k);
         if ((m.table_mode == 0 && !m.cdf) || (m.table_mode == 1 && !m.freq) || !m.payload ||
             (m.rows && !m.cursors))
@@ -411,7 +474,7 @@ extern "C" int matvans_v1_write(const char *path, uint32_t result_hash, const in
         size_t tab_n = m.table_mode == 0 ? size_t(m.G + 1) * 2 : size_t(m.rows) * m.G;
         ok = put(&m.rows, 4) && put(&m.cols, 4) && put(&m.grid_spacing, 4) &&
              put(&m.streams, 4) && put(&m.precision, 4) && put(&m.table_mode, 4) &&
-             put(&m.min_value, 4) && put(&m.G, 4) && put(&m.payload_size, 4) &&
+             put(&m.coder, 4) && put(&m.min_value, 4) && put(&m.G, 4) && put(&m.payload_size, 4) &&
              put(m.cursors, size_t(m.rows) * 4) &&
              put(m.table_mode == 0 ? static_cast<const void *>(m.cdf) : m.freq, tab_n) &&
              put(zeros, pad4(tab_n)) && put(m.payload, size_t(m.payload_size) * 2) &&

@@ -26,7 +26,7 @@ struct V1Args {
     int32_t *dst;            // [n_vec][rows]
     const int8_t *vecs;      // [n_vec][vec_stride]
     uint32_t n_vec, vec_stride;
-    uint32_t rows, cols, streams, precision, table_mode;
+    uint32_t rows, cols, streams, precision, table_mode, coder;
     int32_t min_value;
     uint32_t G;
     const uint32_t *cursors;

@@ -238,5 +238,144 @@ struct Lane {
     }
 };
 
+// Quad coder (coder 1): per-lane 64-bit state {lo, hi}, 32-bit words in a ring of
+// R4 = 16 u32 words per lane (lane-interleaved: word slot w of lane i at (w*64 + i)*4
+// bytes of the wave's 4 KiB ring).  A 16-symbol group is four quads and takes at most
+// M = 4 words; NG = M / G4 granules (G4 = 128/S words per stream per granule) are written
+// per group, loaded two groups ahead, with the same fit rule and no-unwritten-slot bound
+// as Lane (R4 = 16 >= 3M + G4).
+template <int G4>
+struct QuadLane {
+    static constexpr int R4 = 16;
+    static constexpr uint32_t kRingWaveBytes = R4 * 64 * 4;
+    static constexpr int M = 4;
+    static_assert(R4 >= 3 * M + G4, "ring too small");
+    static constexpr int NG = M / G4;
+    static constexpr int kPre = (R4 * 3 / 4) / G4;
+
+    __amdgpu_buffer_rsrc_t rsrc;
+    uint32_t lane_off, K, gl, j;
+    uint32_t lo, hi;       // state
+    uint32_t W;            // next unread word
+    uint32_t waddr;        // LDS byte offset of that word within the ring array
+    uint32_t wave_base;
+    Chunk<2 * G4> pa[NG], pb[NG];   // a chunk of 2*G4 u16 = G4 u32 words
+    uint32_t ga[NG], gb[NG];
+    uint32_t *ring, *dummy;
+    uint32_t lane;
+
+    __device__ void init(const uint16_t *payload, uint32_t win_beg, uint32_t win_end, bool valid,
+                         uint32_t row_cur, uint32_t row_end, uint32_t slot, uint32_t *ring_array,
+                         uint32_t *dummy_rows, uint32_t wave, uint32_t lane_in_wave) {
+        rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(payload) + win_beg, 0,
+                                                 static_cast<int>((win_end - win_beg) * 2u),
+                                                 0x00020000);
+        K = 0;
+        uint32_t roff = 0;
+        if (valid) {
+            if (row_end > win_end) row_end = win_end;
+            if ((row_cur & 255u) == 0 && row_cur >= win_beg && row_end > row_cur) {
+                K = (row_end - row_cur) >> 8;
+                roff = (row_cur - win_beg) * 2u;
+            }
+        }
+        lane_off = roff + slot * (G4 * 4u);
+        ring = ring_array;
+        dummy = dummy_rows;
+        lane = lane_in_wave;
+        wave_base = wave * kRingWaveBytes;
+#pragma unroll
+        for (int i = 0; i < NG; i++) ga[i] = gb[i] = 0xFFFFFFFFu;
+    }
+
+    __device__ __forceinline__ uint32_t &ring_at(uint32_t w) {
+        return ring[(wave_base >> 2) + w * 64 + lane];
+    }
+
+    __device__ __forceinline__ void write_granule(const Chunk<2 * G4> &d, uint32_t g) {
+        uint32_t s = (g * G4) & (R4 - 1);
+        uint32_t *p = g != 0xFFFFFFFFu ? &ring_at(s) : dummy + lane;
+#pragma unroll
+        for (int i = 0; i < G4; i++) p[i * 64] = d.d[i];
+    }
+
+    __device__ __forceinline__ void issue(Chunk<2 * G4> &p, uint32_t &gp) {
+        bool ok = gl < K && (gl + 1) * G4 - j <= uint32_t(R4);
+        p = load_chunk<2 * G4>(rsrc, ok ? lane_off + gl * 512u : kOOB);
+        gp = ok ? gl : 0xFFFFFFFFu;
+        gl += ok ? 1u : 0u;
+    }
+
+    __device__ void prologue() {
+        Chunk<2 * G4> pre[kPre];
+#pragma unroll
+        for (uint32_t g = 0; g < uint32_t(kPre); g++)
+            pre[g] = load_chunk<2 * G4>(rsrc, g < K ? lane_off + g * 512u : kOOB);
+#pragma unroll
+        for (uint32_t g = 0; g < uint32_t(kPre); g++) write_granule(pre[g], g < K ? g : 0xFFFFFFFFu);
+        gl = K < uint32_t(kPre) ? K : uint32_t(kPre);
+    }
+
+    __device__ void start() {
+        lo = ring_at(0);
+        hi = ring_at(1);
+        j = 2;
+        W = ring_at(2);
+        waddr = wave_base | (2u * 256u) | (lane * 4u);
+#pragma unroll
+        for (int i = 0; i < NG; i++) issue(pa[i], ga[i]);
+#pragma unroll
+        for (int i = 0; i < NG; i++) issue(pb[i], gb[i]);
+    }
+
+    __device__ __forceinline__ void refill_a() {
+#pragma unroll
+        for (int i = 0; i < NG; i++) write_granule(pa[i], ga[i]);
+#pragma unroll
+        for (int i = 0; i < NG; i++) issue(pa[i], ga[i]);
+    }
+    __device__ __forceinline__ void refill_b() {
+#pragma unroll
+        for (int i = 0; i < NG; i++) write_granule(pb[i], gb[i]);
+#pragma unroll
+        for (int i = 0; i < NG; i++) issue(pb[i], gb[i]);
+    }
+
+    __device__ __forceinline__ void end_group(uint32_t waddr_before) {
+        j += ((waddr - waddr_before) >> 8) & uint32_t(R4 - 1);
+    }
+
+    // One quad: refill if x < 2^32, then four independent lookups of the bytes of lo
+    // (A[q] = f | sym << 24 at lut[q], B[q] = q - cdf[s] at lut[256 + q]) and
+    // x = hi * f0 f1 f2 f3 + (r0 + f0 (r1 + f1 (r2 + f2 r3))).  Returns the four symbols
+    // packed in column order.
+    __device__ __forceinline__ uint32_t quad(const uint32_t *lut) {
+        bool need = hi == 0;
+        hi = need ? lo : hi;
+        lo = need ? W : lo;
+        uint32_t t = waddr + (need ? 256u : 0u);
+        waddr = (t & (kRingWaveBytes - 1)) | (wave_base & ~(kRingWaveBytes - 1));
+        W = *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(ring) + waddr);
+        uint32_t ea[4], eb[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            uint32_t q = (lo >> (8 * i)) & 0xFFu;
+            ea[i] = lut[q];
+            eb[i] = lut[q + 256];
+        }
+        uint32_t f01 = __umul24(ea[0], ea[1]);
+        uint32_t r01 = __umul24(eb[1], ea[0]) + eb[0];
+        uint32_t f23 = __umul24(ea[2], ea[3]);
+        uint32_t r23 = __umul24(eb[3], ea[2]) + eb[2];
+        uint32_t F = __umul24(f01, f23);
+        uint32_t R = __umul24(r23, f01) + r01;
+        uint64_t nx = uint64_t(hi) * F + R;
+        lo = uint32_t(nx);
+        hi = uint32_t(nx >> 32);
+        return __builtin_amdgcn_perm(ea[1], ea[0], 0x0c0c0703u) |
+               __builtin_amdgcn_perm(ea[3], ea[2], 0x07030c0cu);
+    }
+};
+
 }  // namespace v1
 }  // namespace mv

@@ -38,7 +38,8 @@ class CompressedMatrixV1:
     or freq uint8[rows, G] (table_mode 1), payload uint16[payload_size]."""
 
     def __init__(self, rows, cols, streams, precision, table_mode, min_value, G, cursors,
-                 payload, cdf=None, freq=None, grid_spacing=1.0):
+                 payload, cdf=None, freq=None, grid_spacing=1.0, coder=0):
+        self.coder = int(coder)
         self.rows, self.cols = int(rows), int(cols)
         self.streams, self.precision, self.table_mode = int(streams), int(precision), int(table_mode)
         self.min_value, self.G = int(min_value), int(G)
@@ -58,7 +59,7 @@ class CompressedMatrixV1:
         return 2 * self.payload_size + 4 * self.rows + tab
 
     @classmethod
-    def encode(cls, matrix, streams=64, precision=8, table_mode=0, n_threads=0):
+    def encode(cls, matrix, streams=64, precision=8, table_mode=0, n_threads=0, coder=0):
         m = np.ascontiguousarray(np.asarray(matrix), dtype=np.int8)
         rows, cols = m.shape
         mn, G = ctypes.c_int32(), ctypes.c_uint32()
@@ -72,12 +73,12 @@ class CompressedMatrixV1:
         pay = np.empty(cap, np.uint16)
         size = ctypes.c_uint64()
         call("matvans_v1_encode", _np_ptr(m), rows, cols, streams, precision, table_mode,
-             mn.value, g, _np_ptr(cdf), _np_ptr(freq), int(n_threads), _np_ptr(cur),
+             coder, mn.value, g, _np_ptr(cdf), _np_ptr(freq), int(n_threads), _np_ptr(cur),
              _np_ptr(pay), cap, ctypes.byref(size))
         return cls(rows, cols, streams, precision, table_mode, mn.value, g, cur,
                    pay[:size.value].copy(),
                    cdf=cdf[:g + 1].copy() if table_mode == 0 else None,
-                   freq=freq[:rows * g].copy() if table_mode == 1 else None)
+                   freq=freq[:rows * g].copy() if table_mode == 1 else None, coder=coder)
 
     def lut(self):
         """Decode table of a one-table matrix (2 * 2^P uint32)."""
@@ -101,7 +102,7 @@ class CompressedMatrixV1:
         if self.d_payload is None:
             raise RuntimeError("matrix is not on the device; call to_device() first")
         return V1DevMatrix(self.rows, self.cols, self.streams, self.precision, self.table_mode,
-                           self.min_value, self.G, self.payload_size, self.d_cursors.data_ptr(),
+                           self.coder, self.min_value, self.G, self.payload_size, self.d_cursors.data_ptr(),
                            self.d_lut.data_ptr() if self.d_lut is not None else 0,
                            self.d_freq.data_ptr() if self.d_freq is not None else 0,
                            self.d_payload.data_ptr())
@@ -124,7 +125,7 @@ class CompressedMatrixV1:
     def host_struct(self, keep) -> V1Host:
         keep += [self.cursors, self.payload, self.cdf, self.freq]
         return V1Host(self.rows, self.cols, self.grid_spacing, self.streams, self.precision,
-                      self.table_mode, self.min_value, self.G, self.payload_size,
+                      self.table_mode, self.coder, self.min_value, self.G, self.payload_size,
                       self.cursors.ctypes.data,
                       self.cdf.ctypes.data if self.cdf is not None else 0,
                       self.freq.ctypes.data if self.freq is not None else 0,
@@ -156,7 +157,7 @@ def read_mvx1(path):
                 arr(m.payload, ctypes.c_uint16, m.payload_size),
                 cdf=arr(m.cdf, ctypes.c_uint16, m.G + 1) if m.table_mode == 0 else None,
                 freq=arr(m.freq, ctypes.c_uint8, m.rows * m.G) if m.table_mode == 1 else None,
-                grid_spacing=m.grid_spacing))
+                grid_spacing=m.grid_spacing, coder=m.coder))
         return {"num_matrices": num.value, "result_hash": rh.value}, vec, mats
     finally:
         lib().matvans_v1_free(h)

@@ -29,11 +29,25 @@ GW = 256 / S consecutive words (8 for S = 32, as in the reference layout; 4 for 
 2 for S = 128),
 so stream l's word j sits at u16 index cursor + 256*(j // GW) + GW*l + j % GW.
 
+Quad coder (coder = 1; P = 8, one table): 64-bit state x, 32-bit words.  A stream's
+symbols go in quads (decode order); a quad is the four bytes of the low word of x:
+  decode: if x < 2^32: x = (x << 32) | next word
+          q_i = byte i of x; s_i = ppf[q_i]; F = f0*f1*f2*f3; R = r0 + f0*(r1 + f1*(r2 + f2*r3))
+          with r_i = q_i - c[s_i]; x = (x >> 32) * F + R
+  encode (reverse quad order, from x = 2^32): if (x >> 32) >= F: emit x & 0xFFFFFFFF,
+          x >>= 32; then for i = 0..3: r_i = x % f_i, x //= f_i; x = (x << 32) | sum (c_i+r_i) << 8i
+It is the family-B quad coder of buffered/encoder.py:113-246 with 8-bit quantiles (so
+x >> 32 is the high word and the quantiles are whole bytes) and one stream per coder
+instead of a stream shared by a warp.  Stream words are u32, stored as two u16 of the
+payload; word j of stream l is at u16 index cursor + 256*(j // G4) + 2*(G4*l + j % G4)
+with G4 = 128 / S words per stream per 512-byte granule; words 0 and 1 are the low and
+high words of the initial state.
+
 MVX1 container (little-endian):
   "MVX1", u32 version = 1, u32 num_matrices, u32 result_hash, u32 len_v,
   i8 vector[len_v], zero pad to 4 bytes; then per matrix:
   u32 rows, u32 cols, f32 grid_spacing, u32 streams, u32 precision, u32 table_mode,
-  i32 min_value, u32 G, u32 payload_size, u32 cursors[rows],
+  u32 coder, i32 min_value, u32 G, u32 payload_size, u32 cursors[rows],
   table_mode 0: u16 cdf[G + 1] | table_mode 1: u8 freq[rows][G], zero pad to 4 bytes,
   u16 payload[payload_size], zero pad to 4 bytes.
 """
@@ -166,6 +180,93 @@ def encode_stream(symbols, cdf, precision):
     return bulk[::-1]
 
 
+def encode_stream_quad(symbols, cdf):
+    """Quad coder (coder = 1) for one stream, symbols in decode order (a multiple of
+    four); returns its u32 words in decode order [x_lo, x_hi, last flushed, ...]."""
+    cdf = [int(c) for c in cdf]
+    sym = [int(v) for v in symbols]
+    x = 1 << 32
+    bulk = []
+    for j in reversed(range(len(sym) // 4)):
+        quad = sym[4 * j:4 * j + 4]
+        f = [cdf[t + 1] - cdf[t] for t in quad]
+        F = f[0] * f[1] * f[2] * f[3]
+        if (x >> 32) >= F:
+            bulk.append(x & 0xFFFFFFFF)
+            x >>= 32
+        concat = 0
+        for i in range(4):
+            concat |= (cdf[quad[i]] + x % f[i]) << (8 * i)
+            x //= f[i]
+        x = (x << 32) | concat
+    return [x & 0xFFFFFFFF, x >> 32] + bulk[::-1]
+
+
+def encode_rows_quad(matrix, streams, min_value, cdf):
+    """Quad-coder encode of a matrix.  Returns (cursors uint32[rows] in u16 units,
+    payload uint16)."""
+    m = np.asarray(matrix, np.int8)
+    rows, cols = m.shape
+    assert streams in (32, 64, 128) and cols % (16 * streams) == 0
+    G4 = 128 // streams
+    out, cursors, pos = [], [], 0
+    for r in range(rows):
+        sym = m[r].astype(np.int64) - min_value
+        lanes = [encode_stream_quad(sym[stream_columns(cols, streams, l)], cdf)
+                 for l in range(streams)]
+        K = (max(len(s) for s in lanes) + G4 - 1) // G4
+        row = np.zeros(K * 128, np.uint32)
+        for l, s in enumerate(lanes):
+            j = np.arange(len(s))
+            row[128 * (j // G4) + G4 * l + j % G4] = s
+        cursors.append(pos)
+        pos += 2 * len(row)
+        out.append(row.view(np.uint16))
+    return np.asarray(cursors, np.uint32), np.concatenate(out)
+
+
+def decode_rows_quad(rows, cols, streams, cursors, payload, min_value, cdf):
+    """Decode a quad-coder matrix, vectorised over the rows x streams streams."""
+    pay32 = np.asarray(payload, np.uint16).view(np.uint32)
+    S, G4 = streams, 128 // streams
+    cdf = np.asarray(cdf, np.int64)
+    ppf = ppf_of(cdf, 8)
+    n = rows * S
+    row = np.repeat(np.arange(rows), S)
+    lane = np.tile(np.arange(S), rows)
+    base = np.repeat(np.asarray(cursors, np.int64)[:rows] // 2, S) + G4 * lane
+
+    def word(b, j):
+        p = b + 128 * (j // G4) + j % G4
+        if p.size and p.max() >= len(pay32):
+            raise IndexError("stream reads past the payload")
+        return pay32[p].astype(np.uint64)
+
+    x = (word(base, np.ones(n, np.int64)) << np.uint64(32)) | word(base, np.zeros(n, np.int64))
+    ptr = np.full(n, 2, np.int64)
+    out = np.empty((rows, cols), np.int8)
+    for j in range(cols // S // 4):
+        need = x < (np.uint64(1) << np.uint64(32))
+        if need.any():
+            i = np.nonzero(need)[0]
+            x[i] = (x[i] << np.uint64(32)) | word(base[i], ptr[i])
+            ptr[i] += 1
+        q = [((x >> np.uint64(8 * i)) & np.uint64(0xFF)).astype(np.int64) for i in range(4)]
+        F = np.ones(n, np.uint64)
+        R = np.zeros(n, np.uint64)
+        mul = np.ones(n, np.uint64)
+        for i in range(4):
+            s = ppf[q[i]]
+            k = 4 * j + i
+            col = 16 * S * (k // 16) + 16 * lane + k % 16
+            out[row, col] = ((min_value + s) & 0xFF).astype(np.uint8).view(np.int8)
+            f = (cdf[s + 1] - cdf[s]).astype(np.uint64)
+            R = R + mul * (q[i] - cdf[s]).astype(np.uint64)
+            mul = mul * f
+        x = (x >> np.uint64(32)) * mul + R
+    return out
+
+
 def encode_rows(matrix, streams, precision, min_value, cdf=None, freq=None):
     """Encode a matrix with ``streams`` streams per row, one table (``cdf``) or one per
     row (``freq``).  Returns (cursors uint32[rows], payload uint16)."""
@@ -256,10 +357,35 @@ def decode_stream_textbook(words, n, cdf, precision):
     return out
 
 
+def decode_stream_quad_textbook(words, n_quads, cdf):
+    """One quad-coder stream (u32 words in decode order, [x_lo, x_hi, ...]) decoded by
+    four plain 8-bit rANS pops per quad, x = q0 + 256*(q1 + 256*(q2 + 256*(q3 + 256*y))):
+    pop i takes q_i = x % 256 and leaves y_i = f(s_i) * (x // 256) + q_i - cdf[s_i] --
+    applied from the last quantile inwards, which is the quad coder's closed form.
+    Independent of the vectorised decoder; returns the symbol indices."""
+    cdf = [int(c) for c in cdf]
+    x = (words[1] << 32) | words[0]
+    p = 2
+    out = []
+    for _ in range(n_quads):
+        if x < 1 << 32:
+            x = (x << 32) | words[p]
+            p += 1
+        qs = [(x >> (8 * i)) & 0xFF for i in range(4)]
+        ss = [max(k for k in range(len(cdf) - 1) if cdf[k] <= q < cdf[k + 1]) for q in qs]
+        y = x >> 32
+        for q, t in zip(reversed(qs), reversed(ss)):
+            y = (cdf[t + 1] - cdf[t]) * y + q - cdf[t]
+        x = y
+        out += ss
+    return out
+
+
 # --------------------------------------------------------------------- container
 class RecordV1:
     def __init__(self, rows, cols, streams, precision, table_mode, min_value, G, cursors,
-                 payload, cdf=None, freq=None, grid_spacing=1.0):
+                 payload, cdf=None, freq=None, grid_spacing=1.0, coder=0):
+        self.coder = coder
         self.rows, self.cols, self.streams, self.precision = rows, cols, streams, precision
         self.table_mode, self.min_value, self.G = table_mode, min_value, G
         self.cursors = np.asarray(cursors, np.uint32)
@@ -269,12 +395,21 @@ class RecordV1:
         self.grid_spacing = grid_spacing
 
     def decode(self):
+        if self.coder == 1:
+            return decode_rows_quad(self.rows, self.cols, self.streams, self.cursors,
+                                    self.payload, self.min_value, self.cdf)
         return decode_rows(self.rows, self.cols, self.streams, self.precision, self.cursors,
                            self.payload, self.min_value, self.cdf, self.freq)
 
 
-def encode_record(matrix, streams=64, precision=8, table_mode=0):
+def encode_record(matrix, streams=64, precision=8, table_mode=0, coder=0):
     m = np.asarray(matrix, np.int8)
+    if coder == 1:
+        assert precision == 8 and table_mode == 0
+        mn, G, cdf = global_model(m, 8)
+        cur, pay = encode_rows_quad(m, streams, mn, cdf)
+        return RecordV1(m.shape[0], m.shape[1], streams, 8, 0, mn, G, cur, pay, cdf=cdf,
+                        coder=1)
     if table_mode == 0:
         mn, G, cdf = global_model(m, precision)
         cur, pay = encode_rows(m, streams, precision, mn, cdf=cdf)
@@ -296,8 +431,9 @@ def write_mvx1(path, records, vector=np.zeros(0, np.int8), result_hash=0):
         f.write(vector.tobytes())
         _pad4(f, len(vector))
         for r in records:
-            f.write(struct.pack("<LLfLLLlLL", r.rows, r.cols, r.grid_spacing, r.streams,
-                                r.precision, r.table_mode, r.min_value, r.G, len(r.payload)))
+            f.write(struct.pack("<LLfLLLLlLL", r.rows, r.cols, r.grid_spacing, r.streams,
+                                r.precision, r.table_mode, r.coder, r.min_value, r.G,
+                                len(r.payload)))
             f.write(np.asarray(r.cursors, "<u4").tobytes())
             tab = (np.asarray(r.cdf, "<u2").tobytes() if r.table_mode == 0
                    else np.asarray(r.freq, np.uint8).tobytes())
@@ -321,8 +457,8 @@ def read_mvx1(path):
     o += lv + (4 - lv % 4) % 4
     recs = []
     for _ in range(num):
-        rows, cols, gs, S, P, mode, mn, G, psize = struct.unpack_from("<LLfLLLlLL", data, o)
-        o += 36
+        rows, cols, gs, S, P, mode, coder, mn, G, psize = struct.unpack_from("<LLfLLLLlLL", data, o)
+        o += 40
         cur = np.frombuffer(data, "<u4", rows, o).copy()
         o += 4 * rows
         if mode == 0:
@@ -334,7 +470,7 @@ def read_mvx1(path):
         o += n + (4 - n % 4) % 4
         pay = np.frombuffer(data, "<u2", psize, o).copy()
         o += 2 * psize + (4 - (2 * psize) % 4) % 4
-        recs.append(RecordV1(rows, cols, S, P, mode, mn, G, cur, pay, cdf, freq, gs))
+        recs.append(RecordV1(rows, cols, S, P, mode, mn, G, cur, pay, cdf, freq, gs, coder))
     if o != len(data):
         raise ValueError("trailing bytes")
     return {"num_matrices": num, "result_hash": rhash}, vec, recs

@@ -15,8 +15,8 @@ from .helpers import gaussian_int8, gaussian_vec, laplace_int8, uniform_vec
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = [(32, 8, 0), (64, 8, 0), (128, 8, 0), (32, 12, 0), (64, 12, 0), (128, 12, 0),
-            (64, 8, 1)]
+VARIANTS = [(32, 8, 0, 0), (64, 8, 0, 0), (128, 8, 0, 0), (32, 12, 0, 0), (64, 12, 0, 0),
+            (128, 12, 0, 0), (64, 8, 1, 0), (32, 8, 0, 1), (64, 8, 0, 1), (128, 8, 0, 1)]
 
 
 def note(msg):
@@ -40,51 +40,52 @@ SMALL = [(1, 2048, "g", 6.0), (3, 1024, "g", 0.3), (7, 3072, "l", 4.0), (65, 204
          (129, 4096, "l", 4.0), (333, 1024, "g", 60.0)]
 
 
-@pytest.mark.parametrize("S,P,mode", VARIANTS)
+@pytest.mark.parametrize("S,P,mode,coder", VARIANTS)
 @pytest.mark.parametrize("rows,cols,kind,p", SMALL)
-def test_shapes(cuda, S, P, mode, rows, cols, kind, p):
+def test_shapes(cuda, S, P, mode, coder, rows, cols, kind, p):
     if cols % (16 * S):
         pytest.skip("cols not a multiple of 16 * streams")
-    note(f"v1 S={S} P={P} mode={mode} {rows}x{cols} {kind}{p}")
+    note(f"v1 S={S} P={P} mode={mode} coder={coder} {rows}x{cols} {kind}{p}")
     maker = gaussian_int8 if kind == "g" else laplace_int8
     m, _ = maker(rows, cols, p, seed=rows + 3 * cols)
-    c = CompressedMatrixV1.encode(m, S, P, mode).to_device()
+    c = CompressedMatrixV1.encode(m, S, P, mode, coder=coder).to_device()
     for n_vec in (1, 3, 8):
         vecs = np.stack([uniform_vec(cols, seed=200 + i) for i in range(n_vec)])
         got = gpu_matvec(c, vecs)
         assert np.array_equal(got, want(m, vecs)), (n_vec, np.argwhere(got != want(m, vecs))[:4])
 
 
-@pytest.mark.parametrize("S,P,mode", VARIANTS)
-def test_edge_alphabets(cuda, S, P, mode):
+@pytest.mark.parametrize("S,P,mode,coder", VARIANTS)
+def test_edge_alphabets(cuda, S, P, mode, coder):
     """Constant rows, a near-constant matrix (few renormalisations) and a uniform
     wide-alphabet matrix (a renormalisation at almost every step)."""
-    note(f"v1 edge alphabets S={S} P={P} mode={mode}")
+    note(f"v1 edge alphabets S={S} P={P} mode={mode} coder={coder}")
     near = np.zeros((9, 2048), np.int8)
     near[1, 5] = 2
     near[4] = -127
     near[7] = 127
     wide = np.random.default_rng(1).integers(-128, 128, (6, 2048)).astype(np.int8)
     for m in (near, wide, np.full((4, 2048), -3, np.int8)):
-        c = CompressedMatrixV1.encode(m, S, P, mode).to_device()
+        c = CompressedMatrixV1.encode(m, S, P, mode, coder=coder).to_device()
         vecs = np.stack([uniform_vec(2048, seed=1), np.full(2048, -128, np.int8)])
         got = gpu_matvec(c, vecs)
         assert np.array_equal(got, want(m, vecs))
-        r = E.encode_record(m, S, P, mode)   # oracle decode of the same bytes
+        r = E.encode_record(m, S, P, mode, coder=coder)   # oracle decode of the same bytes
         assert np.array_equal(r.decode(), m)
 
 
 LLM = [(4096, 14336, "l", 4.0), (14336, 4096, "l", 4.0), (4096, 4096, "g", 6.0)]
 
 
-@pytest.mark.parametrize("S,P,mode", [(128, 8, 0), (64, 8, 0), (64, 12, 0), (64, 8, 1),
-                                      (32, 12, 0)])
+@pytest.mark.parametrize("S,P,mode,coder", [(128, 8, 0, 0), (64, 8, 0, 0), (64, 12, 0, 0),
+                                            (64, 8, 1, 0), (32, 12, 0, 0), (32, 8, 0, 1),
+                                            (64, 8, 0, 1), (128, 8, 0, 1)])
 @pytest.mark.parametrize("rows,cols,kind,p", LLM)
-def test_llm_shapes(cuda, S, P, mode, rows, cols, kind, p):
-    note(f"v1 LLM S={S} P={P} mode={mode} {rows}x{cols} {kind}{p}")
+def test_llm_shapes(cuda, S, P, mode, coder, rows, cols, kind, p):
+    note(f"v1 LLM S={S} P={P} mode={mode} coder={coder} {rows}x{cols} {kind}{p}")
     maker = gaussian_int8 if kind == "g" else laplace_int8
     m, _ = maker(rows, cols, p, seed=rows + cols)
-    c = CompressedMatrixV1.encode(m, S, P, mode, n_threads=16).to_device()
+    c = CompressedMatrixV1.encode(m, S, P, mode, n_threads=16, coder=coder).to_device()
     vecs = np.stack([gaussian_vec(cols, seed=3), uniform_vec(cols, seed=4)])
     assert np.array_equal(gpu_matvec(c, vecs), want(m, vecs))
 

@@ -37,6 +37,17 @@ def test_encoder_matches_oracle(S, P, mode):
         assert np.array_equal(r.decode(), m)
 
 
+@pytest.mark.parametrize("S", [32, 64, 128])
+def test_quad_encoder_matches_oracle(S):
+    for m in _mats():
+        if m.shape[1] % (16 * S):
+            continue
+        c = CompressedMatrixV1.encode(m, S, 8, 0, n_threads=3, coder=1)
+        r = E.encode_record(m, S, 8, 0, coder=1)
+        assert np.array_equal(c.cursors, r.cursors) and np.array_equal(c.payload, r.payload)
+        assert np.array_equal(c.cdf, r.cdf) and np.array_equal(r.decode(), m)
+
+
 def test_reference_special_case_matches_v0_library():
     import matvans_amd as mv
     m = laplace_int8(9, 3072, 4.0, seed=4)[0]
@@ -48,21 +59,24 @@ def test_reference_special_case_matches_v0_library():
 def test_container_round_trip(tmp_path):
     ms = _mats()
     cs = [CompressedMatrixV1.encode(ms[0], 64, 12, 0), CompressedMatrixV1.encode(ms[2], 64, 8, 1),
-          CompressedMatrixV1.encode(ms[1], 32, 8, 0)]
+          CompressedMatrixV1.encode(ms[1], 32, 8, 0), CompressedMatrixV1.encode(ms[1], 64, coder=1)]
     vec = np.arange(7, dtype=np.int8)
     p = tmp_path / "a.mvx"
     write_mvx1(p, cs, vec, 77)
     hdr, v, back = read_mvx1(p)
-    assert hdr == {"num_matrices": 3, "result_hash": 77} and np.array_equal(v, vec)
+    assert hdr == {"num_matrices": 4, "result_hash": 77} and np.array_equal(v, vec)
     ohdr, ov, orecs = E.read_mvx1(p)
     for c, b, o in zip(cs, back, orecs):
         assert np.array_equal(c.payload, b.payload) and np.array_equal(c.cursors, b.cursors)
-        assert np.array_equal(o.decode(), E.decode_rows(c.rows, c.cols, c.streams, c.precision,
-                                                        c.cursors, c.payload, c.min_value,
-                                                        c.cdf, c.freq))
+        assert np.array_equal(o.decode(), b_mat(c)) and b.coder == c.coder
     q = tmp_path / "b.mvx"
     E.write_mvx1(q, orecs, ov, ohdr["result_hash"])
     assert open(p, "rb").read() == open(q, "rb").read()
+
+
+def b_mat(c):
+    return E.RecordV1(c.rows, c.cols, c.streams, c.precision, c.table_mode, c.min_value, c.G,
+                      c.cursors, c.payload, c.cdf, c.freq, coder=c.coder).decode()
 
 
 def test_reader_rejects_bad_files(tmp_path):
@@ -88,3 +102,6 @@ def test_encoder_argument_errors():
     with pytest.raises(MatvansError) as e:       # cols not a multiple of 16 * 64
         CompressedMatrixV1.encode(m[:, :512], 64, 8, 0)
     assert e.value.status == 2
+    with pytest.raises(MatvansError) as e:       # quad coder with 12-bit tables
+        CompressedMatrixV1.encode(m, 64, 12, 0, coder=1)
+    assert e.value.status == 8

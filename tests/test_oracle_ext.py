@@ -97,6 +97,41 @@ def test_textbook_decode_of_single_streams(S, P, mode):
             assert syms == want.tolist()
 
 
+@pytest.mark.parametrize("S", [32, 64, 128])
+@pytest.mark.parametrize("name", list(CASES))
+def test_quad_round_trip(S, name):
+    m = CASES[name]
+    if m.shape[1] % (16 * S):
+        pytest.skip("cols not a multiple of 16 * streams")
+    r = E.encode_record(m, S, 8, 0, coder=1)
+    assert r.payload.size % 256 == 0 and np.all(r.cursors % 256 == 0)
+    assert np.array_equal(r.decode(), m)
+
+
+@pytest.mark.parametrize("S", [32, 64, 128])
+def test_quad_textbook_decode(S):
+    """Gather one quad stream's u32 words with the layout formula and decode them with
+    plain 8-bit pops."""
+    m = CASES["laplace4"][:, :2048] if S >= 64 else CASES["laplace4"]
+    r = E.encode_record(m, S, 8, 0, coder=1)
+    p32 = r.payload.view(np.uint32)
+    G4 = 128 // S
+    for row in (0, m.shape[0] - 1):
+        end = (r.cursors[row + 1] if row + 1 < m.shape[0] else len(r.payload)) // 2
+        for lane in (0, 3, S - 1):
+            words, j = [], 0
+            while True:
+                p = int(r.cursors[row]) // 2 + 128 * (j // G4) + G4 * lane + j % G4
+                if p >= end:
+                    break
+                words.append(int(p32[p]))
+                j += 1
+            n = m.shape[1] // S
+            syms = E.decode_stream_quad_textbook(words, n // 4, r.cdf)
+            want = m[row, E.stream_columns(m.shape[1], S, lane)].astype(np.int64) - r.min_value
+            assert syms == want.tolist()
+
+
 def test_models():
     m = CASES["gauss6"]
     for P in (8, 12):
@@ -144,12 +179,13 @@ def test_planted_column_mistake_is_caught():
 
 def test_container_round_trip(tmp_path):
     recs = [E.encode_record(CASES["gauss6"], 64, 12, 0), E.encode_record(CASES["near_const"], 64, 8, 1),
-            E.encode_record(CASES["laplace4"], 32, 8, 0)]
+            E.encode_record(CASES["laplace4"], 32, 8, 0),
+            E.encode_record(CASES["gauss6"], 64, 8, 0, coder=1)]
     vec = np.arange(-5, 6, dtype=np.int8)
     p = tmp_path / "a.mvx"
     E.write_mvx1(p, recs, vec, 0xDEADBEEF)
     hdr, v, back = E.read_mvx1(p)
-    assert hdr == {"num_matrices": 3, "result_hash": 0xDEADBEEF} and np.array_equal(v, vec)
+    assert hdr == {"num_matrices": 4, "result_hash": 0xDEADBEEF} and np.array_equal(v, vec)
     for a, b in zip(recs, back):
         assert np.array_equal(a.decode(), b.decode())
     q = tmp_path / "b.mvx"
