@@ -267,7 +267,7 @@ __global__ __launch_bounds__(256) void v1q_fwd(int32_t *__restrict__ dst,
     constexpr int G4 = 128 / S;
     using QL = v1::QuadLane<G4>;
     __shared__ uint32_t ring[kWaves * QL::R4 * 64];
-    __shared__ uint32_t lut[512];
+    __shared__ uint32_t tab[256];
     __shared__ uint32_t dummy[4 * 64];
     __shared__ int32_t part[WPR > 1 ? kWaves * NV : 1];
 
@@ -278,11 +278,9 @@ __global__ __launch_bounds__(256) void v1q_fwd(int32_t *__restrict__ dst,
                                   : (blockIdx.x * kWaves + wave) * RPW;
     const uint32_t row = row0 + (S >= 64 ? 0u : lane / S);
     const bool valid = row < rows;
-    {
-        const uint4 *src = reinterpret_cast<const uint4 *>(glut);
-        uint4 *d = reinterpret_cast<uint4 *>(lut);
-        if (tid < 128) d[tid] = src[tid];
-    }
+    // packed table T[q] = f | r << 8 | sym << 24 from the A / B tables (A = f | sym << 24,
+    // B = r); f <= 255 and r < f for every table the encoder accepts
+    tab[tid] = (glut[tid] & 0xFF0000FFu) | ((glut[256 + tid] & 0xFFu) << 8);
     uint32_t wbeg = 0, wend = 0;
     if (row0 < rows) {
         wbeg = cursors[row0];
@@ -321,7 +319,7 @@ __global__ __launch_bounds__(256) void v1q_fwd(int32_t *__restrict__ dst,
         uint32_t wbefore = st.waddr;
 #pragma unroll
         for (int d = 0; d < 4; d++) {
-            uint32_t packed = st.quad(lut);
+            uint32_t packed = st.quad(tab);
 #pragma unroll
             for (int v = 0; v < NV; v++)
                 acc[v] = __builtin_amdgcn_sdot4(static_cast<int>(packed),

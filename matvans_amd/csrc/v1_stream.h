@@ -39,6 +39,42 @@ constexpr uint32_t kSelKeep8 = 0x04070605u;     // rotr(x1, 8)
 constexpr uint32_t kSelRenorm12 = 0x05040100u;  // x1 << 16 | word
 constexpr uint32_t kSelKeep12 = 0x07060504u;    // x1
 
+// 24-bit multiplies on the low 24 bits of each operand (v_mul_u32_u24 / v_mad_u32_u24),
+// written out because the table words carry the symbol in their top byte and the compiler
+// would otherwise rebuild the masked products with full 32-bit multiplies.
+__device__ __forceinline__ uint32_t mul_u24(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// Byte-field products of packed table words, one SDWA instruction each:
+// mul_b0_b0(a, b) = (a & 0xFF) * (b & 0xFF); mul_b1_b0(a, b) = ((a >> 8) & 0xFF) * (b & 0xFF);
+// add_b1(a, b) = a + ((b >> 8) & 0xFF).
+__device__ __forceinline__ uint32_t mul_b0_b0(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_mul_u32_u24_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_0 "
+        "src1_sel:BYTE_0" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ uint32_t mul_b1_b0(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_mul_u32_u24_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_1 "
+        "src1_sel:BYTE_0" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ uint32_t add_b1(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD "
+        "src1_sel:BYTE_1" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 template <int GW>
 struct Chunk;
 template <>
@@ -345,35 +381,32 @@ struct QuadLane {
         j += ((waddr - waddr_before) >> 8) & uint32_t(R4 - 1);
     }
 
-    // One quad: refill if x < 2^32, then four independent lookups of the bytes of lo
-    // (A[q] = f | sym << 24 at lut[q], B[q] = q - cdf[s] at lut[256 + q]) and
-    // x = hi * f0 f1 f2 f3 + (r0 + f0 (r1 + f1 (r2 + f2 r3))).  Returns the four symbols
-    // packed in column order.
-    __device__ __forceinline__ uint32_t quad(const uint32_t *lut) {
+    // One quad: refill if x < 2^32, then four independent lookups of the bytes q of lo
+    // in the packed table T[q] = f | r << 8 | sym << 24 (r = q - cdf[s]; one dword per
+    // quantile, so one LDS access per symbol) and
+    // x = hi * f0 f1 f2 f3 + (r0 + f0 (r1 + f1 (r2 + f2 r3))), with the byte fields taken
+    // by SDWA operand selects.  Returns the four symbols packed in column order.
+    __device__ __forceinline__ uint32_t quad(const uint32_t *tab) {
         bool need = hi == 0;
         hi = need ? lo : hi;
         lo = need ? W : lo;
         uint32_t t = waddr + (need ? 256u : 0u);
         waddr = (t & (kRingWaveBytes - 1)) | (wave_base & ~(kRingWaveBytes - 1));
         W = *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(ring) + waddr);
-        uint32_t ea[4], eb[4];
+        uint32_t e[4];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            uint32_t q = (lo >> (8 * i)) & 0xFFu;
-            ea[i] = lut[q];
-            eb[i] = lut[q + 256];
-        }
-        uint32_t f01 = __umul24(ea[0], ea[1]);
-        uint32_t r01 = __umul24(eb[1], ea[0]) + eb[0];
-        uint32_t f23 = __umul24(ea[2], ea[3]);
-        uint32_t r23 = __umul24(eb[3], ea[2]) + eb[2];
-        uint32_t F = __umul24(f01, f23);
-        uint32_t R = __umul24(r23, f01) + r01;
+        for (int i = 0; i < 4; i++) e[i] = tab[(lo >> (8 * i)) & 0xFFu];
+        uint32_t f01 = mul_b0_b0(e[0], e[1]);
+        uint32_t r01 = add_b1(mul_b1_b0(e[1], e[0]), e[0]);
+        uint32_t f23 = mul_b0_b0(e[2], e[3]);
+        uint32_t r23 = add_b1(mul_b1_b0(e[3], e[2]), e[2]);
+        uint32_t F = mul_u24(f01, f23);
+        uint32_t R = mad_u24(r23, f01, r01);
         uint64_t nx = uint64_t(hi) * F + R;
         lo = uint32_t(nx);
         hi = uint32_t(nx >> 32);
-        return __builtin_amdgcn_perm(ea[1], ea[0], 0x0c0c0703u) |
-               __builtin_amdgcn_perm(ea[3], ea[2], 0x07030c0cu);
+        return __builtin_amdgcn_perm(e[1], e[0], 0x0c0c0703u) |
+               __builtin_amdgcn_perm(e[3], e[2], 0x07030c0cu);
     }
 };
 
