@@ -62,7 +62,8 @@ int matvans_decompress_and_multiply(int32_t *dst, const int8_t *vec, uint32_t ro
 
 /* Decode kernel variant used by the launch entry points (all give the same bits):
  * 0 = reference form of the step, 1 = rotated state with dword tables (default),
- * 2 = rotated state with three conflict-free 256-byte tables; -1 restores the default
+ * 2 = rotated state with three conflict-free 256-byte tables, 3 = rotated state with one
+ * packed table dword per quantile; -1 restores the default
  * (or MATVANS_DECODE_VARIANT).  No reference counterpart: a tuning control for
  * measurements and the autotuner.  Process-wide. */
 int matvans_set_decode_variant(int variant);

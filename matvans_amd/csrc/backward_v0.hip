@@ -66,16 +66,15 @@ __global__ __launch_bounds__(512) void v0_bwd_partial(
     const uint8_t *__restrict__ cdf, const uint8_t *__restrict__ ppf,
     const uint16_t *__restrict__ payload, uint32_t payload_size) {
     __shared__ uint16_t ring[kBwdWaves * kRingWords * 64];
-    __shared__ uint32_t lut[512];
+    __shared__ uint32_t lut[256];
     __shared__ uint16_t dummy[kDummyBytes / 2];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid >> 6, lane = tid & 63;
-    if (tid < 256) {
+    if (tid < 256) {   // packed table f | bias << 8 | sym << 24
         uint32_t f, sym, bias;
         v0_table_entry(tid, min_value, G, cdf, ppf, f, sym, bias);
-        lut[tid] = f | (sym << 24);
-        lut[tid + 256] = bias;
+        lut[tid] = f | (bias << 8) | (sym << 24);
     }
     const uint32_t stream = blockIdx.y * kBwdWaves + wave;   // 0..31
     const uint32_t row0 = blockIdx.x * 64u;
@@ -115,8 +114,8 @@ __global__ __launch_bounds__(512) void v0_bwd_partial(
 #pragma unroll
         for (int d = 0; d < 4; d++) {
             uint32_t e[4];
-            st.pair(lut, e[0], e[1]);
-            st.pair(lut, e[2], e[3]);
+            st.pair_packed(lut, e[0], e[1]);
+            st.pair_packed(lut, e[2], e[3]);
             float w[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) w[u] = static_cast<float>(static_cast<int8_t>(e[u] >> 24));

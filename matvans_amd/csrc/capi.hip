@@ -21,15 +21,18 @@ int hip_status(hipError_t e, const char *what) {
 }
 
 // Decode kernel variant, all computing the same bits: 0 = reference form of the step,
-// 1 = rotated state with dword tables (default), 2 = rotated state with three
-// conflict-free byte tables.  MATVANS_DECODE_VARIANT overrides the default for A/B runs.
+// 1 = rotated state with dword tables (default: the 32-stream layout leaves few waves
+// per SIMD at long rows, so the shorter dependent chain of the read2 form wins),
+// 2 = rotated state with three conflict-free byte tables, 3 = rotated state with one
+// packed dword table (half the LDS accesses, one more dependent op).
+// MATVANS_DECODE_VARIANT overrides the default for A/B runs.
 std::atomic<int> g_variant{-1};
 
 int decode_variant() {
     int v = g_variant.load(std::memory_order_relaxed);
     if (v >= 0) return v;
     const char *e = std::getenv("MATVANS_DECODE_VARIANT");
-    v = (e && e[0] >= '0' && e[0] <= '2' && e[1] == 0) ? e[0] - '0' : 1;
+    v = (e && e[0] >= '0' && e[0] <= '3' && e[1] == 0) ? e[0] - '0' : 1;
     g_variant.store(v, std::memory_order_relaxed);
     return v;
 }
@@ -53,7 +56,7 @@ int check_v0(const int32_t *dst, const int8_t *vec, uint32_t n_vec, uint32_t vec
 }  // namespace
 
 extern "C" int matvans_set_decode_variant(int variant) {
-    if (variant < -1 || variant > 2) return fail(MATVANS_E_INVALID, "variant %d outside -1..2", variant);
+    if (variant < -1 || variant > 3) return fail(MATVANS_E_INVALID, "variant %d outside -1..3", variant);
     g_variant.store(variant, std::memory_order_relaxed);
     mv::clear_error();
     return MATVANS_OK;

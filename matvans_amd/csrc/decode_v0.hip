@@ -65,8 +65,11 @@ __device__ __forceinline__ RowMajorSetup row_major_setup(uint32_t row0, uint32_t
 
 // TABLE 0: dword tables A[q] = f | sym << 24 and B[q] = bias, read with one
 // ds_read2st64_b32.  TABLE 2: three 256-byte tables f, bias, sym (64 dwords each, one per
-// LDS bank, so every read is conflict-free); same speed on the shapes measured, kept for
-// the per-row table format whose tables must be small.
+// LDS bank, so every read is conflict-free).  TABLE 3: one packed dword
+// T[q] = f | bias << 8 | sym << 24, one LDS access per step and one more dependent VALU
+// op.  A random lookup costs the LDS about 7 CU-cycles per wave instruction (a
+// two-address read2 about 13); at long rows this layout has two waves per SIMD and is
+// bound by the dependent chain, where TABLE 0 measures faster.
 template <int NV, int TABLE>
 __global__ __launch_bounds__(256) void v0_fwd(int32_t *__restrict__ dst,
                                               const int8_t *__restrict__ vecs,
@@ -89,6 +92,8 @@ __global__ __launch_bounds__(256) void v0_fwd(int32_t *__restrict__ dst,
         if (TABLE == 0) {
             lut[tid] = f | (sym << 24);
             lut[tid + 256] = bias;
+        } else if (TABLE == 3) {
+            lut[tid] = f | (bias << 8) | (sym << 24);
         } else {
             uint8_t *tab = reinterpret_cast<uint8_t *>(lut);
             tab[tid] = static_cast<uint8_t>(f);
@@ -127,9 +132,14 @@ __global__ __launch_bounds__(256) void v0_fwd(int32_t *__restrict__ dst,
 #pragma unroll
         for (int d = 0; d < 4; d++) {
             uint32_t e0, e1, e2, e3, packed;
-            if (TABLE == 0) {
-                st.pair(lut, e0, e1);
-                st.pair(lut, e2, e3);
+            if (TABLE == 0 || TABLE == 3) {
+                if (TABLE == 0) {
+                    st.pair(lut, e0, e1);
+                    st.pair(lut, e2, e3);
+                } else {
+                    st.pair_packed(lut, e0, e1);
+                    st.pair_packed(lut, e2, e3);
+                }
                 packed = __builtin_amdgcn_perm(e1, e0, 0x0c0c0703u) |
                          __builtin_amdgcn_perm(e3, e2, 0x07030c0cu);
             } else {
@@ -296,6 +306,10 @@ static hipError_t launch_nv(const V0Args &a, hipStream_t s) {
                            a.ppf, a.payload, a.payload_size);
     else if (a.variant == 1)
         hipLaunchKernelGGL((v0_fwd<NV, 0>), grid, block, 0, s, a.dst, a.vecs, a.vec_stride,
+                           a.rows, a.cols, a.cursors, int(a.min_value), uint32_t(a.G), a.cdf,
+                           a.ppf, a.payload, a.payload_size);
+    else if (a.variant == 3)
+        hipLaunchKernelGGL((v0_fwd<NV, 3>), grid, block, 0, s, a.dst, a.vecs, a.vec_stride,
                            a.rows, a.cols, a.cursors, int(a.min_value), uint32_t(a.G), a.cdf,
                            a.ppf, a.payload, a.payload_size);
     else

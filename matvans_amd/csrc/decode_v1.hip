@@ -86,7 +86,11 @@ __global__ __launch_bounds__(256) void v1_fwd(int32_t *__restrict__ dst,
     const uint32_t row = row0 + (S >= 64 ? 0u : lane / S);
     const bool valid = row < rows;
 
-    if constexpr (MODE == 0) {
+    if constexpr (MODE == 0 && P == 8) {
+        // packed table T[q] = f | bias << 8 | sym << 24 from A = f | sym << 24, B = bias
+        // (f <= 255 whenever the alphabet has two or more symbols, as the model ensures)
+        lut[tid] = (glut[tid] & 0xFF0000FFu) | ((glut[256 + tid] & 0xFFu) << 8);
+    } else if constexpr (MODE == 0) {
         const uint4 *src = reinterpret_cast<const uint4 *>(glut);
         uint4 *d = reinterpret_cast<uint4 *>(lut);
         for (uint32_t i = tid; i < uint32_t(kLut / 4); i += 256) d[i] = src[i];
@@ -185,8 +189,8 @@ __global__ __launch_bounds__(256) void v1_fwd(int32_t *__restrict__ dst,
                          __builtin_amdgcn_perm(e3, e2, 0x04000c0cu);
             } else {
                 if constexpr (P == 8) {
-                    st.pair8(lut, e0, e1);
-                    st.pair8(lut, e2, e3);
+                    st.pair8_packed(lut, e0, e1);
+                    st.pair8_packed(lut, e2, e3);
                 } else {
                     e0 = st.step12(lut);
                     e1 = st.step12(lut);

@@ -22,6 +22,8 @@
 
 #include <cstdint>
 
+#include "alu.h"
+
 namespace mv {
 
 constexpr int kRingWords = 64;             // ring words per lane (power of two)
@@ -172,6 +174,23 @@ struct V0Lane {
             xr = __builtin_amdgcn_perm(x1, W, c ? kSelRenorm : kSelKeep);
             cc += c ? 1u : 0u;
             (h == 0 ? e0 : e1) = ea;
+        }
+        advance_word(cc);
+    }
+
+    // Two decode steps with the packed table T[q] = f | bias << 8 | sym << 24: one LDS
+    // access per step, x1 = (x >> 8) * f + bias from SDWA byte selects.  Returns the two
+    // table words (symbol in byte 3).
+    __device__ __forceinline__ void pair_packed(const uint32_t *tab, uint32_t &e0, uint32_t &e1) {
+        uint32_t cc = 0;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            uint32_t e = tab[xr >> 24];
+            uint32_t x1 = add_b1(mul_d_b0(xr, e), e);
+            bool c = x1 < 0x10000u;
+            xr = __builtin_amdgcn_perm(x1, W, c ? kSelRenorm : kSelKeep);
+            cc += c ? 1u : 0u;
+            (h == 0 ? e0 : e1) = e;
         }
         advance_word(cc);
     }

@@ -30,6 +30,8 @@
 
 #include <cstdint>
 
+#include "alu.h"
+
 namespace mv {
 namespace v1 {
 
@@ -38,42 +40,6 @@ constexpr uint32_t kSelRenorm8 = 0x00050401u;   // rotr(x1 << 16 | word, 8)
 constexpr uint32_t kSelKeep8 = 0x04070605u;     // rotr(x1, 8)
 constexpr uint32_t kSelRenorm12 = 0x05040100u;  // x1 << 16 | word
 constexpr uint32_t kSelKeep12 = 0x07060504u;    // x1
-
-// 24-bit multiplies on the low 24 bits of each operand (v_mul_u32_u24 / v_mad_u32_u24),
-// written out because the table words carry the symbol in their top byte and the compiler
-// would otherwise rebuild the masked products with full 32-bit multiplies.
-__device__ __forceinline__ uint32_t mul_u24(uint32_t a, uint32_t b) {
-    uint32_t r;
-    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
-__device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c) {
-    uint32_t r;
-    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
-    return r;
-}
-
-// Byte-field products of packed table words, one SDWA instruction each:
-// mul_b0_b0(a, b) = (a & 0xFF) * (b & 0xFF); mul_b1_b0(a, b) = ((a >> 8) & 0xFF) * (b & 0xFF);
-// add_b1(a, b) = a + ((b >> 8) & 0xFF).
-__device__ __forceinline__ uint32_t mul_b0_b0(uint32_t a, uint32_t b) {
-    uint32_t r;
-    asm("v_mul_u32_u24_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_0 "
-        "src1_sel:BYTE_0" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
-__device__ __forceinline__ uint32_t mul_b1_b0(uint32_t a, uint32_t b) {
-    uint32_t r;
-    asm("v_mul_u32_u24_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_1 "
-        "src1_sel:BYTE_0" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
-__device__ __forceinline__ uint32_t add_b1(uint32_t a, uint32_t b) {
-    uint32_t r;
-    asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD "
-        "src1_sel:BYTE_1" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
 
 template <int GW>
 struct Chunk;
@@ -238,6 +204,23 @@ struct Lane {
             x = __builtin_amdgcn_perm(x1, W, c ? kSelRenorm8 : kSelKeep8);
             cc += c ? 1u : 0u;
             (h == 0 ? e0 : e1) = ea;
+        }
+        advance_word(cc);
+    }
+
+    // P = 8, two steps, packed table T[q] = f | bias << 8 | sym << 24 (one LDS access per
+    // step): x1 = (x >> 8) * f + bias with the byte fields taken by SDWA selects.
+    // Returns the two table words (symbol in byte 3).
+    __device__ __forceinline__ void pair8_packed(const uint32_t *tab, uint32_t &e0, uint32_t &e1) {
+        uint32_t cc = 0;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            uint32_t e = tab[x >> 24];
+            uint32_t x1 = add_b1(mul_d_b0(x, e), e);
+            bool c = x1 < 0x10000u;
+            x = __builtin_amdgcn_perm(x1, W, c ? kSelRenorm8 : kSelKeep8);
+            cc += c ? 1u : 0u;
+            (h == 0 ? e0 : e1) = e;
         }
         advance_word(cc);
     }

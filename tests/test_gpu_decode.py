@@ -121,7 +121,7 @@ def test_llm_shapes(cuda, rows, cols, kind, p):
     assert np.array_equal(got[0], O.c_matvec(oracle_rec(cm), vecs[0], 16))
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_decode_variants_agree(cuda, golden_dir, variant):
     """Every decode kernel variant gives the oracle's bits (golden file + a ragged shape)."""
     note(f"decode variant {variant}")
