@@ -367,6 +367,68 @@ void matvans_st_free(matvans_st *st);
 int matvans_st_tensor(const matvans_st *st, const char *name, const char **dtype,
                       uint32_t *ndim, uint64_t *shape, const void **data, uint64_t *nbytes);
 
+/* ------------------------------------------------------------------------------
+ * N1. Persistent batched decode-matvec: all matrices of a batch (uneven shapes, mixed
+ * encodings, 1..8 vectors each) in ONE kernel launch whose blocks pull row tiles from a
+ * device work queue.  No reference counterpart: the reference launches one kernel per
+ * matrix (matvansWarpPerRow.cu:156-164; buffered/ans-warp-rows.cu:143-164) with a
+ * hand-set warpsPerBlock (:156).  The plan is created once (it allocates its device
+ * descriptors, tile table and queue counters); matvans_batch_launch allocates nothing,
+ * synchronises nothing and is one kernel launch, so it can be captured in a HIP graph.
+ * The queue resets itself at the end of every launch.  Launches of one plan must not
+ * overlap (they share the queue counters).
+ * ----------------------------------------------------------------------------*/
+enum {
+    MATVANS_BATCH_V0 = 0,        /* .bin v0 record: cursors, cdf, ppf, min_value, G, payload */
+    MATVANS_BATCH_QUAD = 1,      /* MVX1 quad coder, 32 or 64 streams: cursors, lut, payload */
+    MATVANS_BATCH_B_ROWS = 2,    /* family-B ans-warp-rows */
+    MATVANS_BATCH_B_STRIPES = 3  /* family-B ans-warp-stripes */
+};
+
+typedef struct {
+    uint32_t format;             /* MATVANS_BATCH_* */
+    uint32_t rows, cols;
+    uint32_t n_vec, vec_stride;  /* 1..8 vectors, vec_stride bytes apart */
+    const int8_t *vecs;          /* device */
+    int32_t *dst;                /* int32 [n_vec][rows] (v0, quad); family B: use acc32/out8 */
+    /* v0 and quad */
+    const uint32_t *cursors;
+    const uint16_t *payload;
+    uint32_t payload_size;
+    int32_t min_value;           /* v0 */
+    uint32_t G;                  /* v0 */
+    const uint8_t *cdf, *ppf;    /* v0 */
+    uint32_t streams;            /* quad: 32 or 64 */
+    const uint32_t *lut;         /* quad: the 2 * 256 dwords of matvans_v1_lut */
+    /* family B (as matvans_b_rows_matvec) */
+    const uint32_t *compressed;
+    uint64_t compressed_words;
+    const uint32_t *warp_offsets;
+    uint32_t end_offset;
+    const uint32_t *b_ppf, *b_cum, *b_prob;
+    float scale;
+    int32_t *acc32;
+    int8_t *out8;
+} matvans_batch_item;
+
+typedef struct {
+    uint32_t units_per_tile;     /* row groups per work-queue tile (1..64) */
+    uint32_t order;              /* 0 matrix-major, 1 round-robin, 2 longest chain first */
+    uint32_t blocks_per_cu;      /* persistent blocks per compute unit (1..32) */
+} matvans_batch_config;
+
+typedef struct matvans_batch matvans_batch;
+
+int matvans_batch_create(const matvans_batch_item *items, uint32_t n, matvans_batch **out);
+void matvans_batch_destroy(matvans_batch *plan);
+int matvans_batch_configure(matvans_batch *plan, const matvans_batch_config *cfg);
+int matvans_batch_get_config(const matvans_batch *plan, matvans_batch_config *cfg);
+int matvans_batch_launch(const matvans_batch *plan, void *stream);
+/* Autotuner: times units_per_tile x order x blocks_per_cu over `reps` launches each,
+ * keeps the fastest and returns it (and its ms per launch).  Synchronises. */
+int matvans_batch_autotune(matvans_batch *plan, void *stream, uint32_t reps,
+                           matvans_batch_config *best, float *best_ms);
+
 #ifdef __cplusplus
 }
 #endif

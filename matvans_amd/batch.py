@@ -1,0 +1,109 @@
+"""Persistent batched decode-matvec over the C ABI (matvans_batch_*): every matrix of a
+batch -- .bin v0, MVX1 quad coder, family-B rows / stripes, uneven shapes, 1..8 vectors
+each -- in one kernel launch fed by a device work queue, with an autotuner for the tile
+size, tile order and persistent grid.  No reference counterpart (the reference launches
+one kernel per matrix, matvansWarpPerRow.cu:156-164)."""
+from __future__ import annotations
+
+import ctypes
+
+from .buffered import BufferedMatrices
+from .capi import BatchConfig, BatchItem, call, lib
+from .mvx import CompressedMatrixV1
+
+__all__ = ["Batch"]
+
+V0, QUAD, B_ROWS, B_STRIPES = 0, 1, 2, 3
+
+
+def _p(t):
+    return t.data_ptr() if t is not None else None
+
+
+class Batch:
+    """entries: list of (matrix, vecs) where matrix is a CompressedMatrix (.bin v0), a
+    CompressedMatrixV1 with coder=1 and 32 or 64 streams, or a (BufferedMatrices, k)
+    pair; vecs is an int8 device tensor [n_vec, cols] (or [cols]).  All matrices must be
+    on the device.  ``outputs[i]`` is the int32 [n_vec, rows] result of entry i (family-B
+    entries also fill ``out8[i]`` = clamp(rint(acc * scale)))."""
+
+    def __init__(self, entries):
+        import torch
+        from . import CompressedMatrix
+        self._keep = []
+        self.outputs, self.out8 = [], []
+        arr = (BatchItem * len(entries))()
+        for i, (m, v) in enumerate(entries):
+            v = v.unsqueeze(0) if v.dim() == 1 else v
+            self._keep.append(v)
+            it = arr[i]
+            it.n_vec, it.vec_stride, it.vecs = v.shape[0], v.stride(0), v.data_ptr()
+            if isinstance(m, tuple):
+                bm, k = m
+                rows, cols = bm.shapes[k]
+                acc = torch.empty((v.shape[0], rows), dtype=torch.int32, device=v.device)
+                o8 = torch.empty((v.shape[0], rows), dtype=torch.int8, device=v.device)
+                it.format = B_ROWS if bm.layout == "rows" else B_STRIPES
+                it.compressed = _p(bm.d["compressed"])
+                it.compressed_words = bm.compressed.size
+                it.warp_offsets = _p(bm.d["offsets"][k])
+                it.end_offset = bm.end_offset(k)
+                it.b_ppf = bm.d["ppf"].data_ptr() + 128 * k
+                it.b_cum = bm.d["cum"].data_ptr() + 128 * k
+                it.b_prob = bm.d["prob"].data_ptr() + 128 * k
+                it.scale = float(bm.scales[k])
+                it.acc32, it.out8 = acc.data_ptr(), o8.data_ptr()
+                it.dst = acc.data_ptr()
+            else:
+                rows, cols = m.rows, m.cols
+                acc = torch.empty((v.shape[0], rows), dtype=torch.int32, device=v.device)
+                o8 = None
+                it.dst, it.cursors = acc.data_ptr(), _p(m.d_cursors)
+                it.payload, it.payload_size = _p(m.d_payload), m.payload_size
+                if isinstance(m, CompressedMatrix):
+                    it.format = V0
+                    it.min_value, it.G = m.min_value, m.G
+                    it.cdf, it.ppf = _p(m.d_cdf_data), _p(m.d_ppf_data)
+                elif isinstance(m, CompressedMatrixV1) and m.coder == 1:
+                    it.format, it.streams, it.lut = QUAD, m.streams, _p(m.d_lut)
+                else:
+                    raise ValueError("batch entries: .bin v0, MVX1 quad or family-B matrices")
+            it.rows, it.cols = rows, cols
+            self.outputs.append(acc)
+            self.out8.append(o8)
+            self._keep.append(m)
+        self._items = arr
+        h = ctypes.c_void_p()
+        call("matvans_batch_create", arr, len(entries), ctypes.byref(h))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().matvans_batch_destroy(h)
+            self._h = None
+
+    def configure(self, units_per_tile=1, order=0, blocks_per_cu=8):
+        cfg = BatchConfig(units_per_tile, order, blocks_per_cu)
+        call("matvans_batch_configure", self._h, ctypes.byref(cfg))
+
+    @property
+    def config(self):
+        cfg = BatchConfig()
+        call("matvans_batch_get_config", self._h, ctypes.byref(cfg))
+        return (cfg.units_per_tile, cfg.order, cfg.blocks_per_cu)
+
+    def launch(self, stream=None):
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        call("matvans_batch_launch", self._h, ctypes.c_void_p(s.cuda_stream))
+
+    def autotune(self, reps=5, stream=None):
+        """Returns ((units_per_tile, order, blocks_per_cu), ms per launch) of the best."""
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        cfg = BatchConfig()
+        ms = ctypes.c_float()
+        call("matvans_batch_autotune", self._h, ctypes.c_void_p(s.cuda_stream), reps,
+             ctypes.byref(cfg), ctypes.byref(ms))
+        return (cfg.units_per_tile, cfg.order, cfg.blocks_per_cu), ms.value

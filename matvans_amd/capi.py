@@ -53,6 +53,21 @@ class V1DevMatrix(ctypes.Structure):
                 ("payload_size", u32), ("cursors", P), ("lut", P), ("freq", P), ("payload", P)]
 
 
+class BatchItem(ctypes.Structure):
+    """matvans_batch_item."""
+    _fields_ = [("format", u32), ("rows", u32), ("cols", u32), ("n_vec", u32),
+                ("vec_stride", u32), ("vecs", P), ("dst", P), ("cursors", P), ("payload", P),
+                ("payload_size", u32), ("min_value", i32), ("G", u32), ("cdf", P), ("ppf", P),
+                ("streams", u32), ("lut", P), ("compressed", P), ("compressed_words", u64),
+                ("warp_offsets", P), ("end_offset", u32), ("b_ppf", P), ("b_cum", P),
+                ("b_prob", P), ("scale", f32), ("acc32", P), ("out8", P)]
+
+
+class BatchConfig(ctypes.Structure):
+    """matvans_batch_config."""
+    _fields_ = [("units_per_tile", u32), ("order", u32), ("blocks_per_cu", u32)]
+
+
 # name -> (restype, argtypes); status-returning functions use i32
 SIGNATURES = {
     "matvans_last_error": (ctypes.c_char_p, []),
@@ -103,6 +118,13 @@ SIGNATURES = {
     "matvans_b_encode_bound": (u64, [u32, u32, i32]),
     "matvans_b_encode": (i32, [P, u32, u32, i32, i32, P, u32, u32, P, u64, ctypes.POINTER(u64),
                                P]),
+    "matvans_batch_create": (i32, [ctypes.POINTER(BatchItem), u32, ctypes.POINTER(P)]),
+    "matvans_batch_destroy": (None, [P]),
+    "matvans_batch_configure": (i32, [P, ctypes.POINTER(BatchConfig)]),
+    "matvans_batch_get_config": (i32, [P, ctypes.POINTER(BatchConfig)]),
+    "matvans_batch_launch": (i32, [P, P]),
+    "matvans_batch_autotune": (i32, [P, P, u32, ctypes.POINTER(BatchConfig),
+                                     ctypes.POINTER(f32)]),
     "matvans_st_read": (i32, [ctypes.c_char_p, ctypes.POINTER(P)]),
     "matvans_st_free": (None, [P]),
     "matvans_st_tensor": (i32, [P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p),

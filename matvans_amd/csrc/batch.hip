@@ -1,0 +1,357 @@
+// Persistent batched fused decode-matvec: every matrix of a batch (uneven shapes, mixed
+// encodings, 1..8 vectors each) in one launch that pulls row tiles from a device work
+// queue.  The reference launches one kernel per matrix (matvansWarpPerRow.cu:156-164,
+// buffered/ans-warp-rows.cu:143-164); this has no counterpart there.
+//
+// Work items: a "unit" is the group of rows one 4-wave block decodes in the one-matrix
+// kernels (blk_v1.h, blk_b.h); a tile is `units_per_tile` consecutive units of one
+// matrix.  The tile table is built on the host in the configured order (matrix-major,
+// round-robin over matrices, or longest decode chain first) and copied to the device at
+// configuration time.  Blocks take tile numbers with one atomic add on a device counter;
+// a block that finds the queue empty adds one to a second counter, and the last block to
+// do so resets both, so the next launch (or graph replay) starts from zero without any
+// host-side or memset work on the launch path.
+//
+// The plan owns its device descriptors, tile table, counters and the decode tables it
+// builds for .bin v0 items; matvans_batch_launch allocates nothing and is one kernel
+// launch, so a caller can capture it in a HIP graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "blk_b.h"
+#include "blk_v1.h"
+#include "matvans_internal.h"
+#include "v0_stream.h"
+
+using mv::fail;
+
+namespace mv {
+namespace {
+
+// decode path of an item: v0, quad with 32 / 64 streams, family-B rows / stripes
+enum : uint32_t { kPathV0 = 0, kPathQ32 = 1, kPathQ64 = 2, kPathBRows = 3, kPathBStripes = 4 };
+
+struct Item {
+    uint32_t path;
+    uint32_t units;
+    blk::V1Job v1;
+    blk::BJob b;
+};
+
+struct Tile {
+    uint32_t item, first, count, pad;
+};
+
+template <int NV>
+union BatchSmem {
+    blk::V1Smem<NV, 32, 8, 0> v0;
+    blk::QuadSmem<NV, 32> q32;
+    blk::QuadSmem<NV, 64> q64;
+    blk::BSmem b;
+};
+
+constexpr uint32_t bit(uint32_t path) { return 1u << path; }
+// kernel instances: the decode paths each one contains (a plan picks the smallest one
+// that covers its items; fewer paths leave fewer registers live)
+constexpr uint32_t kMasks[] = {bit(kPathV0), bit(kPathQ32), bit(kPathQ64),
+                               bit(kPathQ32) | bit(kPathQ64),
+                               bit(kPathBRows) | bit(kPathBStripes), 0x1Fu};
+
+template <int NV, uint32_t MASK>
+__global__ __launch_bounds__(256) void batch_fwd(const Item *__restrict__ items,
+                                                 const Tile *__restrict__ tiles, uint32_t n_tiles,
+                                                 uint32_t *counters) {
+    __shared__ BatchSmem<NV> sm;
+    __shared__ uint32_t s_tile;
+    const uint32_t tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) s_tile = atomicAdd(&counters[0], 1u);
+        __syncthreads();
+        const uint32_t t = s_tile;
+        __syncthreads();
+        if (t >= n_tiles) break;
+        const Tile tl = tiles[t];
+        const Item &it = items[tl.item];
+        const uint32_t path = __builtin_amdgcn_readfirstlane(it.path);
+        for (uint32_t u = 0; u < tl.count; u++) {
+            const uint32_t bid = tl.first + u;
+            if ((MASK & bit(kPathV0)) && path == kPathV0)
+                blk::v1_block<NV, 32, 8, 0>(it.v1, bid, sm.v0);
+            else if ((MASK & bit(kPathQ32)) && path == kPathQ32)
+                blk::quad_block<NV, 32>(it.v1, bid, sm.q32);
+            else if ((MASK & bit(kPathQ64)) && path == kPathQ64)
+                blk::quad_block<NV, 64>(it.v1, bid, sm.q64);
+            else if ((MASK & bit(kPathBRows)) && path == kPathBRows)
+                blk::b_block<NV, 0>(it.b, bid, sm.b);
+            else if ((MASK & bit(kPathBStripes)) && path == kPathBStripes)
+                blk::b_block<NV, 1>(it.b, bid, sm.b);
+            __syncthreads();   // the next unit rebuilds the tables and rings
+        }
+    }
+    if (tid == 0) {
+        __threadfence();
+        uint32_t d = atomicAdd(&counters[1], 1u);
+        if (d == gridDim.x - 1) {   // every other block has left the loop
+            atomicExch(&counters[0], 0u);
+            atomicExch(&counters[1], 0u);
+        }
+    }
+}
+
+// A / B decode tables of a .bin v0 item from its cdf and ppf (the fused entries of the
+// one-matrix kernels, v0_stream.h).
+__global__ void v0_tables(uint32_t *out, const uint8_t *cdf, const uint8_t *ppf, int min_value,
+                          uint32_t G) {
+    uint32_t q = threadIdx.x, f, sym, bias;
+    v0_table_entry(q, min_value, G, cdf, ppf, f, sym, bias);
+    out[q] = f | (sym << 24);
+    out[256 + q] = bias;
+}
+
+}  // namespace
+}  // namespace mv
+
+struct matvans_batch {
+    std::vector<mv::Item> items;          // host copies (device pointers inside)
+    std::vector<uint32_t> chain;          // decode-chain length per unit, for ordering
+    mv::Item *d_items = nullptr;
+    mv::Tile *d_tiles = nullptr;
+    uint32_t *d_counters = nullptr;
+    uint32_t *d_tables = nullptr;         // v0 decode tables, 512 dwords per v0 item
+    uint32_t n_tiles = 0, max_tiles = 0, nv = 1, n_cu = 0, mask = 0, kernel = 5;
+    matvans_batch_config cfg{1, 0, 8};
+};
+
+namespace {
+
+hipError_t build_tiles(matvans_batch *p) {
+    struct T { uint32_t item, first, count, chain; };
+    std::vector<T> t;
+    const uint32_t k = std::max(1u, p->cfg.units_per_tile);
+    for (uint32_t i = 0; i < p->items.size(); i++)
+        for (uint32_t u = 0; u < p->items[i].units; u += k)
+            t.push_back({i, u, std::min(k, p->items[i].units - u), p->chain[i]});
+    if (p->cfg.order == 1) {            // round-robin over matrices
+        std::stable_sort(t.begin(), t.end(), [&](const T &a, const T &b) {
+            uint32_t ra = a.first / k, rb = b.first / k;
+            return ra != rb ? ra < rb : a.item < b.item;
+        });
+    } else if (p->cfg.order == 2) {     // longest decode chain first
+        std::stable_sort(t.begin(), t.end(),
+                         [](const T &a, const T &b) { return a.chain * a.count > b.chain * b.count; });
+    }
+    std::vector<mv::Tile> h(t.size());
+    for (size_t i = 0; i < t.size(); i++) h[i] = {t[i].item, t[i].first, t[i].count, 0};
+    p->n_tiles = uint32_t(h.size());
+    return hipMemcpy(p->d_tiles, h.data(), h.size() * sizeof(mv::Tile), hipMemcpyHostToDevice);
+}
+
+int check_item(const matvans_batch_item &it, uint32_t k) {
+    const bool b = it.format == MATVANS_BATCH_B_ROWS || it.format == MATVANS_BATCH_B_STRIPES;
+    if (!it.vecs || (!b && !it.dst)) return fail(MATVANS_E_INVALID, "item %u: null vecs or dst", k);
+    if (it.n_vec < 1 || it.n_vec > 8) return fail(MATVANS_E_INVALID, "item %u: n_vec %u", k, it.n_vec);
+    if (!it.rows || !it.cols) return fail(MATVANS_E_SHAPE, "item %u: empty matrix", k);
+    uint32_t stride = it.n_vec == 1 && it.vec_stride < it.cols ? it.cols : it.vec_stride;
+    if (stride < it.cols) return fail(MATVANS_E_INVALID, "item %u: vec_stride < cols", k);
+    uint32_t align = b ? 4 : 16;
+    if (reinterpret_cast<uintptr_t>(it.vecs) % align || (it.n_vec > 1 && stride % align))
+        return fail(MATVANS_E_ALIGN, "item %u: vectors must be %u-byte aligned", k, align);
+    switch (it.format) {
+        case MATVANS_BATCH_V0:
+            if (it.cols % 512) return fail(MATVANS_E_SHAPE, "item %u: cols %% 512", k);
+            if (!it.cursors || !it.cdf || !it.ppf || !it.payload)
+                return fail(MATVANS_E_INVALID, "item %u: null v0 arrays", k);
+            break;
+        case MATVANS_BATCH_QUAD:
+            if (it.streams != 32 && it.streams != 64)
+                return fail(MATVANS_E_UNSUPPORTED, "item %u: quad items need 32 or 64 streams", k);
+            if (it.cols % (16 * it.streams)) return fail(MATVANS_E_SHAPE, "item %u: cols", k);
+            if (!it.cursors || !it.lut || !it.payload)
+                return fail(MATVANS_E_INVALID, "item %u: null quad arrays", k);
+            break;
+        case MATVANS_BATCH_B_ROWS:
+        case MATVANS_BATCH_B_STRIPES:
+            if (it.format == MATVANS_BATCH_B_ROWS ? it.cols % 128 : (it.rows % 32 || it.cols % 4))
+                return fail(MATVANS_E_SHAPE, "item %u: shape does not fit the layout", k);
+            if (!it.compressed || !it.warp_offsets || !it.b_ppf || !it.b_cum || !it.b_prob)
+                return fail(MATVANS_E_INVALID, "item %u: null family-B arrays", k);
+            if (!it.acc32 && !it.out8) return fail(MATVANS_E_INVALID, "item %u: no output", k);
+            break;
+        default:
+            return fail(MATVANS_E_INVALID, "item %u: format %u", k, it.format);
+    }
+    if ((it.payload && reinterpret_cast<uintptr_t>(it.payload) % 16) ||
+        (it.compressed && reinterpret_cast<uintptr_t>(it.compressed) % 16))
+        return fail(MATVANS_E_ALIGN, "item %u: compressed data not 16-byte aligned", k);
+    return MATVANS_OK;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    return e == hipSuccess ? MATVANS_OK : fail(MATVANS_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+}  // namespace
+
+extern "C" int matvans_batch_create(const matvans_batch_item *items, uint32_t n,
+                                    matvans_batch **out) {
+    if (!items || !n || !out) return fail(MATVANS_E_INVALID, "null argument or empty batch");
+    *out = nullptr;
+    uint32_t n_v0 = 0, nv = 1;
+    for (uint32_t k = 0; k < n; k++) {
+        if (int st = check_item(items[k], k)) return st;
+        n_v0 += items[k].format == MATVANS_BATCH_V0;
+        nv = std::max(nv, items[k].n_vec);
+    }
+    auto *p = new matvans_batch();
+    p->nv = nv <= 1 ? 1 : nv <= 2 ? 2 : nv <= 4 ? 4 : 8;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) { delete p; return hip_fail(e, "device query"); }
+    p->n_cu = uint32_t(prop.multiProcessorCount);
+    if (n_v0) e = hipMalloc(&p->d_tables, size_t(n_v0) * 512 * sizeof(uint32_t));
+    uint32_t v0_i = 0, max_tiles = 0;
+    for (uint32_t k = 0; e == hipSuccess && k < n; k++) {
+        const auto &it = items[k];
+        mv::Item m{};
+        uint32_t stride = it.n_vec == 1 && it.vec_stride < it.cols ? it.cols : it.vec_stride;
+        if (it.format == MATVANS_BATCH_V0 || it.format == MATVANS_BATCH_QUAD) {
+            const uint32_t *lut = it.lut;
+            if (it.format == MATVANS_BATCH_V0) {
+                uint32_t *tab = p->d_tables + size_t(v0_i++) * 512;
+                hipLaunchKernelGGL(mv::v0_tables, dim3(1), dim3(256), 0, 0, tab, it.cdf, it.ppf,
+                                   int(it.min_value), it.G);
+                e = hipGetLastError();
+                lut = tab;
+                m.path = mv::kPathV0;
+                m.units = (it.rows + 7) / 8;
+                p->chain.push_back(it.cols / 32);
+            } else {
+                m.path = it.streams == 32 ? mv::kPathQ32 : mv::kPathQ64;
+                m.units = it.streams == 32 ? (it.rows + 7) / 8 : (it.rows + 3) / 4;
+                p->chain.push_back(it.cols / it.streams / 4);
+            }
+            m.v1 = mv::blk::V1Job{it.dst, it.vecs, stride, it.n_vec, it.rows, it.cols, it.cursors,
+                                  it.min_value, it.G, lut, nullptr, it.payload, it.payload_size};
+        } else {
+            const bool rows = it.format == MATVANS_BATCH_B_ROWS;
+            m.path = rows ? mv::kPathBRows : mv::kPathBStripes;
+            uint32_t warps = rows ? it.rows : it.rows / 32;
+            m.units = (warps + 7) / 8;
+            p->chain.push_back(rows ? it.cols / 128 : it.cols / 4);
+            m.b = mv::blk::BJob{it.acc32, it.out8, it.scale, it.compressed,
+                                uint32_t(std::min<uint64_t>(it.compressed_words, 0xFFFFFFF0ull)),
+                                it.warp_offsets, it.end_offset, it.b_ppf, it.b_cum, it.b_prob,
+                                it.vecs, stride, it.n_vec, it.cols, it.rows};
+        }
+        max_tiles += m.units;
+        p->mask |= 1u << m.path;
+        p->items.push_back(m);
+    }
+    if (e == hipSuccess) e = hipMalloc(&p->d_items, sizeof(mv::Item) * n);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->d_items, p->items.data(), sizeof(mv::Item) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&p->d_tiles, sizeof(mv::Tile) * max_tiles);
+    if (e == hipSuccess) e = hipMalloc(&p->d_counters, 2 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(p->d_counters, 0, 2 * sizeof(uint32_t));
+    p->max_tiles = max_tiles;
+    for (uint32_t i = 0; i < 6; i++)
+        if ((mv::kMasks[i] & p->mask) == p->mask) { p->kernel = i; break; }
+    if (e == hipSuccess) e = build_tiles(p);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        matvans_batch_destroy(p);
+        return hip_fail(e, "batch plan");
+    }
+    *out = p;
+    mv::clear_error();
+    return MATVANS_OK;
+}
+
+extern "C" void matvans_batch_destroy(matvans_batch *p) {
+    if (!p) return;
+    hipFree(p->d_items);
+    hipFree(p->d_tiles);
+    hipFree(p->d_counters);
+    hipFree(p->d_tables);
+    delete p;
+}
+
+extern "C" int matvans_batch_configure(matvans_batch *p, const matvans_batch_config *cfg) {
+    if (!p || !cfg) return fail(MATVANS_E_INVALID, "null argument");
+    if (cfg->units_per_tile < 1 || cfg->units_per_tile > 64 || cfg->order > 2 ||
+        cfg->blocks_per_cu < 1 || cfg->blocks_per_cu > 32)
+        return fail(MATVANS_E_INVALID, "config out of range");
+    p->cfg = *cfg;
+    return hip_fail(build_tiles(p), "tile table");
+}
+
+extern "C" int matvans_batch_get_config(const matvans_batch *p, matvans_batch_config *cfg) {
+    if (!p || !cfg) return fail(MATVANS_E_INVALID, "null argument");
+    *cfg = p->cfg;
+    return MATVANS_OK;
+}
+
+extern "C" int matvans_batch_launch(const matvans_batch *p, void *stream) {
+    if (!p) return fail(MATVANS_E_INVALID, "null plan");
+    const uint32_t grid = std::max(1u, std::min(p->n_tiles, p->n_cu * p->cfg.blocks_per_cu));
+    auto s = reinterpret_cast<hipStream_t>(stream);
+    using K = void (*)(const mv::Item *, const mv::Tile *, uint32_t, uint32_t *);
+#define MV_ROW(NV) {mv::batch_fwd<NV, mv::kMasks[0]>, mv::batch_fwd<NV, mv::kMasks[1]>, \
+                    mv::batch_fwd<NV, mv::kMasks[2]>, mv::batch_fwd<NV, mv::kMasks[3]>, \
+                    mv::batch_fwd<NV, mv::kMasks[4]>, mv::batch_fwd<NV, mv::kMasks[5]>}
+    static const K table[4][6] = {MV_ROW(1), MV_ROW(2), MV_ROW(4), MV_ROW(8)};
+#undef MV_ROW
+    const int row = p->nv == 1 ? 0 : p->nv == 2 ? 1 : p->nv == 4 ? 2 : 3;
+    hipLaunchKernelGGL(table[row][p->kernel], dim3(grid), dim3(256), 0, s, p->d_items, p->d_tiles,
+                       p->n_tiles, p->d_counters);
+    return hip_fail(hipGetLastError(), "batch launch");
+}
+
+// Times every configuration of a small grid (units per tile x tile order x blocks per
+// CU) over `reps` launches each on `stream` and keeps the fastest.  Synchronises; not for
+// use inside graph capture.  No reference counterpart (the closest is the hand-set
+// warpsPerBlock of matvansWarpPerRow.cu:156).
+extern "C" int matvans_batch_autotune(matvans_batch *p, void *stream, uint32_t reps,
+                                      matvans_batch_config *best_out, float *best_ms) {
+    if (!p) return fail(MATVANS_E_INVALID, "null plan");
+    reps = std::max(1u, reps);
+    auto s = reinterpret_cast<hipStream_t>(stream);
+    hipEvent_t a, b;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e != hipSuccess) return hip_fail(e, "event");
+    matvans_batch_config best = p->cfg;
+    float best_t = 1e30f;
+    const uint32_t units[] = {1, 2, 4}, orders[] = {0, 1, 2}, bpc[] = {4, 8};
+    int st = MATVANS_OK;
+    for (uint32_t u : units)
+        for (uint32_t o : orders)
+            for (uint32_t c : bpc) {
+                matvans_batch_config cfg{u, o, c};
+                if ((st = matvans_batch_configure(p, &cfg))) goto done;
+                if ((st = matvans_batch_launch(p, stream))) goto done;   // warm-up
+                hipEventRecord(a, s);
+                for (uint32_t r = 0; r < reps && !st; r++) st = matvans_batch_launch(p, stream);
+                hipEventRecord(b, s);
+                if (st) goto done;
+                if ((e = hipEventSynchronize(b)) != hipSuccess) { st = hip_fail(e, "sync"); goto done; }
+                float ms = 0;
+                hipEventElapsedTime(&ms, a, b);
+                ms /= float(reps);
+                if (ms < best_t) { best_t = ms; best = cfg; }
+            }
+done:
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (st) return st;
+    if ((st = matvans_batch_configure(p, &best))) return st;
+    if (best_out) *best_out = best;
+    if (best_ms) *best_ms = best_t;
+    mv::clear_error();
+    return MATVANS_OK;
+}

@@ -1,0 +1,412 @@
+// Block-level bodies of the MVX1 fused decode-matvec kernels (one block = 4 waves over
+// a group of rows), shared by the one-matrix kernels (decode_v1.hip) and the persistent
+// batched kernel (batch.hip).  The batched kernel calls them once per tile it takes from
+// the work queue, with bid = the tile's block index within its matrix.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "v1_stream.h"
+
+namespace mv {
+namespace blk {
+
+using v1::Lane;
+
+constexpr int kWaves = 4;
+
+__device__ __forceinline__ uint32_t dword_of(const uint4 &v, int d) {
+    return d == 0 ? v.x : d == 1 ? v.y : d == 2 ? v.z : v.w;
+}
+
+// Exclusive scans over the 64 lanes of a wave (lane 0 gets 0).
+__device__ __forceinline__ uint32_t wave_excl_sum(uint32_t v, uint32_t lane) {
+    uint32_t inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(inc, d, 64);
+        inc += lane >= uint32_t(d) ? t : 0u;
+    }
+    return inc - v;
+}
+__device__ __forceinline__ uint32_t wave_excl_max(uint32_t v, uint32_t lane) {
+    uint32_t inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(inc, d, 64);
+        inc = lane >= uint32_t(d) && t > inc ? t : inc;
+    }
+    uint32_t prev = __shfl_up(inc, 1, 64);
+    return lane ? prev : 0u;
+}
+
+// One MVX1 matrix with its vectors and output (device pointers).  n_vec <= the kernel's
+// NV: vectors past n_vec are neither read nor written.
+struct V1Job {
+    int32_t *dst;            // [n_vec][rows]
+    const int8_t *vecs;      // [n_vec][vec_stride]
+    uint32_t vec_stride, n_vec;
+    uint32_t rows, cols;
+    const uint32_t *cursors;
+    int32_t min_value;
+    uint32_t G;
+    const uint32_t *glut;    // A / B tables (2 * 2^P dwords), table mode 0
+    const uint8_t *freq;     // per-row frequencies, table mode 1
+    const uint16_t *payload;
+    uint32_t payload_size;
+};
+
+// Rows a block covers for S streams per row.
+template <int S>
+constexpr uint32_t rows_per_block() { return S > 64 ? kWaves / (S / 64) : kWaves * (64 / S); }
+
+template <int NV, int S, int P, int MODE>
+struct V1Smem {
+    static constexpr int R = P == 8 ? 32 : 64;
+    static constexpr int WPR = S > 64 ? S / 64 : 1;
+    uint16_t ring[kWaves * R * 64];
+    uint32_t lut[MODE == 1 ? 1 : 2 << P];
+    uint32_t rowtab[MODE == 1 ? kWaves * 384 : 1];   // per wave: f|bias|sym + scratch
+    uint16_t dummy[8 * 64];
+    int32_t part[WPR > 1 ? kWaves * NV : 1];
+};
+
+template <int NV, int S>
+struct QuadSmem {
+    static constexpr int WPR = S > 64 ? S / 64 : 1;
+    uint32_t ring[kWaves * 16 * 64];
+    uint32_t tab[256];
+    uint32_t dummy[4 * 64];
+    int32_t part[WPR > 1 ? kWaves * NV : 1];
+};
+
+// Single-symbol coder (MVX1 coder 0; the reference's .bin v0 layout is S = 32, P = 8).
+template <int NV, int S, int P, int MODE>
+__device__ __forceinline__ void v1_block(const V1Job &J, uint32_t bid, V1Smem<NV, S, P, MODE> &sm) {
+    constexpr int RPW = S < 64 ? 64 / S : 1;   // rows per wave
+    constexpr int WPR = S > 64 ? S / 64 : 1;   // waves per row
+    constexpr int GW = 256 / S;                // words per lane per granule
+    constexpr int R = P == 8 ? 32 : 64;        // ring words per lane
+    constexpr int kLut = MODE == 1 ? 1 : 2 << P;   // dwords of the shared table
+    uint16_t *ring = sm.ring;
+    uint32_t *lut = sm.lut;
+    uint32_t *rowtab = sm.rowtab;
+    uint16_t *dummy = sm.dummy;
+    int32_t *part = sm.part;
+    const uint32_t rows = J.rows, cols = J.cols, G = J.G, vec_stride = J.vec_stride;
+    const uint32_t payload_size = J.payload_size;
+    const int min_value = J.min_value;
+    const uint32_t *__restrict__ cursors = J.cursors;
+    const uint32_t *__restrict__ glut = J.glut;
+    const uint8_t *__restrict__ freq = J.freq;
+    const uint16_t *__restrict__ payload = J.payload;
+    const int8_t *__restrict__ vecs = J.vecs;
+    int32_t *__restrict__ dst = J.dst;
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wave = tid >> 6, lane = tid & 63;
+    // stream slot within the row, and the rows of this wave
+    const uint32_t sl = S >= 64 ? (wave % WPR) * 64 + lane : lane & (S - 1);
+    const uint32_t row0 = S >= 64 ? bid * (kWaves / WPR) + wave / WPR
+                                  : (bid * kWaves + wave) * RPW;
+    const uint32_t row = row0 + (S >= 64 ? 0u : lane / S);
+    const bool valid = row < rows;
+
+    if constexpr (MODE == 0 && P == 8) {
+        // packed table T[q] = f | bias << 8 | sym << 24 from A = f | sym << 24, B = bias
+        // (f <= 255 whenever the alphabet has two or more symbols, as the model ensures)
+        lut[tid] = (glut[tid] & 0xFF0000FFu) | ((glut[256 + tid] & 0xFFu) << 8);
+    } else if constexpr (MODE == 0) {
+        const uint4 *src = reinterpret_cast<const uint4 *>(glut);
+        uint4 *d = reinterpret_cast<uint4 *>(lut);
+        for (uint32_t i = tid; i < uint32_t(kLut / 4); i += 256) d[i] = src[i];
+    } else {
+        // one table per row; S == 64, so the wave owns exactly this row
+        uint8_t *tab = reinterpret_cast<uint8_t *>(rowtab + wave * 384);
+        uint8_t *mark = tab + 768, *sf = tab + 1024, *sc = tab + 1280;
+        uint32_t f[4], c[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            uint32_t s = lane * 4 + k;
+            f[k] = valid && s < G ? freq[size_t(row) * G + s] : 0u;
+        }
+        c[0] = 0; c[1] = f[0]; c[2] = f[0] + f[1]; c[3] = c[2] + f[2];
+        uint32_t e = wave_excl_sum(c[3] + f[3], lane);
+        uint32_t fw = 0, cw = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            c[k] = (c[k] + e) & 0xFFu;
+            fw |= f[k] << (8 * k);
+            cw |= c[k] << (8 * k);
+        }
+        reinterpret_cast<uint32_t *>(mark)[lane] = 0;
+        reinterpret_cast<uint32_t *>(sf)[lane] = fw;
+        reinterpret_cast<uint32_t *>(sc)[lane] = cw;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (f[k]) mark[c[k]] = static_cast<uint8_t>(lane * 4 + k);
+        __syncthreads();
+        uint32_t m = reinterpret_cast<const uint32_t *>(mark)[lane];
+        uint32_t a[4];
+        a[0] = m & 0xFFu;
+#pragma unroll
+        for (int k = 1; k < 4; k++) {
+            uint32_t b = (m >> (8 * k)) & 0xFFu;
+            a[k] = b > a[k - 1] ? b : a[k - 1];
+        }
+        uint32_t pm = wave_excl_max(a[3], lane);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            uint32_t q = lane * 4 + k;
+            uint32_t s = a[k] > pm ? a[k] : pm;
+            tab[q] = sf[s];
+            tab[256 + q] = static_cast<uint8_t>(q - sc[s]);
+            tab[512 + q] = static_cast<uint8_t>(min_value + int(s));
+        }
+    }
+
+    // wave window: rows row0 .. row0 + RPW - 1
+    uint32_t wbeg = 0, wend = 0;
+    if (row0 < rows) {
+        wbeg = cursors[row0];
+        uint32_t last = row0 + RPW - 1 < rows ? row0 + RPW - 1 : rows - 1;
+        wend = last + 1 < rows ? cursors[last + 1] : payload_size;
+        if (wend > payload_size) wend = payload_size;
+        if (wend < wbeg) wend = wbeg;
+    }
+    wbeg = __builtin_amdgcn_readfirstlane(wbeg);
+    wend = __builtin_amdgcn_readfirstlane(wend);
+    const uint32_t rcur = valid ? cursors[row] : 0u;
+    const uint32_t rend = valid ? (row + 1 < rows ? cursors[row + 1] : payload_size) : 0u;
+
+    Lane<GW, P, R> st;
+    st.init(payload, wbeg, wend, valid, rcur, rend, sl, ring, dummy, wave, lane);
+    st.prologue();
+
+    const uint32_t groups = cols / (16u * S);
+    auto vload = [&](uint32_t t, uint4 *c) {
+        uint32_t tt = t < groups ? t : groups - 1;
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+            c[v] = *reinterpret_cast<const uint4 *>(vecs +
+                                                    size_t(v < int(J.n_vec) ? v : 0) * vec_stride +
+                                                    tt * (16u * S) + sl * 16u);
+    };
+    uint4 vc0[NV], vc1[NV];
+    vload(0, vc0);
+    vload(1, vc1);
+    __syncthreads();
+    st.start();
+
+    int32_t acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) acc[v] = 0;
+    const uint8_t *tab = reinterpret_cast<const uint8_t *>(rowtab + wave * 384);
+
+    auto group = [&](const uint4 *vc) {
+        uint32_t wbefore = st.waddr;
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            uint32_t e0, e1, e2, e3, packed;
+            if constexpr (MODE == 1) {
+                st.pair8_u8(tab, e0, e1);
+                st.pair8_u8(tab, e2, e3);
+                packed = __builtin_amdgcn_perm(e1, e0, 0x0c0c0400u) |
+                         __builtin_amdgcn_perm(e3, e2, 0x04000c0cu);
+            } else {
+                if constexpr (P == 8) {
+                    st.pair8_packed(lut, e0, e1);
+                    st.pair8_packed(lut, e2, e3);
+                } else {
+                    e0 = st.step12(lut);
+                    e1 = st.step12(lut);
+                    e2 = st.step12(lut);
+                    e3 = st.step12(lut);
+                }
+                packed = __builtin_amdgcn_perm(e1, e0, 0x0c0c0703u) |
+                         __builtin_amdgcn_perm(e3, e2, 0x07030c0cu);
+            }
+#pragma unroll
+            for (int v = 0; v < NV; v++)
+                acc[v] = __builtin_amdgcn_sdot4(static_cast<int>(packed),
+                                                static_cast<int>(dword_of(vc[v], d)), acc[v],
+                                                false);
+        }
+        st.end_group(wbefore);
+    };
+
+    uint32_t t = 0;
+    for (; t + 1 < groups; t += 2) {
+        st.refill_a();
+        group(vc0);
+        vload(t + 2, vc0);
+        st.refill_b();
+        group(vc1);
+        vload(t + 3, vc1);
+    }
+    if (t < groups) {
+        st.refill_a();
+        group(vc0);
+    }
+
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        int32_t a = acc[v];
+        if constexpr (S >= 64) a += __shfl_xor(a, 32, 64);
+        a += __shfl_xor(a, 16, 64);
+        a += __shfl_xor(a, 8, 64);
+        a += __shfl_xor(a, 4, 64);
+        a += __shfl_xor(a, 2, 64);
+        a += __shfl_xor(a, 1, 64);
+        acc[v] = a;
+    }
+    if constexpr (WPR > 1) {
+        // the row's waves are consecutive in the block: add their totals in wave order
+        if (lane == 0) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) part[wave * NV + v] = acc[v];
+        }
+        __syncthreads();
+        if (lane == 0 && wave % WPR == 0 && valid) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                int32_t a = 0;
+#pragma unroll
+                for (int w = 0; w < WPR; w++) a += part[(wave + w) * NV + v];
+                if (v < int(J.n_vec)) dst[size_t(v) * rows + row] = a;
+            }
+        }
+    } else if ((lane & (S - 1)) == 0 && valid) {
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+            if (v < int(J.n_vec)) dst[size_t(v) * rows + row] = acc[v];
+    }
+}
+
+// Quad coder (MVX1 coder 1, P = 8, one table per matrix): same mapping, per-lane 64-bit
+// state; a 16-symbol group is four quads, each one v_dot4 per vector.
+template <int NV, int S>
+__device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid, QuadSmem<NV, S> &sm) {
+    constexpr int RPW = S < 64 ? 64 / S : 1;
+    constexpr int WPR = S > 64 ? S / 64 : 1;
+    constexpr int G4 = 128 / S;
+    using QL = v1::QuadLane<G4>;
+    uint32_t *ring = sm.ring;
+    uint32_t *tab = sm.tab;
+    uint32_t *dummy = sm.dummy;
+    int32_t *part = sm.part;
+    const uint32_t rows = J.rows, cols = J.cols, vec_stride = J.vec_stride;
+    const uint32_t payload_size = J.payload_size;
+    const uint32_t *__restrict__ cursors = J.cursors;
+    const uint32_t *__restrict__ glut = J.glut;
+    const uint16_t *__restrict__ payload = J.payload;
+    const int8_t *__restrict__ vecs = J.vecs;
+    int32_t *__restrict__ dst = J.dst;
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wave = tid >> 6, lane = tid & 63;
+    const uint32_t sl = S >= 64 ? (wave % WPR) * 64 + lane : lane & (S - 1);
+    const uint32_t row0 = S >= 64 ? bid * (kWaves / WPR) + wave / WPR
+                                  : (bid * kWaves + wave) * RPW;
+    const uint32_t row = row0 + (S >= 64 ? 0u : lane / S);
+    const bool valid = row < rows;
+    // packed table T[q] = f | r << 8 | sym << 24 from the A / B tables (A = f | sym << 24,
+    // B = r); f <= 255 and r < f for every table the encoder accepts
+    tab[tid] = (glut[tid] & 0xFF0000FFu) | ((glut[256 + tid] & 0xFFu) << 8);
+    uint32_t wbeg = 0, wend = 0;
+    if (row0 < rows) {
+        wbeg = cursors[row0];
+        uint32_t last = row0 + RPW - 1 < rows ? row0 + RPW - 1 : rows - 1;
+        wend = last + 1 < rows ? cursors[last + 1] : payload_size;
+        if (wend > payload_size) wend = payload_size;
+        if (wend < wbeg) wend = wbeg;
+    }
+    wbeg = __builtin_amdgcn_readfirstlane(wbeg);
+    wend = __builtin_amdgcn_readfirstlane(wend);
+    const uint32_t rcur = valid ? cursors[row] : 0u;
+    const uint32_t rend = valid ? (row + 1 < rows ? cursors[row + 1] : payload_size) : 0u;
+
+    QL st;
+    st.init(payload, wbeg, wend, valid, rcur, rend, sl, ring, dummy, wave, lane);
+    st.prologue();
+
+    const uint32_t groups = cols / (16u * S);
+    auto vload = [&](uint32_t t, uint4 *c) {
+        uint32_t tt = t < groups ? t : groups - 1;
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+            c[v] = *reinterpret_cast<const uint4 *>(vecs +
+                                                    size_t(v < int(J.n_vec) ? v : 0) * vec_stride +
+                                                    tt * (16u * S) + sl * 16u);
+    };
+    uint4 vc0[NV], vc1[NV];
+    vload(0, vc0);
+    vload(1, vc1);
+    __syncthreads();
+    st.start();
+
+    int32_t acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) acc[v] = 0;
+    auto group = [&](const uint4 *vc) {
+        uint32_t wbefore = st.waddr;
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            uint32_t packed = st.quad(tab);
+#pragma unroll
+            for (int v = 0; v < NV; v++)
+                acc[v] = __builtin_amdgcn_sdot4(static_cast<int>(packed),
+                                                static_cast<int>(dword_of(vc[v], d)), acc[v],
+                                                false);
+        }
+        st.end_group(wbefore);
+    };
+    uint32_t t = 0;
+    for (; t + 1 < groups; t += 2) {
+        st.refill_a();
+        group(vc0);
+        vload(t + 2, vc0);
+        st.refill_b();
+        group(vc1);
+        vload(t + 3, vc1);
+    }
+    if (t < groups) {
+        st.refill_a();
+        group(vc0);
+    }
+
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        int32_t a = acc[v];
+        if constexpr (S >= 64) a += __shfl_xor(a, 32, 64);
+        a += __shfl_xor(a, 16, 64);
+        a += __shfl_xor(a, 8, 64);
+        a += __shfl_xor(a, 4, 64);
+        a += __shfl_xor(a, 2, 64);
+        a += __shfl_xor(a, 1, 64);
+        acc[v] = a;
+    }
+    if constexpr (WPR > 1) {
+        if (lane == 0) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) part[wave * NV + v] = acc[v];
+        }
+        __syncthreads();
+        if (lane == 0 && wave % WPR == 0 && valid) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                int32_t a = 0;
+#pragma unroll
+                for (int w = 0; w < WPR; w++) a += part[(wave + w) * NV + v];
+                if (v < int(J.n_vec)) dst[size_t(v) * rows + row] = a;
+            }
+        }
+    } else if ((lane & (S - 1)) == 0 && valid) {
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+            if (v < int(J.n_vec)) dst[size_t(v) * rows + row] = acc[v];
+    }
+}
+
+}  // namespace blk
+}  // namespace mv

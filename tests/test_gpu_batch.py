@@ -1,0 +1,150 @@
+"""GPU tests of the persistent batched decode-matvec (matvans_batch_*): bit-exact against
+the integer product and the one-matrix launches, for a decoder block and an uneven mixed
+batch; self-resetting work queue across launches; HIP-graph replays; autotuner."""
+import sys
+
+import numpy as np
+import pytest
+
+import matvans_amd as mv
+from matvans_amd.batch import Batch
+from matvans_amd.buffered import BufferedMatrices
+from matvans_amd.capi import MatvansError
+from matvans_amd.mvx import CompressedMatrixV1
+from oracle import family_b as B
+
+from .helpers import gaussian_int8, uniform_vec
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    sys.stdout.write(f"[case] {msg}\n")
+    sys.stdout.flush()
+
+
+def want(m, vecs):
+    return (m.astype(np.int64) @ np.asarray(vecs, np.int64).T).T.astype(np.int32)
+
+
+def vecs_for(cols, n, seed):
+    return np.stack([uniform_vec(cols, seed=seed + i) for i in range(n)])
+
+
+# decoder-block shapes at 1/2 scale (4096 x 14336 block): q, k, v, o, gate, up, down
+BLOCK = [(2048, 2048), (512, 2048), (512, 2048), (2048, 2048), (7168, 2048), (7168, 2048),
+         (2048, 7168)]
+
+
+def make(fmt, m):
+    if fmt == "v0":
+        return mv.CompressedMatrix.encode(m).to_device()
+    S = 32 if fmt == "q32" else 64
+    return CompressedMatrixV1.encode(m, S, 8, 0, coder=1).to_device()
+
+
+@pytest.mark.parametrize("fmt", ["v0", "q32", "q64"])
+@pytest.mark.parametrize("n_vec", [1, 4, 8])
+def test_decoder_block(cuda, fmt, n_vec):
+    import torch
+    note(f"batch decoder block {fmt} n_vec={n_vec}")
+    mats = [gaussian_int8(r, c, 6.0, seed=r + 7 * c + i)[0] for i, (r, c) in enumerate(BLOCK)]
+    cms = [make(fmt, m) for m in mats]
+    vs = [vecs_for(m.shape[1], n_vec, 10 * i) for i, m in enumerate(mats)]
+    b = Batch([(c, torch.from_numpy(v).cuda()) for c, v in zip(cms, vs)])
+    for rep in range(3):           # the queue resets itself between launches
+        for o in b.outputs:
+            o.zero_()
+        b.launch()
+        torch.cuda.synchronize()
+        for m, v, o in zip(mats, vs, b.outputs):
+            assert np.array_equal(o.cpu().numpy(), want(m, v)), rep
+
+
+def test_mixed_uneven_batch(cuda):
+    """.bin v0, quad 32 / 64, family-B rows and stripes, uneven shapes, 1..8 vectors."""
+    import torch
+    note("batch mixed uneven")
+    rng = np.random.default_rng(3)
+    entries, checks = [], []
+    specs = [("v0", 300, 1024), ("q32", 77, 2048), ("q64", 129, 1024), ("brows", 90, 384),
+             ("bstripes", 96, 260), ("v0", 5, 512), ("q64", 1000, 2048), ("brows", 33, 128),
+             ("bstripes", 32, 1024)]
+    for i, (fmt, rows, cols) in enumerate(specs):
+        sigma = [2.0, 4.0, 8.0, 16.0][i % 4]
+        m = gaussian_int8(rows, cols, sigma if fmt[0] != "b" else min(sigma, 8.0), seed=i)[0]
+        n_vec = int(rng.integers(1, 9))
+        v = vecs_for(cols, n_vec, 100 * i)
+        dv = torch.from_numpy(v).cuda()
+        if fmt[0] == "b":
+            bm = BufferedMatrices.encode([m], "rows" if fmt == "brows" else "stripes",
+                                         scales=np.array([0.01], np.float32)).to_device()
+            entries.append(((bm, 0), dv))
+            single = bm.mat_vec(0, dv)
+        else:
+            c = make(fmt, m)
+            entries.append((c, dv))
+            single = c.multiply(dv)
+        checks.append((m, v, single, fmt))
+    b = Batch(entries)
+    b.launch()
+    torch.cuda.synchronize()
+    for (m, v, single, fmt), o, o8 in zip(checks, b.outputs, b.out8):
+        got = o.cpu().numpy()
+        assert np.array_equal(got, want(m, v)), fmt
+        assert np.array_equal(got, single.cpu().numpy()), fmt
+        if o8 is not None:
+            assert np.array_equal(o8.cpu().numpy(), B.requant_b(want(m, v), 0.01))
+    for cfg in [(1, 1, 4), (4, 2, 8), (2, 0, 2)]:
+        b.configure(*cfg)
+        for o in b.outputs:
+            o.zero_()
+        b.launch()
+        torch.cuda.synchronize()
+        for (m, v, _s, fmt), o in zip(checks, b.outputs):
+            assert np.array_equal(o.cpu().numpy(), want(m, v)), (cfg, fmt)
+
+
+def test_graph_replays_and_autotune(cuda):
+    import torch
+    note("batch graph replays + autotune")
+    mats = [gaussian_int8(r, c, 6.0, seed=i)[0] for i, (r, c) in enumerate(BLOCK)]
+    cms = [make("q32", m) for m in mats]
+    vs = [vecs_for(m.shape[1], 2, i) for i, m in enumerate(mats)]
+    dvs = [torch.from_numpy(v).cuda() for v in vs]
+    b = Batch(list(zip(cms, dvs)))
+    cfg, ms = b.autotune(reps=2)
+    assert cfg == b.config and ms > 0
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        b.launch(stream=s)        # warm-up outside the capture
+        s.synchronize()
+        with torch.cuda.graph(g, stream=s):
+            b.launch(stream=s)
+    torch.cuda.synchronize()
+    for rep in range(3):
+        for o in b.outputs:
+            o.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        x = torch.ones(1000, device="cuda").sum()   # eager work between replays
+        assert float(x) == 1000.0
+        for m, v, o in zip(mats, vs, b.outputs):
+            assert np.array_equal(o.cpu().numpy(), want(m, v)), rep
+
+
+def test_argument_errors(cuda):
+    import torch
+    note("batch argument validation")
+    m = gaussian_int8(8, 2048, 6.0)[0]
+    c = CompressedMatrixV1.encode(m, 128, 8, 0, coder=1).to_device()
+    with pytest.raises(MatvansError) as e:       # quad items need 32 or 64 streams
+        Batch([(c, torch.zeros(2048, dtype=torch.int8, device="cuda"))])
+    assert e.value.status == 8
+    c = make("q32", m)
+    with pytest.raises(MatvansError) as e:       # 9 vectors
+        Batch([(c, torch.zeros((9, 2048), dtype=torch.int8, device="cuda"))])
+    assert e.value.status == 1
