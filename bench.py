@@ -43,6 +43,11 @@ def parse():
                     help="run one kernel only (for profiling); prints its timing")
     ap.add_argument("--formats", default=",".join(DEFAULT_FORMATS),
                     help="comma list of encodings to time; the fastest is the headline")
+    ap.add_argument("--batch", default="block,uneven",
+                    help="batched workloads to time beside the headline: block, uneven "
+                         "(comma list, or 'none')")
+    ap.add_argument("--batch-scale", type=float, default=1.0,
+                    help="scale every batched shape by this factor (1 = the full sizes)")
     return ap.parse_args()
 
 
@@ -138,6 +143,216 @@ def cpu_baseline(cm, x, budget_s=10.0):
     return {"value": round(per * 1e6, 1), "unit": "us/matvec", "cores": cores, "kind": "port",
             "sample": f"full {cm.rows}x{cm.cols} matrix, {reps} repetitions, "
                       f"oracle/oracle.c decode+int32 dot threaded over rows"}
+
+
+# ---------------------------------------------------------------- batched workloads
+
+BLOCK_SHAPES = [("q", 4096, 4096), ("k", 1024, 4096), ("v", 1024, 4096), ("o", 4096, 4096),
+                ("gate", 14336, 4096), ("up", 14336, 4096), ("down", 4096, 14336)]
+
+
+def gauss_table(sigma):
+    """int8 values of clip(rint(N(0, sigma^2)), -127, 127) indexed by a uniform 16-bit
+    draw (inverse-CDF sampling at 2^-16 resolution; tails folded into +-127)."""
+    from math import erf, sqrt
+    k = np.arange(-127, 128)
+    cdf_hi = np.array([0.5 * (1 + erf((v + 0.5) / (sigma * sqrt(2)))) for v in k])
+    cdf_hi[-1] = 1.0
+    edges = np.rint(cdf_hi * 65536).astype(np.int64)
+    return np.repeat(k.astype(np.int8), np.diff(np.concatenate([[0], edges])))[:65536]
+
+
+def sample_int8(rng, shape, sigma):
+    return gauss_table(sigma)[rng.integers(0, 65536, size=shape, dtype=np.uint16)]
+
+
+def act_vectors(rng, n, cols):
+    return np.clip(np.rint(rng.normal(0, 30.0, size=(n, cols))), -127, 127).astype(np.int8)
+
+
+def exact_product(m, v):
+    """int32 W @ v^T, exact (|sums| < 2^53 in float64)."""
+    return (np.asarray(v, np.float64) @ np.asarray(m, np.float64).T).astype(np.int32)
+
+
+def _v1_copy(c):
+    from matvans_amd.mvx import CompressedMatrixV1
+    return CompressedMatrixV1(c.rows, c.cols, c.streams, c.precision, c.table_mode, c.min_value,
+                              c.G, c.cursors, c.payload, c.cdf, c.freq, coder=c.coder)
+
+
+def _v0_copy(c):
+    import matvans_amd as mv
+    return mv.CompressedMatrix(c.rows, c.cols, 1.0, c.cursors, c.min_value, c.G, c.cdf_data,
+                               c.ppf_data, c.payload)
+
+
+def bench_block(dev, steps, warmup, scale=1.0, nvecs=(1, 4, 8), fmts=("v0", "q32", "q64")):
+    """One decoder block (q, k, v, o, gate, up, down; sigma=6 int8) with 1, 4 and 8 vectors
+    in one persistent launch per step, per encoding; the tile size, tile order and grid
+    are autotuned per (encoding, n_vec), and the fastest encoding (the streams-per-row
+    choice) is reported as best.  Also times the same matrices as one launch per matrix,
+    the uncompressed int8 matvecs, and the backward dx = W^T (s_w * g) of every matrix."""
+    import torch
+    import matvans_amd as mv
+    from matvans_amd.batch import Batch
+    from matvans_amd.mvx import CompressedMatrixV1
+    rng = np.random.default_rng(20250311)
+    shapes = [(n, max(32, int(r * scale) // 32 * 32), max(1024, int(c * scale) // 1024 * 1024))
+              for n, r, c in BLOCK_SHAPES]
+    mats = [sample_int8(rng, (r, c), 6.0) for _, r, c in shapes]
+    xs = [act_vectors(rng, max(nvecs), c) for _, _, c in shapes]
+    wants = {nv: [exact_product(m, x[:nv]) for m, x in zip(mats, xs)] for nv in nvecs}
+    res = {"shapes": {n: [r, c] for n, r, c in shapes}, "formats": {}}
+    entries = sum(m.size for m in mats)
+    for fmt in fmts:
+        if fmt == "v0":
+            enc = [mv.CompressedMatrix.encode(m) for m in mats]
+            stored = sum(c.payload_size * 2 + 4 * c.rows + c.G + 1 + 256 for c in enc)
+            make = _v0_copy
+        else:
+            S = 32 if fmt == "q32" else 64
+            enc = [CompressedMatrixV1.encode(m, S, 8, 0, coder=1) for m in mats]
+            stored = sum(c.stored_bytes for c in enc)
+            make = _v1_copy
+        n_copies = min(16, max(2, -(-2 * MALL_BYTES // stored)))
+        copies = [[make(c).to_device(dev) for c in enc] for _ in range(n_copies)]
+        per = {"bits_per_entry": round(8 * stored / entries, 4), "copies": n_copies}
+        for nv in nvecs:
+            dv = [torch.from_numpy(x[:nv].copy()).to(dev) for x in xs]
+            batches = [Batch(list(zip(cp, dv))) for cp in copies]
+            cfg, _ = batches[0].autotune(reps=3)
+            for b in batches[1:]:
+                b.configure(*cfg)
+            for b in batches[:2]:
+                b.launch()
+                torch.cuda.synchronize()
+                for o, w in zip(b.outputs, wants[nv]):
+                    if not np.array_equal(o.cpu().numpy(), w):
+                        print(f"block {fmt} n_vec={nv}: batched result differs", file=sys.stderr)
+                        sys.exit(1)
+            ms = time_launches(lambda i: batches[i].launch(), n_copies, steps, warmup)
+            outs = batches[0].outputs
+            ms1 = time_launches(lambda i: [m.multiply(v, out=o) for m, v, o in
+                                           zip(copies[i], dv, outs)], n_copies, steps, warmup)
+            alg = stored + sum(nv * c + 4 * nv * r for _, r, c in shapes)
+            per[f"n_vec={nv}"] = {"us": round(ms * 1e3, 2), "separate_launches_us":
+                                  round(ms1 * 1e3, 2), "tile_units_order_bpc": list(cfg),
+                                  "compressed_GBps": round(alg / (ms * 1e-3) / 1e9, 1)}
+            del batches
+        if fmt == "v0":
+            # backward of every matrix from the same compressed copies
+            for nv in (1, 8):
+                g = [torch.from_numpy(np.random.default_rng(5).standard_normal((nv, r))
+                                      .astype(np.float32)).to(dev) for _, r, _ in shapes]
+                s_w = [torch.full((r,), 0.01, dtype=torch.float32, device=dev)
+                       for _, r, _ in shapes]
+                alpha = torch.ones(nv, dtype=torch.float32, device=dev)
+                ws = [torch.empty(int(mv.lib().matvans_backward_workspace_size(r, c, nv)) // 4,
+                                  dtype=torch.float32, device=dev) for _, r, c in shapes]
+                dx = [torch.empty((nv, c), dtype=torch.float32, device=dev)
+                      for _, _, c in shapes]
+                msb = time_launches(lambda i: [m.backward(gg, sw, alpha, out=d, workspace=w)
+                                               for m, gg, sw, d, w in
+                                               zip(copies[i], g, s_w, dx, ws)],
+                                    n_copies, steps, warmup)
+                per[f"backward_n_vec={nv}_us"] = round(msb * 1e3, 2)
+                fwd = per[f"n_vec={nv}"]["us"]
+                per[f"fwd_plus_bwd_over_fwd_n_vec={nv}"] = round((fwd + msb * 1e3) / fwd, 3)
+                del ws, dx
+        del copies
+        res["formats"][fmt] = per
+    # uncompressed int8 comparator: one sdot4 matvec per matrix
+    w8 = [torch.from_numpy(m).to(dev) for m in mats]
+    n8 = max(2, -(-2 * MALL_BYTES // entries))
+    w8c = [w8] + [[t.clone() for t in w8] for _ in range(n8 - 1)]
+    for nv in nvecs:
+        dv = [torch.from_numpy(x[:nv].copy()).to(dev) for x in xs]
+        o8 = [torch.empty((nv, m.shape[0]), dtype=torch.int32, device=dev) for m in mats]
+        for t, v, o, w in zip(w8, dv, o8, wants[nv]):
+            if not np.array_equal(mv.int8_matvec(t, v, out=o).cpu().numpy(), w):
+                print("block int8 comparator differs", file=sys.stderr)
+                sys.exit(1)
+        ms8 = time_launches(lambda i: [mv.int8_matvec(t, v, out=o) for t, v, o in
+                                       zip(w8c[i], dv, o8)], n8, steps, warmup)
+        res[f"int8_n_vec={nv}_us"] = round(ms8 * 1e3, 2)
+    del w8c, w8
+    best = {}
+    for nv in nvecs:
+        f = min(res["formats"], key=lambda k: res["formats"][k][f"n_vec={nv}"]["us"])
+        us = res["formats"][f][f"n_vec={nv}"]["us"]
+        best[f"n_vec={nv}"] = {"format": f, "us": us,
+                               "speedup_vs_int8": round(res[f"int8_n_vec={nv}_us"] / us, 3)}
+    res["best"] = best
+    return res
+
+
+def bench_uneven(dev, steps, warmup, scale=1.0, n_mat=32):
+    """32 matrices, rows ~ U[256, 16384] (multiple of 32), cols ~ U{512, 768, ..., 8192},
+    entry sigma from {2, 4, 8, 16} (clipped to [-64, 63]), family-B row and stripe layouts mixed, 1..8 vectors
+    each, all in one persistent launch per step (autotuned), against one launch per
+    matrix."""
+    import torch
+    from matvans_amd.batch import Batch
+    from matvans_amd.buffered import BufferedMatrices
+    rng = np.random.default_rng(20250312)
+    specs = []
+    for i in range(n_mat):
+        rows = max(32, int(rng.integers(256 // 32, 16384 // 32 + 1) * 32 * scale) // 32 * 32)
+        cols = max(256, int(rng.integers(2, 33) * 256 * scale) // 256 * 256)
+        sigma = float(rng.choice([2.0, 4.0, 8.0, 16.0]))
+        layout = "rows" if rng.random() < 0.5 else "stripes"
+        specs.append((rows, cols, sigma, layout, int(rng.integers(1, 9))))
+    # family B's 7-bit model holds at most 128 distinct values (the reference encoder
+    # divides by zero beyond that): sigma=16 entries are clipped to [-64, 63], which moves
+    # about 6e-5 of them
+    mats = [np.clip(sample_int8(rng, (r, c), s), -64, 63) for r, c, s, _, _ in specs]
+    xs = [act_vectors(rng, nv, c) for (_, c, _, _, nv) in specs]
+    groups = {}
+    for i, (_, _, _, layout, _) in enumerate(specs):
+        groups.setdefault(layout, []).append(i)
+    bms, where = {}, {}
+    for layout, idx in groups.items():
+        bms[layout] = BufferedMatrices.encode([mats[i] for i in idx], layout)
+        for k, i in enumerate(idx):
+            where[i] = (layout, k)
+    stored = sum(b.compressed.nbytes + sum(o.nbytes for o in b.offsets) + 384 * b.num
+                 for b in bms.values())
+    entries = sum(m.size for m in mats)
+    n_copies = min(8, max(1, -(-2 * MALL_BYTES // stored)))
+    copies = []
+    for _ in range(n_copies):
+        d = {lay: BufferedMatrices(b.layout, b.compressed, b.offsets, b.ppfs, b.cums, b.probs,
+                                   b.scales, shapes=b.shapes).to_device(dev)
+             for lay, b in bms.items()}
+        copies.append(d)
+    dv = [torch.from_numpy(x).to(dev) for x in xs]
+    batches = []
+    for d in copies:
+        batches.append(Batch([((d[where[i][0]], where[i][1]), dv[i]) for i in range(n_mat)]))
+    cfg, _ = batches[0].autotune(reps=2)
+    for b in batches[1:]:
+        b.configure(*cfg)
+    batches[0].launch()
+    torch.cuda.synchronize()
+    for i, (o, m, x) in enumerate(zip(batches[0].outputs, mats, xs)):
+        if not np.array_equal(o.cpu().numpy(), exact_product(m, x)):
+            print(f"uneven batch: matrix {i} differs", file=sys.stderr)
+            sys.exit(1)
+    ms = time_launches(lambda i: batches[i].launch(), n_copies, steps, warmup)
+    outs = batches[0].outputs
+    ms1 = time_launches(lambda i: [copies[i][where[j][0]].mat_vec(where[j][1], dv[j],
+                                                                  acc32=outs[j])
+                                   for j in range(n_mat)], n_copies, steps, warmup)
+    rows_only = [i for i in range(n_mat) if specs[i][3] == "rows"]
+    alg = stored + sum(nv * c + 4 * nv * r for r, c, _, _, nv in specs)
+    return {"matrices": n_mat, "entries": int(entries),
+            "bits_per_entry": round(8 * stored / entries, 4),
+            "layouts": {k: len(v) for k, v in groups.items()},
+            "vectors": int(sum(s[4] for s in specs)), "us": round(ms * 1e3, 2),
+            "separate_launches_us": round(ms1 * 1e3, 2), "tile_units_order_bpc": list(cfg),
+            "compressed_GBps": round(alg / (ms * 1e-3) / 1e9, 1), "copies": n_copies,
+            "rows_layout_matrices": len(rows_only)}
 
 
 def main():
@@ -305,6 +520,15 @@ def main():
                      "traffic": traffic, "alg_bytes": alg_bytes},
         "cpu_baseline": None if args.no_cpu else cpu_baseline(cm, x),
     }
+    del copies
+    torch.cuda.empty_cache()
+    batch = [b for b in args.batch.split(",") if b and b != "none"]
+    bsteps, bwarm = max(1, min(args.steps, 50)), max(1, min(args.warmup, 5))
+    if "block" in batch:
+        line["block"] = bench_block(dev, bsteps, bwarm, args.batch_scale)
+        torch.cuda.empty_cache()
+    if "uneven" in batch:
+        line["uneven"] = bench_uneven(dev, max(1, bsteps // 5), bwarm, args.batch_scale)
     print(json.dumps(line))
 
 

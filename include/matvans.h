@@ -415,6 +415,7 @@ typedef struct {
     uint32_t units_per_tile;     /* row groups per work-queue tile (1..64) */
     uint32_t order;              /* 0 matrix-major, 1 round-robin, 2 longest chain first */
     uint32_t blocks_per_cu;      /* persistent blocks per compute unit (1..32) */
+    uint32_t schedule;           /* 0 device work queue (atomic), 1 static round-robin */
 } matvans_batch_config;
 
 typedef struct matvans_batch matvans_batch;
@@ -424,7 +425,7 @@ void matvans_batch_destroy(matvans_batch *plan);
 int matvans_batch_configure(matvans_batch *plan, const matvans_batch_config *cfg);
 int matvans_batch_get_config(const matvans_batch *plan, matvans_batch_config *cfg);
 int matvans_batch_launch(const matvans_batch *plan, void *stream);
-/* Autotuner: times units_per_tile x order x blocks_per_cu over `reps` launches each,
+/* Autotuner: times units_per_tile x order x blocks_per_cu x schedule over `reps` launches each,
  * keeps the fastest and returns it (and its ms per launch).  Synchronises. */
 int matvans_batch_autotune(matvans_batch *plan, void *stream, uint32_t reps,
                            matvans_batch_config *best, float *best_ms);

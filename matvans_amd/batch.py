@@ -83,15 +83,17 @@ class Batch:
             lib().matvans_batch_destroy(h)
             self._h = None
 
-    def configure(self, units_per_tile=1, order=0, blocks_per_cu=8):
-        cfg = BatchConfig(units_per_tile, order, blocks_per_cu)
+    def configure(self, units_per_tile=1, order=0, blocks_per_cu=8, schedule=0):
+        """schedule 0: blocks take tiles from the device work queue; 1: block b takes
+        tiles b, b + grid, b + 2 grid, ... (no atomics)."""
+        cfg = BatchConfig(units_per_tile, order, blocks_per_cu, schedule)
         call("matvans_batch_configure", self._h, ctypes.byref(cfg))
 
     @property
     def config(self):
         cfg = BatchConfig()
         call("matvans_batch_get_config", self._h, ctypes.byref(cfg))
-        return (cfg.units_per_tile, cfg.order, cfg.blocks_per_cu)
+        return (cfg.units_per_tile, cfg.order, cfg.blocks_per_cu, cfg.schedule)
 
     def launch(self, stream=None):
         import torch
@@ -99,11 +101,12 @@ class Batch:
         call("matvans_batch_launch", self._h, ctypes.c_void_p(s.cuda_stream))
 
     def autotune(self, reps=5, stream=None):
-        """Returns ((units_per_tile, order, blocks_per_cu), ms per launch) of the best."""
+        """Returns ((units_per_tile, order, blocks_per_cu, schedule), ms per launch) of the
+        best."""
         import torch
         s = stream if stream is not None else torch.cuda.current_stream()
         cfg = BatchConfig()
         ms = ctypes.c_float()
         call("matvans_batch_autotune", self._h, ctypes.c_void_p(s.cuda_stream), reps,
              ctypes.byref(cfg), ctypes.byref(ms))
-        return (cfg.units_per_tile, cfg.order, cfg.blocks_per_cu), ms.value
+        return (cfg.units_per_tile, cfg.order, cfg.blocks_per_cu, cfg.schedule), ms.value

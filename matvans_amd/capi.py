@@ -65,7 +65,8 @@ class BatchItem(ctypes.Structure):
 
 class BatchConfig(ctypes.Structure):
     """matvans_batch_config."""
-    _fields_ = [("units_per_tile", u32), ("order", u32), ("blocks_per_cu", u32)]
+    _fields_ = [("units_per_tile", u32), ("order", u32), ("blocks_per_cu", u32),
+                ("schedule", u32)]
 
 
 # name -> (restype, argtypes); status-returning functions use i32

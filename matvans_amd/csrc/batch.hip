@@ -7,10 +7,14 @@
 // kernels (blk_v1.h, blk_b.h); a tile is `units_per_tile` consecutive units of one
 // matrix.  The tile table is built on the host in the configured order (matrix-major,
 // round-robin over matrices, or longest decode chain first) and copied to the device at
-// configuration time.  Blocks take tile numbers with one atomic add on a device counter;
-// a block that finds the queue empty adds one to a second counter, and the last block to
-// do so resets both, so the next launch (or graph replay) starts from zero without any
-// host-side or memset work on the launch path.
+// configuration time.  Schedule 0 is a device work queue in 8 shards (block b draws from
+// shard b % 8, one counter per shard on its own cache line): a block's first tile is
+// fixed, later ones are drawn with one atomic add each, and the block that draws a
+// shard's last ticket resets that shard's counter, so the next launch (or graph replay)
+// starts from zero with no host-side or memset work on the launch path.  One shared
+// counter, with a per-block exit count for the reset, serialised every block on two
+// addresses and cost more than the decode.  Schedule 1 is static: block b of shard g
+// takes the shard's tiles at positions b / 8, b / 8 + nb, ... and uses no atomics.
 //
 // The plan owns its device descriptors, tile table, counters and the decode tables it
 // builds for .bin v0 items; matvans_batch_launch allocates nothing and is one kernel
@@ -60,20 +64,32 @@ constexpr uint32_t kMasks[] = {bit(kPathV0), bit(kPathQ32), bit(kPathQ64),
                                bit(kPathQ32) | bit(kPathQ64),
                                bit(kPathBRows) | bit(kPathBStripes), 0x1Fu};
 
+// work-queue shards: block b draws from shard b % kShards, whose tiles are t = g, g + 8,
+// g + 16, ...; each shard's counter sits on its own 128-byte line
+constexpr uint32_t kShards = 8;
+constexpr uint32_t kCounterStride = 32;
+
 template <int NV, uint32_t MASK>
 __global__ __launch_bounds__(256) void batch_fwd(const Item *__restrict__ items,
                                                  const Tile *__restrict__ tiles, uint32_t n_tiles,
-                                                 uint32_t *counters) {
+                                                 uint32_t *counters, uint32_t queue) {
     __shared__ BatchSmem<NV> sm;
-    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_pos;
     const uint32_t tid = threadIdx.x;
-    for (;;) {
-        if (tid == 0) s_tile = atomicAdd(&counters[0], 1u);
-        __syncthreads();
-        const uint32_t t = s_tile;
-        __syncthreads();
-        if (t >= n_tiles) break;
-        const Tile tl = tiles[t];
+    // shard g holds nt tiles and nb blocks; a block's first tile is static (position
+    // blockIdx / kShards in its shard), later ones come from the shard's counter as
+    // position nb + ticket.  Every block that enters the loop draws tickets until one
+    // is past the end, so the shard hands out exactly max(nt - nb, 0) + min(nb, nt)
+    // tickets, and the block that draws the last one resets the counter for the next
+    // launch: no block touches it after that.
+    const uint32_t g = blockIdx.x % kShards;
+    const uint32_t nb = (gridDim.x - g + kShards - 1) / kShards;
+    const uint32_t nt = n_tiles > g ? (n_tiles - g + kShards - 1) / kShards : 0u;
+    const uint32_t last_ticket = (nt > nb ? nt - nb : 0u) + (nb < nt ? nb : nt) - 1u;
+    uint32_t *counter = counters + g * kCounterStride;
+    uint32_t pos = blockIdx.x / kShards;
+    while (pos < nt) {
+        const Tile tl = tiles[g + kShards * pos];
         const Item &it = items[tl.item];
         const uint32_t path = __builtin_amdgcn_readfirstlane(it.path);
         for (uint32_t u = 0; u < tl.count; u++) {
@@ -90,13 +106,17 @@ __global__ __launch_bounds__(256) void batch_fwd(const Item *__restrict__ items,
                 blk::b_block<NV, 1>(it.b, bid, sm.b);
             __syncthreads();   // the next unit rebuilds the tables and rings
         }
-    }
-    if (tid == 0) {
-        __threadfence();
-        uint32_t d = atomicAdd(&counters[1], 1u);
-        if (d == gridDim.x - 1) {   // every other block has left the loop
-            atomicExch(&counters[0], 0u);
-            atomicExch(&counters[1], 0u);
+        if (queue) {
+            // every thread read s_pos before the barrier that ended the last unit
+            if (tid == 0) {
+                const uint32_t k = atomicAdd(counter, 1u);
+                if (k == last_ticket) atomicExch(counter, 0u);
+                s_pos = nb + k;
+            }
+            __syncthreads();
+            pos = s_pos;
+        } else {
+            pos += nb;
         }
     }
 }
@@ -122,7 +142,7 @@ struct matvans_batch {
     uint32_t *d_counters = nullptr;
     uint32_t *d_tables = nullptr;         // v0 decode tables, 512 dwords per v0 item
     uint32_t n_tiles = 0, max_tiles = 0, nv = 1, n_cu = 0, mask = 0, kernel = 5;
-    matvans_batch_config cfg{1, 0, 8};
+    matvans_batch_config cfg{1, 0, 8, 0};
 };
 
 namespace {
@@ -256,8 +276,9 @@ extern "C" int matvans_batch_create(const matvans_batch_item *items, uint32_t n,
     if (e == hipSuccess)
         e = hipMemcpy(p->d_items, p->items.data(), sizeof(mv::Item) * n, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&p->d_tiles, sizeof(mv::Tile) * max_tiles);
-    if (e == hipSuccess) e = hipMalloc(&p->d_counters, 2 * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(p->d_counters, 0, 2 * sizeof(uint32_t));
+    const size_t counter_bytes = mv::kShards * mv::kCounterStride * sizeof(uint32_t);
+    if (e == hipSuccess) e = hipMalloc(&p->d_counters, counter_bytes);
+    if (e == hipSuccess) e = hipMemset(p->d_counters, 0, counter_bytes);
     p->max_tiles = max_tiles;
     for (uint32_t i = 0; i < 6; i++)
         if ((mv::kMasks[i] & p->mask) == p->mask) { p->kernel = i; break; }
@@ -284,7 +305,7 @@ extern "C" void matvans_batch_destroy(matvans_batch *p) {
 extern "C" int matvans_batch_configure(matvans_batch *p, const matvans_batch_config *cfg) {
     if (!p || !cfg) return fail(MATVANS_E_INVALID, "null argument");
     if (cfg->units_per_tile < 1 || cfg->units_per_tile > 64 || cfg->order > 2 ||
-        cfg->blocks_per_cu < 1 || cfg->blocks_per_cu > 32)
+        cfg->blocks_per_cu < 1 || cfg->blocks_per_cu > 32 || cfg->schedule > 1)
         return fail(MATVANS_E_INVALID, "config out of range");
     p->cfg = *cfg;
     return hip_fail(build_tiles(p), "tile table");
@@ -298,9 +319,11 @@ extern "C" int matvans_batch_get_config(const matvans_batch *p, matvans_batch_co
 
 extern "C" int matvans_batch_launch(const matvans_batch *p, void *stream) {
     if (!p) return fail(MATVANS_E_INVALID, "null plan");
-    const uint32_t grid = std::max(1u, std::min(p->n_tiles, p->n_cu * p->cfg.blocks_per_cu));
+    // every shard that holds a tile needs a block: at least min(n_tiles, kShards) blocks
+    const uint32_t grid = std::max(1u, std::min(p->n_tiles, std::max(mv::kShards,
+                                                                     p->n_cu * p->cfg.blocks_per_cu)));
     auto s = reinterpret_cast<hipStream_t>(stream);
-    using K = void (*)(const mv::Item *, const mv::Tile *, uint32_t, uint32_t *);
+    using K = void (*)(const mv::Item *, const mv::Tile *, uint32_t, uint32_t *, uint32_t);
 #define MV_ROW(NV) {mv::batch_fwd<NV, mv::kMasks[0]>, mv::batch_fwd<NV, mv::kMasks[1]>, \
                     mv::batch_fwd<NV, mv::kMasks[2]>, mv::batch_fwd<NV, mv::kMasks[3]>, \
                     mv::batch_fwd<NV, mv::kMasks[4]>, mv::batch_fwd<NV, mv::kMasks[5]>}
@@ -308,12 +331,12 @@ extern "C" int matvans_batch_launch(const matvans_batch *p, void *stream) {
 #undef MV_ROW
     const int row = p->nv == 1 ? 0 : p->nv == 2 ? 1 : p->nv == 4 ? 2 : 3;
     hipLaunchKernelGGL(table[row][p->kernel], dim3(grid), dim3(256), 0, s, p->d_items, p->d_tiles,
-                       p->n_tiles, p->d_counters);
+                       p->n_tiles, p->d_counters, p->cfg.schedule == 0 ? 1u : 0u);
     return hip_fail(hipGetLastError(), "batch launch");
 }
 
-// Times every configuration of a small grid (units per tile x tile order x blocks per
-// CU) over `reps` launches each on `stream` and keeps the fastest.  Synchronises; not for
+// Times every configuration of a small grid (schedule x units per tile x tile order x
+// blocks per CU) over `reps` launches each on `stream` and keeps the fastest.  Synchronises; not for
 // use inside graph capture.  No reference counterpart (the closest is the hand-set
 // warpsPerBlock of matvansWarpPerRow.cu:156).
 extern "C" int matvans_batch_autotune(matvans_batch *p, void *stream, uint32_t reps,
@@ -327,24 +350,31 @@ extern "C" int matvans_batch_autotune(matvans_batch *p, void *stream, uint32_t r
     if (e != hipSuccess) return hip_fail(e, "event");
     matvans_batch_config best = p->cfg;
     float best_t = 1e30f;
-    const uint32_t units[] = {1, 2, 4}, orders[] = {0, 1, 2}, bpc[] = {4, 8};
+    std::vector<matvans_batch_config> grid;
+    for (uint32_t q : {0u, 1u})
+        for (uint32_t u : {1u, 2u, 4u})
+            for (uint32_t o : {0u, 1u, 2u})
+                for (uint32_t c : {4u, 8u}) grid.push_back({u, o, c, q});
     int st = MATVANS_OK;
-    for (uint32_t u : units)
-        for (uint32_t o : orders)
-            for (uint32_t c : bpc) {
-                matvans_batch_config cfg{u, o, c};
-                if ((st = matvans_batch_configure(p, &cfg))) goto done;
-                if ((st = matvans_batch_launch(p, stream))) goto done;   // warm-up
-                hipEventRecord(a, s);
-                for (uint32_t r = 0; r < reps && !st; r++) st = matvans_batch_launch(p, stream);
-                hipEventRecord(b, s);
-                if (st) goto done;
-                if ((e = hipEventSynchronize(b)) != hipSuccess) { st = hip_fail(e, "sync"); goto done; }
-                float ms = 0;
-                hipEventElapsedTime(&ms, a, b);
-                ms /= float(reps);
-                if (ms < best_t) { best_t = ms; best = cfg; }
-            }
+    for (const matvans_batch_config &cfg : grid) {
+        if ((st = matvans_batch_configure(p, &cfg))) goto done;
+        if ((st = matvans_batch_launch(p, stream))) goto done;   // warm-up
+        hipEventRecord(a, s);
+        for (uint32_t r = 0; r < reps && !st; r++) st = matvans_batch_launch(p, stream);
+        hipEventRecord(b, s);
+        if (st) goto done;
+        if ((e = hipEventSynchronize(b)) != hipSuccess) {
+            st = hip_fail(e, "sync");
+            goto done;
+        }
+        float ms = 0;
+        hipEventElapsedTime(&ms, a, b);
+        ms /= float(reps);
+        if (ms < best_t) {
+            best_t = ms;
+            best = cfg;
+        }
+    }
 done:
     hipEventDestroy(a);
     hipEventDestroy(b);

@@ -95,7 +95,7 @@ def test_mixed_uneven_batch(cuda):
         assert np.array_equal(got, single.cpu().numpy()), fmt
         if o8 is not None:
             assert np.array_equal(o8.cpu().numpy(), B.requant_b(want(m, v), 0.01))
-    for cfg in [(1, 1, 4), (4, 2, 8), (2, 0, 2)]:
+    for cfg in [(1, 1, 4), (4, 2, 8), (2, 0, 2), (1, 2, 8, 1), (4, 1, 3, 1)]:
         b.configure(*cfg)
         for o in b.outputs:
             o.zero_()
