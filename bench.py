@@ -78,6 +78,7 @@ FORMAT_TEXT = {
     "b-rows": "family B (buffered) ans-warp-rows: quad rANS, 7-bit CDF, 32 coders/row",
     "b-stripes": "family B (buffered) ans-warp-stripes: quad rANS, 7-bit CDF, 32 rows/warp",
 }
+BACKWARD_FORMATS = ("v0", "q32", "q64")
 DEFAULT_FORMATS = ["v0", "q32", "q64", "q128", "s128-p8", "s64-p8", "s64-p12", "s64-p8-rowcdf",
                    "s32-p12", "b-rows"]
 
@@ -240,26 +241,25 @@ def bench_block(dev, steps, warmup, scale=1.0, nvecs=(1, 4, 8), fmts=("v0", "q32
                                   round(ms1 * 1e3, 2), "tile_units_order_bpc": list(cfg),
                                   "compressed_GBps": round(alg / (ms * 1e-3) / 1e9, 1)}
             del batches
-        if fmt == "v0":
-            # backward of every matrix from the same compressed copies
-            for nv in (1, 8):
-                g = [torch.from_numpy(np.random.default_rng(5).standard_normal((nv, r))
-                                      .astype(np.float32)).to(dev) for _, r, _ in shapes]
-                s_w = [torch.full((r,), 0.01, dtype=torch.float32, device=dev)
-                       for _, r, _ in shapes]
-                alpha = torch.ones(nv, dtype=torch.float32, device=dev)
-                ws = [torch.empty(int(mv.lib().matvans_backward_workspace_size(r, c, nv)) // 4,
-                                  dtype=torch.float32, device=dev) for _, r, c in shapes]
-                dx = [torch.empty((nv, c), dtype=torch.float32, device=dev)
-                      for _, _, c in shapes]
-                msb = time_launches(lambda i: [m.backward(gg, sw, alpha, out=d, workspace=w)
-                                               for m, gg, sw, d, w in
-                                               zip(copies[i], g, s_w, dx, ws)],
-                                    n_copies, steps, warmup)
-                per[f"backward_n_vec={nv}_us"] = round(msb * 1e3, 2)
-                fwd = per[f"n_vec={nv}"]["us"]
-                per[f"fwd_plus_bwd_over_fwd_n_vec={nv}"] = round((fwd + msb * 1e3) / fwd, 3)
-                del ws, dx
+        # backward of every matrix from the same compressed copies
+        for nv in (1, 8):
+            g = [torch.from_numpy(np.random.default_rng(5).standard_normal((nv, r))
+                                  .astype(np.float32)).to(dev) for _, r, _ in shapes]
+            s_w = [torch.full((r,), 0.01, dtype=torch.float32, device=dev)
+                   for _, r, _ in shapes]
+            alpha = torch.ones(nv, dtype=torch.float32, device=dev)
+            ws = [torch.empty(int(mv.lib().matvans_backward_workspace_size(r, c, nv)) // 4,
+                              dtype=torch.float32, device=dev) for _, r, c in shapes]
+            dx = [torch.empty((nv, c), dtype=torch.float32, device=dev)
+                  for _, _, c in shapes]
+            msb = time_launches(lambda i: [m.backward(gg, sw, alpha, out=d, workspace=w)
+                                           for m, gg, sw, d, w in
+                                           zip(copies[i], g, s_w, dx, ws)],
+                                n_copies, steps, warmup)
+            per[f"backward_n_vec={nv}_us"] = round(msb * 1e3, 2)
+            fwd = per[f"n_vec={nv}"]["us"]
+            per[f"fwd_plus_bwd_over_fwd_n_vec={nv}"] = round((fwd + msb * 1e3) / fwd, 3)
+            del ws, dx
         del copies
         res["formats"][fmt] = per
     # uncompressed int8 comparator: one sdot4 matvec per matrix
@@ -283,6 +283,10 @@ def bench_block(dev, steps, warmup, scale=1.0, nvecs=(1, 4, 8), fmts=("v0", "q32
         us = res["formats"][f][f"n_vec={nv}"]["us"]
         best[f"n_vec={nv}"] = {"format": f, "us": us,
                                "speedup_vs_int8": round(res[f"int8_n_vec={nv}_us"] / us, 3)}
+        bk = res["formats"][f].get(f"backward_n_vec={nv}_us")
+        if bk is not None:
+            best[f"n_vec={nv}"]["backward_us"] = bk
+            best[f"n_vec={nv}"]["fwd_plus_bwd_over_fwd"] = round((us + bk) / us, 3)
     res["best"] = best
     return res
 
@@ -428,10 +432,24 @@ def main():
         ms = time_launches(lambda i: copies[i].multiply(d_vecs, out=out), n_copies, args.steps,
                            args.warmup)
         alg = stored + nvec * cols + 4 * rows * nvec
+        r = {"us": round(ms * 1e3, 3), "bits_per_entry": round(8 * stored / (rows * cols), 4),
+             "alg_bytes": alg, "GBps": round(alg / (ms * 1e-3) / 1e9, 1), "copies": n_copies}
+        if fmt in BACKWARD_FORMATS and not args.no_compare:
+            r["bwd_us"] = round(1e3 * time_backward(copies, rows, cols, nvec), 3)
         del copies
-        return {"us": round(ms * 1e3, 3), "bits_per_entry": round(8 * stored / (rows * cols), 4),
-                "alg_bytes": alg, "GBps": round(alg / (ms * 1e-3) / 1e9, 1),
-                "copies": n_copies}
+        return r
+
+    def time_backward(copies, rows, cols, nvec):
+        """dx = W^T (s_w * g) from the same compressed copies, ms per launch"""
+        g = torch.from_numpy(np.random.default_rng(5).standard_normal((nvec, rows))
+                             .astype(np.float32)).to(dev)
+        s_w = torch.full((rows,), 0.01, dtype=torch.float32, device=dev)
+        alpha = torch.ones(nvec, dtype=torch.float32, device=dev)
+        ws = torch.empty(int(mv.lib().matvans_backward_workspace_size(rows, cols, nvec)) // 4,
+                         dtype=torch.float32, device=dev)
+        dx = torch.empty((nvec, cols), dtype=torch.float32, device=dev)
+        return time_launches(lambda i: copies[i].backward(g, s_w, alpha, out=dx, workspace=ws),
+                             len(copies), args.steps, args.warmup)
 
     fmts = [f for f in args.formats.split(",") if f]
     for f in fmts:
@@ -447,23 +465,7 @@ def main():
     best = min((f for f in per_format if "us" in per_format[f]), key=lambda f: per_format[f]["us"])
     ms_fused = per_format[best]["us"] / 1e3
     n_copies = per_format[best]["copies"]
-    copies = [encode("v0")[2]() for _ in range(min(64, max(2, -(-2 * MALL_BYTES // (
-        cm.payload_size * 2)))))]
-
     res = {}
-    if not args.no_compare:
-        # backward dx = W^T (s_w * g) from the same compressed copies (one vector)
-        g = torch.from_numpy(np.random.default_rng(5).standard_normal((nvec, rows))
-                             .astype(np.float32)).to(dev)
-        s_w = torch.full((rows,), 0.01, dtype=torch.float32, device=dev)
-        alpha = torch.ones(nvec, dtype=torch.float32, device=dev)
-        ws = torch.empty(int(mv.lib().matvans_backward_workspace_size(rows, cols, nvec)) // 4,
-                         dtype=torch.float32, device=dev)
-        dx = torch.empty((nvec, cols), dtype=torch.float32, device=dev)
-        res["bwd_us"] = 1e3 * time_launches(
-            lambda i: copies[i].backward(g, s_w, alpha, out=dx, workspace=ws),
-            len(copies), args.steps, args.warmup)
-        del ws
     if not args.no_compare or args.only in ("int8", "f32"):
         d_w8 = [torch.from_numpy(w).to(dev) for _ in range(max(2, -(-2 * MALL_BYTES // w.nbytes)))]
         o8 = torch.empty((nvec, rows), dtype=torch.int32, device=dev)
@@ -511,8 +513,9 @@ def main():
         "speedup_vs_int8": round(res["int8_us"] / us, 4) if "int8_us" in res else None,
         "int8_us": round(res["int8_us"], 3) if "int8_us" in res else None,
         "f32_us": round(res["f32_us"], 3) if "f32_us" in res else None,
-        "bwd_us": round(res["bwd_us"], 3) if "bwd_us" in res else None,
-        "fwd_plus_bwd_over_fwd": round((us + res["bwd_us"]) / us, 3) if "bwd_us" in res else None,
+        "bwd_us": per_format[best].get("bwd_us"),
+        "fwd_plus_bwd_over_fwd": round((us + per_format[best]["bwd_us"]) / us, 3)
+        if "bwd_us" in per_format[best] else None,
         "compressed_GBps": round(achieved, 1),
         "uncompressed_equiv_GBps": round(rows * cols * nvec / (us * 1e-6) / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
@@ -520,7 +523,6 @@ def main():
                      "traffic": traffic, "alg_bytes": alg_bytes},
         "cpu_baseline": None if args.no_cpu else cpu_baseline(cm, x),
     }
-    del copies
     torch.cuda.empty_cache()
     batch = [b for b in args.batch.split(",") if b and b != "none"]
     bsteps, bwarm = max(1, min(args.steps, 50)), max(1, min(args.warmup, 5))

@@ -265,6 +265,34 @@ extern "C" int matvans_decompress_backward(float *dx, const float *g, uint32_t n
     return hip_status(mv::launch_v0_backward(a, as_stream(stream)), "backward launch");
 }
 
+extern "C" int matvans_v1_decompress_backward(const matvans_v1_dev_matrix *m, float *dx,
+                                              const float *g, uint32_t n_vec, uint32_t g_stride,
+                                              const float *s_w, const float *alpha,
+                                              float *workspace, size_t workspace_bytes,
+                                              void *stream) {
+    if (!m || !dx || !g || !s_w || !alpha || !workspace || !m->cursors || !m->payload || !m->lut)
+        return fail(MATVANS_E_INVALID, "null argument or device pointer");
+    if (n_vec < 1 || n_vec > 8) return fail(MATVANS_E_INVALID, "n_vec %u outside 1..8", n_vec);
+    if (m->coder != 1 || m->precision != 8 || m->table_mode != 0 ||
+        (m->streams != 32 && m->streams != 64))
+        return fail(MATVANS_E_UNSUPPORTED,
+                    "backward needs the quad coder with 32 or 64 streams and one 8-bit table");
+    if (m->rows == 0 || m->cols == 0 || m->cols % (16 * m->streams))
+        return fail(MATVANS_E_SHAPE, "shape %u x %u (cols must be a multiple of %u)", m->rows,
+                    m->cols, 16 * m->streams);
+    if (n_vec > 1 && g_stride < m->rows)
+        return fail(MATVANS_E_INVALID, "g_stride %u < rows", g_stride);
+    if (!aligned(m->payload, 16) || !aligned(workspace, 16) || !aligned(dx, 4))
+        return fail(MATVANS_E_ALIGN, "payload and workspace must be 16-byte aligned");
+    size_t need = mv::v0_bwd_workspace_bytes(m->rows, m->cols, n_vec);
+    if (workspace_bytes < need)
+        return fail(MATVANS_E_CAPACITY, "workspace %zu bytes < %zu", workspace_bytes, need);
+    mv::QuadBwdArgs a{dx, workspace, g, n_vec, n_vec > 1 ? g_stride : m->rows, s_w, alpha,
+                      m->rows, m->cols, m->streams, m->cursors, m->lut, m->payload,
+                      m->payload_size};
+    return hip_status(mv::launch_quad_backward(a, as_stream(stream)), "quad backward launch");
+}
+
 extern "C" int matvans_quantize_vectors(const float *x, uint32_t n_vec, uint32_t x_stride,
                                         uint32_t cols, int8_t *xq, uint32_t xq_stride,
                                         float *s_x, void *stream) {

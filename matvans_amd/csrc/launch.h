@@ -72,6 +72,22 @@ struct V0BwdArgs {
 
 size_t v0_bwd_workspace_bytes(uint32_t rows, uint32_t cols, uint32_t n_vec);
 hipError_t launch_v0_backward(const V0BwdArgs &a, hipStream_t s);
+
+struct QuadBwdArgs {
+    float *dx;               // [n_vec][cols]
+    float *partial;          // workspace [n_vec][ceil(rows/64)][cols]
+    const float *g;          // [n_vec][g_stride]
+    uint32_t n_vec, g_stride;
+    const float *s_w;        // [rows]
+    const float *alpha;      // [n_vec]
+    uint32_t rows, cols, streams;
+    const uint32_t *cursors;
+    const uint32_t *lut;     // A / B decode tables, 2 x 256 dwords
+    const uint16_t *payload;
+    uint32_t payload_size;
+};
+
+hipError_t launch_quad_backward(const QuadBwdArgs &a, hipStream_t s);
 hipError_t launch_quantize_vectors(const float *x, uint32_t n_vec, uint32_t x_stride,
                                    uint32_t cols, int8_t *xq, uint32_t xq_stride, float *s_x,
                                    hipStream_t s);

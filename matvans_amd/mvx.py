@@ -122,6 +122,30 @@ class CompressedMatrixV1:
              v.stride(0), _stream(stream))
         return out[0] if squeeze else out
 
+    def backward(self, g, s_w, alpha=None, out=None, workspace=None, stream=None):
+        """dx[v] = alpha[v] * W^T (s_w * g[v]) re-decoded from the quad-coder streams
+        (coder 1, 32 or 64 streams), g [rows] or [n_vec, rows] fp32 (n_vec <= 8); bitwise
+        reproducible."""
+        import torch
+        s = self.dev_struct()
+        squeeze = g.dim() == 1
+        gg = (g.unsqueeze(0) if squeeze else g).contiguous()
+        if gg.dtype != torch.float32 or gg.shape[-1] != self.rows:
+            raise ValueError(f"expected fp32 gradients of length {self.rows}")
+        n = gg.shape[0]
+        dev = gg.device
+        if alpha is None:
+            alpha = torch.ones(n, dtype=torch.float32, device=dev)
+        need = int(lib().matvans_backward_workspace_size(self.rows, self.cols, n))
+        if workspace is None or workspace.numel() * workspace.element_size() < need:
+            workspace = torch.empty(max(1, need // 4), dtype=torch.float32, device=dev)
+        if out is None:
+            out = torch.empty((n, self.cols), dtype=torch.float32, device=dev)
+        call("matvans_v1_decompress_backward", ctypes.byref(s), _ptr(out), _ptr(gg), n,
+             gg.stride(0), _ptr(s_w), _ptr(alpha), _ptr(workspace),
+             workspace.numel() * workspace.element_size(), _stream(stream))
+        return out[0] if squeeze else out
+
     def host_struct(self, keep) -> V1Host:
         keep += [self.cursors, self.payload, self.cdf, self.freq]
         return V1Host(self.rows, self.cols, self.grid_spacing, self.streams, self.precision,

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import matvans_amd as mv
+from matvans_amd.mvx import CompressedMatrixV1
 from oracle import oracle as O
 
 from .helpers import gaussian_int8
@@ -19,19 +20,34 @@ def note(msg):
     sys.stdout.flush()
 
 
+FORMATS = ["v0", "q32", "q64"]
+
+
+def encode(fmt, w):
+    """The matrix on the device in one of the encodings with a backward: .bin v0, or the
+    MVX1 quad coder with 32 / 64 streams."""
+    if fmt == "v0":
+        return mv.CompressedMatrix.encode(w, n_threads=16).to_device()
+    S = 32 if fmt == "q32" else 64
+    if w.shape[1] % (16 * S):
+        pytest.skip(f"cols {w.shape[1]} not a multiple of {16 * S}")
+    return CompressedMatrixV1.encode(w, S, 8, 0, coder=1).to_device()
+
+
 def dense_ref(w, s_w, g, alpha):
     """FP64 alpha[v] * W^T (s_w * g[v])."""
     return alpha[:, None] * ((g.astype(np.float64) * s_w[None, :].astype(np.float64)) @
                              w.astype(np.float64))
 
 
+@pytest.mark.parametrize("fmt", FORMATS)
 @pytest.mark.parametrize("rows,cols,n_vec", [(256, 512, 1), (100, 1024, 3), (4096, 14336, 1),
-                                             (14336, 4096, 2), (1000, 2048, 8)])
-def test_backward_vs_dense(cuda, rows, cols, n_vec):
+                                             (14336, 4096, 2), (1000, 2048, 8), (65, 2048, 5)])
+def test_backward_vs_dense(cuda, fmt, rows, cols, n_vec):
     import torch
-    note(f"backward {rows}x{cols} n_vec={n_vec}")
+    note(f"backward {fmt} {rows}x{cols} n_vec={n_vec}")
     w, _ = gaussian_int8(rows, cols, 6.0, seed=rows + 3 * cols)
-    cm = mv.CompressedMatrix.encode(w, n_threads=16).to_device()
+    cm = encode(fmt, w)
     rng = np.random.default_rng(7)
     g = rng.standard_normal((n_vec, rows)).astype(np.float32)
     s_w = (rng.random(rows) * 0.02 + 0.001).astype(np.float32)
@@ -49,11 +65,12 @@ def test_backward_vs_dense(cuda, rows, cols, n_vec):
     assert np.all(np.abs(got - ref) <= 1e-4 * absum + 1e-30)
 
 
-def test_backward_deterministic(cuda):
+@pytest.mark.parametrize("fmt", FORMATS)
+def test_backward_deterministic(cuda, fmt):
     import torch
-    note("backward determinism")
+    note(f"backward determinism {fmt}")
     w, _ = gaussian_int8(2048, 4096, 6.0, seed=5)
-    cm = mv.CompressedMatrix.encode(w).to_device()
+    cm = encode(fmt, w)
     rng = np.random.default_rng(1)
     g = torch.from_numpy(rng.standard_normal((4, 2048)).astype(np.float32)).cuda()
     s_w = torch.from_numpy((rng.random(2048) + 0.1).astype(np.float32)).cuda()
@@ -62,15 +79,19 @@ def test_backward_deterministic(cuda):
     assert a.tobytes() == b.tobytes()
 
 
-def test_finite_differences_slice(cuda):
+@pytest.mark.parametrize("fmt", ["v0", "q32"])
+def test_finite_differences_slice(cuda, fmt):
     """dL/dxq of L(xq) = sum_r g_r * s_x * s_w[r] * (W xq)_r by FP64 central differences
     on a 256 x 512 slice equals the kernel's s_x * W^T (s_w * g)."""
     import torch
-    note("finite differences 256x512")
+    note(f"finite differences 256x512 {fmt}")
     w, _ = gaussian_int8(256, 512, 6.0, seed=11)
-    cm = mv.CompressedMatrix.encode(w).to_device()
-    wd = O.c_decode(O.Record(256, 512, 1.0, cm.cursors, cm.payload_size, cm.min_value, cm.G,
-                             cm.cdf_data, cm.ppf_data, cm.payload)).astype(np.float64)
+    cm = encode(fmt, w)
+    if fmt == "v0":
+        wd = O.c_decode(O.Record(256, 512, 1.0, cm.cursors, cm.payload_size, cm.min_value, cm.G,
+                                 cm.cdf_data, cm.ppf_data, cm.payload)).astype(np.float64)
+    else:
+        wd = w.astype(np.float64)   # the quad decode is bit-exact (test_gpu_v1)
     rng = np.random.default_rng(2)
     g = rng.standard_normal(256).astype(np.float32)
     s_w = (rng.random(256) * 0.1 + 0.01).astype(np.float32)
@@ -88,13 +109,14 @@ def test_finite_differences_slice(cuda):
     assert np.linalg.norm(dx - fd) <= 1e-5 * np.linalg.norm(fd)
 
 
+@pytest.mark.parametrize("fmt", FORMATS)
 @pytest.mark.parametrize("batch", [1, 4, 8])
-def test_autograd_function(cuda, batch):
+def test_autograd_function(cuda, fmt, batch):
     import torch
-    note(f"autograd batch={batch}")
+    note(f"autograd {fmt} batch={batch}")
     rows, cols = 1024, 2048
     w, _ = gaussian_int8(rows, cols, 6.0, seed=batch)
-    cm = mv.CompressedMatrix.encode(w).to_device()
+    cm = encode(fmt, w)
     rng = np.random.default_rng(batch)
     s_w = torch.from_numpy((rng.random(rows) * 0.02 + 0.001).astype(np.float32)).cuda()
     x = torch.from_numpy(rng.standard_normal((batch, cols)).astype(np.float32)).cuda()
