@@ -1,4 +1,5 @@
-"""Build ``matvans_amd/lib/libmatvans.so`` from ``matvans_amd/csrc`` with hipcc (gfx950).
+"""Build ``matvans_amd/lib/libmatvans.so`` from ``matvans_amd/csrc`` with hipcc (gfx950),
+and the host driver ``matvans_amd/lib/matvans_run`` from ``matvans_amd/driver``.
 
     python -m matvans_amd.build [--jobs N] [--asm DIR]
 
@@ -66,7 +67,24 @@ def build(jobs: int | None = None, asm_dir: str | None = None, verbose: bool = F
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")
     os.replace(tmp, LIB)
+    build_driver()
     return LIB
+
+
+DRIVER_SRC = os.path.join(HERE, "driver", "matvans_run.cpp")
+DRIVER = os.path.join(LIBDIR, "matvans_run")
+
+
+def build_driver() -> str:
+    """The C++ host driver (the reference's chained benchmark harness) linked against the
+    library, with an rpath to find it next to itself."""
+    cmd = [HIPCC, "-x", "hip"] + COMMON + [DRIVER_SRC, "-o", DRIVER + ".tmp", "-L", LIBDIR,
+                                          "-lmatvans", "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{res.stderr}")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 def main():

@@ -1,0 +1,281 @@
+// matvans_run: the reference's benchmark harness as a C++ host program over the C ABI of
+// libmatvans (matvansWarpPerRow/matvansWarpPerRow.cu:225-393).
+//
+//     matvans_run FILE [--iters N] [--quiet]
+//
+// FILE is a .bin v0 file (create_matrices_WarpPerRow.py) or an MVX1 file.  The program
+// reads it with the library reader, uploads every matrix once, and runs the reference's
+// chain N times: v_0 = the file's vector, v_k = requant(W_k v_{k-1}) with requant the
+// absmax/127 epilogue (matvansWarpPerRow.cu:51-72,150-176), then compares FxHash(v_last)
+// with the header's result_hash (:360-374).  Per iteration it prints the reference's three
+// lines (hash check, time with the payload upload, time of the chain alone) and at the
+// end one JSON line with the best times.  Exit status: 0 when every iteration's hash
+// matched, 1 otherwise, 2 on an error.
+//
+// Differences from the reference harness, on purpose: device buffers are allocated once
+// (the reference allocates and frees them in every iteration, :287-313,377-383); "with
+// upload" re-copies the payloads host -> device inside the timed region instead; the
+// chain is stream-ordered with one synchronisation at its end (the reference copies each
+// intermediate vector to the host, :344); throughput counts rows x cols MACs per matrix
+// (the reference's rows * rows assumes square matrices, :386).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "matvans.h"
+
+namespace {
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "matvans_run: %s: %s\n", what, hipGetErrorString(e));
+        std::exit(2);
+    }
+}
+
+void mv_check(int st, const char *what) {
+    if (st != MATVANS_OK) {
+        std::fprintf(stderr, "matvans_run: %s: %s\n", what, matvans_last_error());
+        std::exit(2);
+    }
+}
+
+// Device allocations of the run, freed at exit.
+struct Arena {
+    std::vector<void *> bufs;
+    template <class T>
+    T *alloc(size_t count) {
+        void *p = nullptr;
+        hip_check(hipMalloc(&p, std::max<size_t>(count * sizeof(T), 16)), "hipMalloc");
+        bufs.push_back(p);
+        return static_cast<T *>(p);
+    }
+    ~Arena() {
+        for (void *p : bufs) (void)hipFree(p);
+    }
+};
+
+template <class T>
+T *upload(Arena &a, const T *host, size_t count, size_t slack = 0) {
+    T *d = a.alloc<T>(count + slack);
+    if (slack) hip_check(hipMemset(d, 0, (count + slack) * sizeof(T)), "hipMemset");
+    if (count) hip_check(hipMemcpy(d, host, count * sizeof(T), hipMemcpyHostToDevice), "upload");
+    return d;
+}
+
+// One matrix of the chain, either container.
+struct Step {
+    bool v1 = false;
+    matvans_dev_matrix m0{};
+    matvans_v1_dev_matrix m1{};
+    const uint16_t *host_payload = nullptr;
+    uint16_t *dev_payload = nullptr;
+    uint32_t payload_words = 0;
+    uint32_t rows = 0, cols = 0;
+};
+
+constexpr size_t kSlackWords = 512;   // zero words after each payload
+
+bool is_mvx1(const char *path) {
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return false;
+    char magic[4] = {0, 0, 0, 0};
+    size_t n = std::fread(magic, 1, 4, f);
+    std::fclose(f);
+    return n == 4 && std::memcmp(magic, "MVX1", 4) == 0;
+}
+
+void usage() {
+    std::printf(
+        "usage: matvans_run FILE [--iters N] [--quiet]\n"
+        "  FILE: .bin v0 (create_matrices_WarpPerRow.py) or MVX1 compressed matrices\n"
+        "  runs the chain v_k = requant(W_k v_(k-1)) N times (default 10) and checks the\n"
+        "  FxHash of the last vector against the file's result_hash\n");
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    const char *path = nullptr;
+    int iters = 10;
+    bool quiet = false;
+    for (int i = 1; i < argc; i++) {
+        std::string a = argv[i];
+        if (a == "--help" || a == "-h") {
+            usage();
+            return 0;
+        } else if (a == "--iters" && i + 1 < argc) {
+            iters = std::max(1, std::atoi(argv[++i]));
+        } else if (a == "--quiet") {
+            quiet = true;
+        } else if (!path && a[0] != '-') {
+            path = argv[i];
+        } else {
+            usage();
+            return 2;
+        }
+    }
+    if (!path) {
+        usage();
+        return 2;
+    }
+
+    // ---- read the file (library reader) and upload every matrix once
+    Arena arena;
+    std::vector<Step> steps;
+    uint32_t num = 0, result_hash = 0, len_v = 0;
+    const int8_t *h_vec = nullptr;
+    matvans_bin *bin = nullptr;
+    matvans_v1_file *mvx = nullptr;
+    if (is_mvx1(path)) {
+        mv_check(matvans_v1_read(path, &mvx), "read MVX1");
+        mv_check(matvans_v1_header(mvx, &num, &result_hash, &h_vec, &len_v), "MVX1 header");
+        for (uint32_t k = 0; k < num; k++) {
+            matvans_v1_host h{};
+            mv_check(matvans_v1_matrix(mvx, k, &h), "MVX1 matrix");
+            Step s;
+            s.v1 = true;
+            s.rows = h.rows;
+            s.cols = h.cols;
+            s.m1.rows = h.rows;
+            s.m1.cols = h.cols;
+            s.m1.streams = h.streams;
+            s.m1.precision = h.precision;
+            s.m1.table_mode = h.table_mode;
+            s.m1.coder = h.coder;
+            s.m1.min_value = h.min_value;
+            s.m1.G = h.G;
+            s.m1.payload_size = h.payload_size;
+            s.m1.cursors = upload(arena, h.cursors, h.rows);
+            if (h.table_mode == 0) {
+                std::vector<uint32_t> lut(size_t(2) << h.precision);
+                mv_check(matvans_v1_lut(h.cdf, h.G, h.min_value, h.precision, lut.data()),
+                         "decode table");
+                s.m1.lut = upload(arena, lut.data(), lut.size());
+            } else {
+                s.m1.freq = upload(arena, h.freq, size_t(h.rows) * h.G);
+            }
+            s.dev_payload = upload(arena, h.payload, h.payload_size, kSlackWords);
+            s.m1.payload = s.dev_payload;
+            s.host_payload = h.payload;
+            s.payload_words = h.payload_size;
+            steps.push_back(s);
+        }
+    } else {
+        uint32_t max_words = 0;
+        mv_check(matvans_bin_read(path, &bin), "read .bin");
+        mv_check(matvans_bin_header(bin, &num, &result_hash, &max_words, &h_vec, &len_v),
+                 ".bin header");
+        for (uint32_t k = 0; k < num; k++) {
+            matvans_host_matrix h{};
+            mv_check(matvans_bin_matrix(bin, k, &h), ".bin matrix");
+            Step s;
+            s.rows = h.rows;
+            s.cols = h.cols;
+            std::vector<uint8_t> cdf(256, 0);
+            std::memcpy(cdf.data(), h.cdf, size_t(h.G) + 1);
+            s.m0 = {h.rows, h.cols, h.grid_spacing, h.min_value, h.G, 0, h.payload_size,
+                    upload(arena, h.cursors, h.rows), upload(arena, cdf.data(), cdf.size()),
+                    upload(arena, h.ppf, 256), nullptr};
+            s.dev_payload = upload(arena, h.payload, h.payload_size, kSlackWords);
+            s.m0.payload = s.dev_payload;
+            s.host_payload = h.payload;
+            s.payload_words = h.payload_size;
+            steps.push_back(s);
+        }
+    }
+    if (steps.empty()) {
+        std::fprintf(stderr, "matvans_run: %s holds no matrices\n", path);
+        return 2;
+    }
+    uint32_t max_len = len_v;
+    double macs = 0;
+    for (const Step &s : steps) {
+        max_len = std::max(max_len, std::max(s.rows, s.cols));
+        macs += double(s.rows) * s.cols;
+    }
+    // the chain's shapes must line up: matrix k maps a vector of cols_k to rows_k
+    uint32_t len = len_v;
+    for (size_t k = 0; k < steps.size(); k++) {
+        if (steps[k].cols != len) {
+            std::fprintf(stderr, "matvans_run: matrix %zu has %u cols, the vector %u entries\n", k,
+                         steps[k].cols, len);
+            return 2;
+        }
+        len = steps[k].rows;
+    }
+
+    int8_t *d_v0 = upload(arena, h_vec, len_v);
+    int8_t *d_a = arena.alloc<int8_t>(max_len);
+    int8_t *d_b = arena.alloc<int8_t>(max_len);
+    int32_t *d_acc = arena.alloc<int32_t>(max_len);
+    float *d_delta = arena.alloc<float>(1);
+    hipStream_t stream;
+    hip_check(hipStreamCreate(&stream), "stream");
+    hipEvent_t start1, start2, stop1, stop2;
+    for (hipEvent_t *e : {&start1, &start2, &stop1, &stop2}) hip_check(hipEventCreate(e), "event");
+    std::vector<int8_t> h_out(max_len);
+
+    float best1 = 1e30f, best2 = 1e30f;
+    int matched = 0;
+    for (int it = 0; it < iters; it++) {
+        hip_check(hipEventRecord(start1, stream), "record");
+        for (const Step &s : steps)
+            hip_check(hipMemcpyAsync(s.dev_payload, s.host_payload, size_t(s.payload_words) * 2,
+                                     hipMemcpyHostToDevice, stream),
+                      "payload upload");
+        hip_check(hipMemcpyAsync(d_a, d_v0, len_v, hipMemcpyDeviceToDevice, stream), "vector");
+        hip_check(hipEventRecord(start2, stream), "record");
+        int8_t *vec = d_a, *res = d_b;
+        for (const Step &s : steps) {
+            if (s.v1) {
+                mv_check(matvans_v1_decompress_and_multiply(&s.m1, d_acc, vec, 1, s.cols, stream),
+                         "decode-matvec");
+                mv_check(matvans_absmax_requant(d_acc, s.rows, res, d_delta, 0, stream),
+                         "requant");
+            } else {
+                mv_check(matvans_decompress_and_mult(&s.m0, res, d_acc, vec, d_delta, stream),
+                         "decompress_and_mult");
+            }
+            std::swap(vec, res);
+        }
+        hip_check(hipEventRecord(stop2, stream), "record");
+        const uint32_t out_len = steps.back().rows;
+        hip_check(hipMemcpyAsync(h_out.data(), vec, out_len, hipMemcpyDeviceToHost, stream), "result");
+        hip_check(hipEventRecord(stop1, stream), "record");
+        hip_check(hipEventSynchronize(stop1), "sync");
+        float ms1 = 0, ms2 = 0;
+        hip_check(hipEventElapsedTime(&ms1, start1, stop1), "elapsed");
+        hip_check(hipEventElapsedTime(&ms2, start2, stop2), "elapsed");
+        best1 = std::min(best1, ms1);
+        best2 = std::min(best2, ms2);
+        const uint32_t h = matvans_fxhash(h_out.data(), out_len);
+        if (h == result_hash) {
+            matched++;
+            if (!quiet) std::printf("Hashes match! (%u)\n", h);
+        } else if (!quiet) {
+            std::printf("Hashes *don't* match! (%u, header %u)\n", h, result_hash);
+        }
+        if (!quiet) {
+            std::printf("Time with memcpy:    %f ms\n", ms1);
+            std::printf("Time without memcpy: %f ms\n", ms2);
+            std::printf("Throughput: %e MAC/s\n", 1000.0 * macs / ms2);
+        }
+    }
+    std::printf(
+        "{\"file\": \"%s\", \"container\": \"%s\", \"matrices\": %u, \"iters\": %d, "
+        "\"hash_matches\": %d, \"result_hash\": %u, \"best_ms_with_upload\": %.6f, "
+        "\"best_ms_chain\": %.6f, \"macs_per_chain\": %.0f}\n",
+        path, mvx ? "MVX1" : "bin v0", num, iters, matched, result_hash, best1, best2, macs);
+    for (hipEvent_t e : {start1, start2, stop1, stop2}) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(stream);
+    if (bin) matvans_bin_free(bin);
+    if (mvx) matvans_v1_free(mvx);
+    return matched == iters ? 0 : 1;
+}

@@ -1,0 +1,69 @@
+"""The C++ host driver (matvans_amd/lib/matvans_run): the reference's chained benchmark
+harness (matvansWarpPerRow.cu:225-393) over the C ABI.  CPU tests check the binary and its
+argument / error handling; GPU tests run the chain on the golden .bin files and on MVX1
+files re-encoded from the golden raw matrices, and check the FxHash of the last vector
+against the header's result_hash."""
+import json
+import os
+import subprocess
+
+import pytest
+
+from matvans_amd.build import DRIVER
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+
+
+def run(*args, timeout=120):
+    return subprocess.run([DRIVER, *args], capture_output=True, text=True, timeout=timeout)
+
+
+def test_driver_built_and_usage():
+    assert os.access(DRIVER, os.X_OK)
+    r = run("--help")
+    assert r.returncode == 0 and "usage: matvans_run" in r.stdout
+
+
+def test_driver_rejects_missing_file(tmp_path):
+    r = run(str(tmp_path / "nope.bin"))
+    assert r.returncode == 2 and "read .bin" in r.stderr
+
+
+def test_driver_rejects_truncated_file(tmp_path):
+    src = open(os.path.join(GOLDEN, "compressed_matrices_512_5bit_count2.bin"), "rb").read()
+    p = tmp_path / "cut.bin"
+    p.write_bytes(src[: len(src) // 2])
+    r = run(str(p))
+    assert r.returncode == 2 and "read .bin" in r.stderr
+
+
+def last_json(out):
+    return json.loads(out.strip().splitlines()[-1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fx", ["compressed_matrices_512_5bit_count2.bin",
+                                "compressed_matrices_1024_5bit_count1.bin"])
+def test_driver_chain_on_golden_bin(fx):
+    r = run(os.path.join(GOLDEN, fx), "--iters", "3")
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("Hashes match!") == 3
+    j = last_json(r.stdout)
+    assert j["hash_matches"] == 3 and j["container"] == "bin v0"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("streams,precision,coder", [(32, 8, 1), (32, 12, 0), (32, 8, 0)])
+def test_driver_chain_on_mvx1(tmp_path, streams, precision, coder):
+    """The golden raw matrices re-encoded as MVX1: same chain, same result_hash."""
+    from matvans_amd.mvx import CompressedMatrixV1, write_mvx1
+    from oracle import oracle as O
+    num, result_hash, vec, mats = O.read_raw_bin(os.path.join(GOLDEN, "raw-matrices_512.bin.gz"))
+    enc = [CompressedMatrixV1.encode(m, streams, precision, 0, coder=coder) for m in mats]
+    p = tmp_path / "chain.mvx"
+    write_mvx1(p, enc, vector=vec, result_hash=result_hash)
+    r = run(str(p), "--iters", "2", "--quiet")
+    assert r.returncode == 0, r.stdout + r.stderr
+    j = last_json(r.stdout)
+    assert j["hash_matches"] == 2 and j["container"] == "MVX1"
