@@ -196,7 +196,7 @@ def bench_block(dev, steps, warmup, scale=1.0, nvecs=(1, 4, 8), fmts=("v0", "q32
     the uncompressed int8 matvecs, and the backward dx = W^T (s_w * g) of every matrix."""
     import torch
     import matvans_amd as mv
-    from matvans_amd.batch import Batch
+    from matvans_amd.batch import Batch, BatchBackward
     from matvans_amd.mvx import CompressedMatrixV1
     rng = np.random.default_rng(20250311)
     shapes = [(n, max(32, int(r * scale) // 32 * 32), max(1024, int(c * scale) // 1024 * 1024))
@@ -256,10 +256,27 @@ def bench_block(dev, steps, warmup, scale=1.0, nvecs=(1, 4, 8), fmts=("v0", "q32
                                            for m, gg, sw, d, w in
                                            zip(copies[i], g, s_w, dx, ws)],
                                 n_copies, steps, warmup)
-            per[f"backward_n_vec={nv}_us"] = round(msb * 1e3, 2)
+            per[f"backward_separate_n_vec={nv}_us"] = round(msb * 1e3, 2)
+            # the same backward as one batched plan per copy (two launches per step)
+            bb = [BatchBackward([(m, gg, sw, alpha) for m, gg, sw in zip(cp, g, s_w)])
+                  for cp in copies]
+            bcfg, _ = bb[0].autotune(reps=3)
+            for b in bb[1:]:
+                b.configure(*bcfg)
+            # dx holds the one-matrix backward of identical copies: the batch must match
+            # it bit for bit
+            bb[0].launch()
+            torch.cuda.synchronize()
+            for o, d in zip(bb[0].outputs, dx):
+                if o.cpu().numpy().tobytes() != d.cpu().numpy().tobytes():
+                    print(f"block {fmt}: batched backward differs", file=sys.stderr)
+                    sys.exit(1)
+            msbb = time_launches(lambda i: bb[i].launch(), n_copies, steps, warmup)
+            per[f"backward_n_vec={nv}_us"] = round(msbb * 1e3, 2)
+            per[f"backward_batch_cfg_n_vec={nv}"] = list(bcfg)
             fwd = per[f"n_vec={nv}"]["us"]
-            per[f"fwd_plus_bwd_over_fwd_n_vec={nv}"] = round((fwd + msb * 1e3) / fwd, 3)
-            del ws, dx
+            per[f"fwd_plus_bwd_over_fwd_n_vec={nv}"] = round((fwd + msbb * 1e3) / fwd, 3)
+            del ws, dx, bb
         del copies
         res["formats"][fmt] = per
     # uncompressed int8 comparator: one sdot4 matvec per matrix

@@ -432,6 +432,33 @@ typedef struct {
 typedef struct matvans_batch matvans_batch;
 
 int matvans_batch_create(const matvans_batch_item *items, uint32_t n, matvans_batch **out);
+
+/* N1 x N3: the backward dx = alpha * W^T (s_w * g) of every matrix of a batch in two
+ * launches (a persistent launch over (64-row block, 8-stream group) units from the same
+ * sharded work queue, then one fixed-order reduce over every item's columns).  Results
+ * are bitwise identical to matvans_decompress_backward / matvans_v1_decompress_backward.
+ * The plan owns the partial-sum workspace.  .bin v0 and quad (32 / 64 streams) items.
+ * No reference counterpart. */
+typedef struct {
+    uint32_t format;             /* MATVANS_BATCH_V0 or MATVANS_BATCH_QUAD */
+    uint32_t rows, cols;
+    uint32_t n_vec, g_stride;    /* 1..8 gradient vectors, g_stride floats apart */
+    const float *g;              /* [n_vec][g_stride] */
+    const float *s_w;            /* [rows] */
+    const float *alpha;          /* [n_vec] */
+    float *dx;                   /* [n_vec][cols] */
+    const uint32_t *cursors;
+    const uint16_t *payload;     /* 16-byte aligned */
+    uint32_t payload_size;
+    int32_t min_value;           /* v0 */
+    uint32_t G;                  /* v0 */
+    const uint8_t *cdf, *ppf;    /* v0 */
+    uint32_t streams;            /* quad: 32 or 64 */
+    const uint32_t *lut;         /* quad: the 2 * 256 dwords of matvans_v1_lut */
+} matvans_batch_bwd_item;
+
+int matvans_batch_create_backward(const matvans_batch_bwd_item *items, uint32_t n,
+                                  matvans_batch **out);
 void matvans_batch_destroy(matvans_batch *plan);
 int matvans_batch_configure(matvans_batch *plan, const matvans_batch_config *cfg);
 int matvans_batch_get_config(const matvans_batch *plan, matvans_batch_config *cfg);

@@ -8,10 +8,10 @@ from __future__ import annotations
 import ctypes
 
 from .buffered import BufferedMatrices
-from .capi import BatchConfig, BatchItem, call, lib
+from .capi import BatchBwdItem, BatchConfig, BatchItem, call, lib
 from .mvx import CompressedMatrixV1
 
-__all__ = ["Batch"]
+__all__ = ["Batch", "BatchBackward"]
 
 V0, QUAD, B_ROWS, B_STRIPES = 0, 1, 2, 3
 
@@ -110,3 +110,44 @@ class Batch:
         call("matvans_batch_autotune", self._h, ctypes.c_void_p(s.cuda_stream), reps,
              ctypes.byref(cfg), ctypes.byref(ms))
         return (cfg.units_per_tile, cfg.order, cfg.blocks_per_cu, cfg.schedule), ms.value
+
+
+class BatchBackward(Batch):
+    """The backward dx = alpha * W^T (s_w * g) of every matrix of a batch in two launches
+    (persistent partial sums from the work queue, then one fixed-order reduce); bitwise
+    equal to each matrix's own ``backward``.  entries: list of (matrix, g, s_w, alpha)
+    with matrix a CompressedMatrix (.bin v0) or a quad CompressedMatrixV1 (32 or 64
+    streams) on the device, g fp32 [n_vec, rows] (or [rows]), s_w fp32 [rows], alpha
+    fp32 [n_vec] or None (ones).  ``outputs[i]`` is dx of entry i, fp32 [n_vec, cols]."""
+
+    def __init__(self, entries):  # noqa: D107 (the base initialiser is not used)
+        import torch
+        from . import CompressedMatrix
+        self._keep = []
+        self.outputs, self.out8 = [], []
+        arr = (BatchBwdItem * len(entries))()
+        for i, (m, g, s_w, alpha) in enumerate(entries):
+            gg = (g.unsqueeze(0) if g.dim() == 1 else g).contiguous()
+            n = gg.shape[0]
+            if alpha is None:
+                alpha = torch.ones(n, dtype=torch.float32, device=gg.device)
+            dx = torch.empty((n, m.cols), dtype=torch.float32, device=gg.device)
+            it = arr[i]
+            it.rows, it.cols, it.n_vec, it.g_stride = m.rows, m.cols, n, gg.stride(0)
+            it.g, it.s_w, it.alpha, it.dx = gg.data_ptr(), s_w.data_ptr(), alpha.data_ptr(), dx.data_ptr()
+            it.cursors, it.payload, it.payload_size = _p(m.d_cursors), _p(m.d_payload), m.payload_size
+            if isinstance(m, CompressedMatrix):
+                it.format = V0
+                it.min_value, it.G = m.min_value, m.G
+                it.cdf, it.ppf = _p(m.d_cdf_data), _p(m.d_ppf_data)
+            elif isinstance(m, CompressedMatrixV1) and m.coder == 1:
+                it.format, it.streams, it.lut = QUAD, m.streams, _p(m.d_lut)
+            else:
+                raise ValueError("backward batch entries: .bin v0 or MVX1 quad matrices")
+            self._keep += [m, gg, s_w, alpha]
+            self.outputs.append(dx)
+            self.out8.append(None)
+        self._items = arr
+        h = ctypes.c_void_p()
+        call("matvans_batch_create_backward", arr, len(entries), ctypes.byref(h))
+        self._h = h

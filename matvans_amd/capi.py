@@ -63,6 +63,14 @@ class BatchItem(ctypes.Structure):
                 ("b_prob", P), ("scale", f32), ("acc32", P), ("out8", P)]
 
 
+class BatchBwdItem(ctypes.Structure):
+    """matvans_batch_bwd_item."""
+    _fields_ = [("format", u32), ("rows", u32), ("cols", u32), ("n_vec", u32),
+                ("g_stride", u32), ("g", P), ("s_w", P), ("alpha", P), ("dx", P),
+                ("cursors", P), ("payload", P), ("payload_size", u32), ("min_value", i32),
+                ("G", u32), ("cdf", P), ("ppf", P), ("streams", u32), ("lut", P)]
+
+
 class BatchConfig(ctypes.Structure):
     """matvans_batch_config."""
     _fields_ = [("units_per_tile", u32), ("order", u32), ("blocks_per_cu", u32),
@@ -123,6 +131,8 @@ SIGNATURES = {
                                P]),
     "matvans_batch_create": (i32, [ctypes.POINTER(BatchItem), u32, ctypes.POINTER(P)]),
     "matvans_batch_destroy": (None, [P]),
+    "matvans_batch_create_backward": (i32, [ctypes.POINTER(BatchBwdItem), u32,
+                                            ctypes.POINTER(P)]),
     "matvans_batch_configure": (i32, [P, ctypes.POINTER(BatchConfig)]),
     "matvans_batch_get_config": (i32, [P, ctypes.POINTER(BatchConfig)]),
     "matvans_batch_launch": (i32, [P, P]),

@@ -48,4 +48,18 @@ __device__ __forceinline__ uint32_t add_b1(uint32_t a, uint32_t b) {
     return r;
 }
 
+// float of the signed byte 0 / byte 2 of a (sign-extending SDWA source select)
+__device__ __forceinline__ float cvt_f32_sbyte0(uint32_t a) {
+    float r;
+    asm("v_cvt_f32_i32_sdwa %0, sext(%1) dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_0"
+        : "=v"(r) : "v"(a));
+    return r;
+}
+__device__ __forceinline__ float cvt_f32_sbyte2(uint32_t a) {
+    float r;
+    asm("v_cvt_f32_i32_sdwa %0, sext(%1) dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_2"
+        : "=v"(r) : "v"(a));
+    return r;
+}
+
 }  // namespace mv

@@ -23,9 +23,11 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "blk_b.h"
+#include "blk_bwd.h"
 #include "blk_v1.h"
 #include "matvans_internal.h"
 #include "v0_stream.h"
@@ -131,17 +133,112 @@ __global__ void v0_tables(uint32_t *out, const uint8_t *cdf, const uint8_t *ppf,
     out[256 + q] = bias;
 }
 
+// ---- backward: dx = alpha * W^T (s_w * g) for every matrix of a batch (blk_bwd.h)
+
+enum : uint32_t { kBwdV0 = 0, kBwdQ32 = 1, kBwdQ64 = 2 };
+constexpr uint32_t kBwdMasks[] = {bit(kBwdV0), bit(kBwdQ32), bit(kBwdQ64),
+                                  bit(kBwdQ32) | bit(kBwdQ64), 0x7u};
+
+struct BwdItem {
+    uint32_t path;
+    uint32_t units;
+    uint32_t sgs;    // stream groups (units per row block)
+    uint32_t nrb;    // row blocks of 64 rows
+    blk::BwdJob job;
+};
+
+// reduce tile: 64 columns of one gradient vector of one item
+struct RTile {
+    uint32_t item, v, cb, pad;
+};
+
+struct NoSmem {};
+template <bool On, class T>
+using Opt = typename std::conditional<On, T, NoSmem>::type;
+
+template <uint32_t MASK>
+union BwdSmem {
+    Opt<(MASK & bit(kBwdV0)) != 0, blk::V0BwdSmem> v0;
+    Opt<(MASK & bit(kBwdQ32)) != 0, blk::QuadBwdSmem<32>> q32;
+    Opt<(MASK & bit(kBwdQ64)) != 0, blk::QuadBwdSmem<64>> q64;
+};
+
+// Same work queue as batch_fwd (sharded counters, static first tile, last-ticket reset);
+// a unit is one (row block, stream group) of the one-matrix backward.
+template <int NV, uint32_t MASK>
+__global__ __launch_bounds__(512) void batch_bwd(const BwdItem *__restrict__ items,
+                                                 const Tile *__restrict__ tiles, uint32_t n_tiles,
+                                                 uint32_t *counters, uint32_t queue) {
+    __shared__ BwdSmem<MASK> sm;
+    __shared__ uint32_t s_pos;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t g = blockIdx.x % kShards;
+    const uint32_t nb = (gridDim.x - g + kShards - 1) / kShards;
+    const uint32_t nt = n_tiles > g ? (n_tiles - g + kShards - 1) / kShards : 0u;
+    const uint32_t last_ticket = (nt > nb ? nt - nb : 0u) + (nb < nt ? nb : nt) - 1u;
+    uint32_t *counter = counters + g * kCounterStride;
+    uint32_t pos = blockIdx.x / kShards;
+    while (pos < nt) {
+        const Tile tl = tiles[g + kShards * pos];
+        const BwdItem &it = items[tl.item];
+        const uint32_t path = __builtin_amdgcn_readfirstlane(it.path);
+        const uint32_t sgs = __builtin_amdgcn_readfirstlane(it.sgs);
+        const uint32_t nrb = __builtin_amdgcn_readfirstlane(it.nrb);
+        for (uint32_t u = 0; u < tl.count; u++) {
+            const uint32_t unit = tl.first + u;
+            const uint32_t rb = unit / sgs, sg = unit % sgs;
+            if constexpr ((MASK & bit(kBwdV0)) != 0) {
+                if (path == kBwdV0) blk::v0_bwd_unit<NV>(it.job, rb, sg, nrb, sm.v0);
+            }
+            if constexpr ((MASK & bit(kBwdQ32)) != 0) {
+                if (path == kBwdQ32) blk::quad_bwd_unit<NV, 32>(it.job, rb, sg, nrb, sm.q32);
+            }
+            if constexpr ((MASK & bit(kBwdQ64)) != 0) {
+                if (path == kBwdQ64) blk::quad_bwd_unit<NV, 64>(it.job, rb, sg, nrb, sm.q64);
+            }
+            __syncthreads();   // the next unit rebuilds the tables and rings
+        }
+        if (queue) {
+            if (tid == 0) {
+                const uint32_t k = atomicAdd(counter, 1u);
+                if (k == last_ticket) atomicExch(counter, 0u);
+                s_pos = nb + k;
+            }
+            __syncthreads();
+            pos = s_pos;
+        } else {
+            pos += nb;
+        }
+    }
+}
+
+// Second launch of a batched backward: every (item, vector, 64-column block) reduce tile.
+__global__ __launch_bounds__(1024) void batch_bwd_reduce(const BwdItem *__restrict__ items,
+                                                         const RTile *__restrict__ rtiles) {
+    __shared__ float part[blk::kRedParts][64];
+    const RTile rt = rtiles[blockIdx.x];
+    const BwdItem &it = items[rt.item];
+    blk::reduce_tile(it.job, it.nrb, rt.v, rt.cb, part);
+}
+
 }  // namespace
 }  // namespace mv
 
 struct matvans_batch {
-    std::vector<mv::Item> items;          // host copies (device pointers inside)
+    bool backward = false;
+    std::vector<mv::Item> items;          // forward items (device pointers inside)
+    std::vector<mv::BwdItem> bitems;      // backward items
+    std::vector<uint32_t> units;          // units per item
     std::vector<uint32_t> chain;          // decode-chain length per unit, for ordering
     mv::Item *d_items = nullptr;
+    mv::BwdItem *d_bitems = nullptr;
     mv::Tile *d_tiles = nullptr;
+    mv::RTile *d_rtiles = nullptr;        // backward reduce tiles
     uint32_t *d_counters = nullptr;
     uint32_t *d_tables = nullptr;         // v0 decode tables, 512 dwords per v0 item
-    uint32_t n_tiles = 0, max_tiles = 0, nv = 1, n_cu = 0, mask = 0, kernel = 5;
+    float *d_partial = nullptr;           // backward 64-row partial sums, owned by the plan
+    uint32_t n_tiles = 0, max_tiles = 0, n_rtiles = 0, nv = 1, n_cu = 0, mask = 0, kernel = 5;
+    uint32_t max_bpc = 32;                // resident blocks per CU of the plan's kernel
     matvans_batch_config cfg{1, 0, 8, 0};
 };
 
@@ -151,9 +248,9 @@ hipError_t build_tiles(matvans_batch *p) {
     struct T { uint32_t item, first, count, chain; };
     std::vector<T> t;
     const uint32_t k = std::max(1u, p->cfg.units_per_tile);
-    for (uint32_t i = 0; i < p->items.size(); i++)
-        for (uint32_t u = 0; u < p->items[i].units; u += k)
-            t.push_back({i, u, std::min(k, p->items[i].units - u), p->chain[i]});
+    for (uint32_t i = 0; i < p->units.size(); i++)
+        for (uint32_t u = 0; u < p->units[i]; u += k)
+            t.push_back({i, u, std::min(k, p->units[i] - u), p->chain[i]});
     if (p->cfg.order == 1) {            // round-robin over matrices
         std::stable_sort(t.begin(), t.end(), [&](const T &a, const T &b) {
             uint32_t ra = a.first / k, rb = b.first / k;
@@ -215,6 +312,55 @@ int hip_fail(hipError_t e, const char *what) {
 
 }  // namespace
 
+namespace {
+
+using FwdKernel = void (*)(const mv::Item *, const mv::Tile *, uint32_t, uint32_t *, uint32_t);
+using BwdKernel = void (*)(const mv::BwdItem *, const mv::Tile *, uint32_t, uint32_t *, uint32_t);
+
+int nv_row(uint32_t nv) { return nv == 1 ? 0 : nv == 2 ? 1 : nv == 4 ? 2 : 3; }
+
+FwdKernel fwd_kernel(const matvans_batch *p) {
+#define MV_ROW(NV) {mv::batch_fwd<NV, mv::kMasks[0]>, mv::batch_fwd<NV, mv::kMasks[1]>, \
+                    mv::batch_fwd<NV, mv::kMasks[2]>, mv::batch_fwd<NV, mv::kMasks[3]>, \
+                    mv::batch_fwd<NV, mv::kMasks[4]>, mv::batch_fwd<NV, mv::kMasks[5]>}
+    static const FwdKernel table[4][6] = {MV_ROW(1), MV_ROW(2), MV_ROW(4), MV_ROW(8)};
+#undef MV_ROW
+    return table[nv_row(p->nv)][p->kernel];
+}
+
+BwdKernel bwd_kernel(const matvans_batch *p) {
+#define MV_ROW(NV) {mv::batch_bwd<NV, mv::kBwdMasks[0]>, mv::batch_bwd<NV, mv::kBwdMasks[1]>, \
+                    mv::batch_bwd<NV, mv::kBwdMasks[2]>, mv::batch_bwd<NV, mv::kBwdMasks[3]>, \
+                    mv::batch_bwd<NV, mv::kBwdMasks[4]>}
+    static const BwdKernel table[4][5] = {MV_ROW(1), MV_ROW(2), MV_ROW(4), MV_ROW(8)};
+#undef MV_ROW
+    return table[nv_row(p->nv)][p->kernel];
+}
+
+// Blocks of the plan's kernel that fit on one CU at once: a persistent grid larger than
+// that would leave blocks (and their fixed first tiles) waiting for others to finish.
+hipError_t set_max_bpc(matvans_batch *p) {
+    int nb = 0;
+    hipError_t e = p->backward
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+              &nb, reinterpret_cast<const void *>(bwd_kernel(p)), 512, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+              &nb, reinterpret_cast<const void *>(fwd_kernel(p)), 256, 0);
+    p->max_bpc = e == hipSuccess && nb > 0 ? uint32_t(nb) : 1u;
+    return e;
+}
+
+hipError_t device_cus(uint32_t &n_cu) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
+    n_cu = e == hipSuccess ? uint32_t(prop.multiProcessorCount) : 0u;
+    return e;
+}
+
+}  // namespace
+
 extern "C" int matvans_batch_create(const matvans_batch_item *items, uint32_t n,
                                     matvans_batch **out) {
     if (!items || !n || !out) return fail(MATVANS_E_INVALID, "null argument or empty batch");
@@ -227,12 +373,8 @@ extern "C" int matvans_batch_create(const matvans_batch_item *items, uint32_t n,
     }
     auto *p = new matvans_batch();
     p->nv = nv <= 1 ? 1 : nv <= 2 ? 2 : nv <= 4 ? 4 : 8;
-    int dev = 0;
-    hipDeviceProp_t prop;
-    hipError_t e = hipGetDevice(&dev);
-    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
+    hipError_t e = device_cus(p->n_cu);
     if (e != hipSuccess) { delete p; return hip_fail(e, "device query"); }
-    p->n_cu = uint32_t(prop.multiProcessorCount);
     if (n_v0) e = hipMalloc(&p->d_tables, size_t(n_v0) * 512 * sizeof(uint32_t));
     uint32_t v0_i = 0, max_tiles = 0;
     for (uint32_t k = 0; e == hipSuccess && k < n; k++) {
@@ -270,6 +412,7 @@ extern "C" int matvans_batch_create(const matvans_batch_item *items, uint32_t n,
         }
         max_tiles += m.units;
         p->mask |= 1u << m.path;
+        p->units.push_back(m.units);
         p->items.push_back(m);
     }
     if (e == hipSuccess) e = hipMalloc(&p->d_items, sizeof(mv::Item) * n);
@@ -282,6 +425,7 @@ extern "C" int matvans_batch_create(const matvans_batch_item *items, uint32_t n,
     p->max_tiles = max_tiles;
     for (uint32_t i = 0; i < 6; i++)
         if ((mv::kMasks[i] & p->mask) == p->mask) { p->kernel = i; break; }
+    if (e == hipSuccess) e = set_max_bpc(p);
     if (e == hipSuccess) e = build_tiles(p);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
@@ -293,12 +437,111 @@ extern "C" int matvans_batch_create(const matvans_batch_item *items, uint32_t n,
     return MATVANS_OK;
 }
 
+namespace {
+
+int check_bwd_item(const matvans_batch_bwd_item &it, uint32_t k) {
+    if (!it.g || !it.s_w || !it.alpha || !it.dx || !it.cursors || !it.payload)
+        return fail(MATVANS_E_INVALID, "item %u: null device pointer", k);
+    if (it.n_vec < 1 || it.n_vec > 8) return fail(MATVANS_E_INVALID, "item %u: n_vec %u", k, it.n_vec);
+    if (!it.rows || !it.cols) return fail(MATVANS_E_SHAPE, "item %u: empty matrix", k);
+    if (it.n_vec > 1 && it.g_stride < it.rows)
+        return fail(MATVANS_E_INVALID, "item %u: g_stride < rows", k);
+    if (reinterpret_cast<uintptr_t>(it.payload) % 16)
+        return fail(MATVANS_E_ALIGN, "item %u: payload not 16-byte aligned", k);
+    if (it.format == MATVANS_BATCH_V0) {
+        if (it.cols % 512) return fail(MATVANS_E_SHAPE, "item %u: cols %% 512", k);
+        if (!it.cdf || !it.ppf) return fail(MATVANS_E_INVALID, "item %u: null v0 tables", k);
+    } else if (it.format == MATVANS_BATCH_QUAD) {
+        if (it.streams != 32 && it.streams != 64)
+            return fail(MATVANS_E_UNSUPPORTED, "item %u: quad items need 32 or 64 streams", k);
+        if (it.cols % (16 * it.streams)) return fail(MATVANS_E_SHAPE, "item %u: cols", k);
+        if (!it.lut) return fail(MATVANS_E_INVALID, "item %u: null decode table", k);
+    } else {
+        return fail(MATVANS_E_UNSUPPORTED, "item %u: the backward takes .bin v0 and quad items", k);
+    }
+    return MATVANS_OK;
+}
+
+}  // namespace
+
+extern "C" int matvans_batch_create_backward(const matvans_batch_bwd_item *items, uint32_t n,
+                                             matvans_batch **out) {
+    if (!items || !n || !out) return fail(MATVANS_E_INVALID, "null argument or empty batch");
+    *out = nullptr;
+    uint32_t nv = 1;
+    size_t partial_floats = 0;
+    for (uint32_t k = 0; k < n; k++) {
+        if (int st = check_bwd_item(items[k], k)) return st;
+        nv = std::max(nv, items[k].n_vec);
+        const size_t nrb = (items[k].rows + 63) / 64;
+        partial_floats += (nrb * items[k].cols * items[k].n_vec + 63) / 64 * 64;
+    }
+    auto *p = new matvans_batch();
+    p->backward = true;
+    p->nv = nv <= 1 ? 1 : nv <= 2 ? 2 : nv <= 4 ? 4 : 8;
+    hipError_t e = device_cus(p->n_cu);
+    if (e == hipSuccess) e = hipMalloc(&p->d_partial, partial_floats * sizeof(float));
+    std::vector<mv::RTile> rt;
+    size_t off = 0;
+    uint32_t max_tiles = 0;
+    for (uint32_t k = 0; e == hipSuccess && k < n; k++) {
+        const auto &it = items[k];
+        mv::BwdItem b{};
+        const bool v0 = it.format == MATVANS_BATCH_V0;
+        b.path = v0 ? mv::kBwdV0 : it.streams == 32 ? mv::kBwdQ32 : mv::kBwdQ64;
+        b.sgs = (v0 ? 32u : it.streams) / mv::blk::kBwdWaves;
+        b.nrb = (it.rows + 63) / 64;
+        b.units = b.nrb * b.sgs;
+        b.job = mv::blk::BwdJob{p->d_partial + off, it.dx, it.g, it.s_w, it.alpha,
+                                it.n_vec > 1 ? it.g_stride : it.rows, it.n_vec, it.rows, it.cols,
+                                it.cursors, it.payload, it.payload_size, it.min_value, it.G,
+                                it.cdf, it.ppf, it.lut};
+        off += (size_t(b.nrb) * it.cols * it.n_vec + 63) / 64 * 64;
+        for (uint32_t v = 0; v < it.n_vec; v++)
+            for (uint32_t cb = 0; cb < (it.cols + 63) / 64; cb++) rt.push_back({k, v, cb, 0});
+        p->chain.push_back(v0 ? it.cols / 32 : it.cols / it.streams / 4);
+        p->units.push_back(b.units);
+        max_tiles += b.units;
+        p->mask |= 1u << b.path;
+        p->bitems.push_back(b);
+    }
+    if (e == hipSuccess) e = hipMalloc(&p->d_bitems, sizeof(mv::BwdItem) * n);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->d_bitems, p->bitems.data(), sizeof(mv::BwdItem) * n, hipMemcpyHostToDevice);
+    p->n_rtiles = uint32_t(rt.size());
+    if (e == hipSuccess) e = hipMalloc(&p->d_rtiles, sizeof(mv::RTile) * rt.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(p->d_rtiles, rt.data(), sizeof(mv::RTile) * rt.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&p->d_tiles, sizeof(mv::Tile) * max_tiles);
+    const size_t counter_bytes = mv::kShards * mv::kCounterStride * sizeof(uint32_t);
+    if (e == hipSuccess) e = hipMalloc(&p->d_counters, counter_bytes);
+    if (e == hipSuccess) e = hipMemset(p->d_counters, 0, counter_bytes);
+    p->max_tiles = max_tiles;
+    p->kernel = 4;
+    for (uint32_t i = 0; i < 5; i++)
+        if ((mv::kBwdMasks[i] & p->mask) == p->mask) { p->kernel = i; break; }
+    p->cfg.blocks_per_cu = 4;
+    if (e == hipSuccess) e = set_max_bpc(p);
+    if (e == hipSuccess) e = build_tiles(p);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        matvans_batch_destroy(p);
+        return hip_fail(e, "backward batch plan");
+    }
+    *out = p;
+    mv::clear_error();
+    return MATVANS_OK;
+}
+
 extern "C" void matvans_batch_destroy(matvans_batch *p) {
     if (!p) return;
     hipFree(p->d_items);
+    hipFree(p->d_bitems);
     hipFree(p->d_tiles);
+    hipFree(p->d_rtiles);
     hipFree(p->d_counters);
     hipFree(p->d_tables);
+    hipFree(p->d_partial);
     delete p;
 }
 
@@ -319,19 +562,23 @@ extern "C" int matvans_batch_get_config(const matvans_batch *p, matvans_batch_co
 
 extern "C" int matvans_batch_launch(const matvans_batch *p, void *stream) {
     if (!p) return fail(MATVANS_E_INVALID, "null plan");
-    // every shard that holds a tile needs a block: at least min(n_tiles, kShards) blocks
-    const uint32_t grid = std::max(1u, std::min(p->n_tiles, std::max(mv::kShards,
-                                                                     p->n_cu * p->cfg.blocks_per_cu)));
+    // every shard that holds a tile needs a block: at least min(n_tiles, kShards) blocks;
+    // no more blocks per CU than fit at once
+    const uint32_t bpc = std::min(p->cfg.blocks_per_cu, p->max_bpc);
+    const uint32_t grid = std::max(1u, std::min(p->n_tiles, std::max(mv::kShards, p->n_cu * bpc)));
+    const uint32_t queue = p->cfg.schedule == 0 ? 1u : 0u;
     auto s = reinterpret_cast<hipStream_t>(stream);
-    using K = void (*)(const mv::Item *, const mv::Tile *, uint32_t, uint32_t *, uint32_t);
-#define MV_ROW(NV) {mv::batch_fwd<NV, mv::kMasks[0]>, mv::batch_fwd<NV, mv::kMasks[1]>, \
-                    mv::batch_fwd<NV, mv::kMasks[2]>, mv::batch_fwd<NV, mv::kMasks[3]>, \
-                    mv::batch_fwd<NV, mv::kMasks[4]>, mv::batch_fwd<NV, mv::kMasks[5]>}
-    static const K table[4][6] = {MV_ROW(1), MV_ROW(2), MV_ROW(4), MV_ROW(8)};
-#undef MV_ROW
-    const int row = p->nv == 1 ? 0 : p->nv == 2 ? 1 : p->nv == 4 ? 2 : 3;
-    hipLaunchKernelGGL(table[row][p->kernel], dim3(grid), dim3(256), 0, s, p->d_items, p->d_tiles,
-                       p->n_tiles, p->d_counters, p->cfg.schedule == 0 ? 1u : 0u);
+    if (p->backward) {
+        hipLaunchKernelGGL(bwd_kernel(p), dim3(grid), dim3(512), 0, s, p->d_bitems, p->d_tiles,
+                           p->n_tiles, p->d_counters, queue);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "backward batch launch");
+        hipLaunchKernelGGL(mv::batch_bwd_reduce, dim3(p->n_rtiles), dim3(64 * mv::blk::kRedParts),
+                           0, s, p->d_bitems, p->d_rtiles);
+        return hip_fail(hipGetLastError(), "backward reduce launch");
+    }
+    hipLaunchKernelGGL(fwd_kernel(p), dim3(grid), dim3(256), 0, s, p->d_items, p->d_tiles,
+                       p->n_tiles, p->d_counters, queue);
     return hip_fail(hipGetLastError(), "batch launch");
 }
 

@@ -1,0 +1,301 @@
+// Block-level bodies of the backward dx = alpha * W^T (s_w * g) from the compressed
+// streams, shared by the one-matrix kernels (backward.hip) and the persistent batched
+// backward (batch.hip).  The reference has no backward pass.
+//
+// Mapping (stream-major): a unit is 8 waves = 8 consecutive streams of 64 consecutive
+// rows; a wave decodes one stream of the 64 rows, one row per lane.  A stream covers the
+// same columns in every row, so at each decode step all 64 lanes contribute to the same
+// columns.  Every four columns (one decode quad, its int8 symbols packed in one dword) a
+// fixed butterfly sums the 64 rows (bwd_quad): one v_permlane32_swap of the packed
+// dword, a multiply and an fma per pair of rows, one v_permlane16_swap and four DPP row
+// adds, in the same order on every run.  A 16-column group's totals are collected in one
+// register and stored once, to partial[v][row_block][c]; reduce_tile sums the row blocks
+// in a fixed order and applies alpha.  Both stages are deterministic bit for bit, so one-matrix and batched launches
+// give identical results.
+//
+// .bin v0 streams (create_matrices_WarpPerRow.py:361-366): stream l covers columns
+// 512*(k/16) + 16*l + k%16.  Quad-coder streams (S = 32 or 64): stream l's group t covers
+// columns 16*S*t + 16*l .. +15 and a quad step yields four consecutive ones.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "alu.h"
+#include "v0_stream.h"
+#include "v1_stream.h"
+
+namespace mv {
+namespace blk {
+
+constexpr int kBwdWaves = 8;
+constexpr int kRedParts = 16;
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_f(float v) {
+    int r = __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, true);
+    return __int_as_float(r);
+}
+
+// Sum over the 16 lanes of each DPP row in a fixed order (every lane gets its row sum).
+__device__ __forceinline__ float row_sum_fixed(float v) {
+    v += dpp_f<0xB1, 0xF>(v);    // quad_perm [1,0,3,2]
+    v += dpp_f<0x4E, 0xF>(v);    // quad_perm [2,3,0,1]
+    v += dpp_f<0x141, 0xF>(v);   // row_half_mirror
+    v += dpp_f<0x140, 0xF>(v);   // row_mirror
+    return v;
+}
+
+// One matrix of a backward (device pointers).
+struct BwdJob {
+    float *partial;              // workspace [n_vec][nrb][cols], nrb = ceil(rows / 64)
+    float *dx;                   // [n_vec][cols]
+    const float *g;              // [n_vec][g_stride]
+    const float *s_w;            // [rows]
+    const float *alpha;          // [n_vec]
+    uint32_t g_stride, n_vec, rows, cols;
+    const uint32_t *cursors;
+    const uint16_t *payload;
+    uint32_t payload_size;
+    int32_t min_value;           // .bin v0
+    uint32_t G;
+    const uint8_t *cdf, *ppf;
+    const uint32_t *glut;        // quad: A / B decode tables (2 x 256 dwords)
+};
+
+struct V0BwdSmem {
+    uint16_t ring[kBwdWaves * kRingWords * 64];
+    uint32_t lut[256];
+    uint16_t dummy[kDummyBytes / 2];
+};
+
+template <int S>
+struct QuadBwdSmem {
+    uint32_t ring[kBwdWaves * v1::QuadLane<128 / S>::R4 * 64];
+    uint32_t tab[256];
+    uint32_t dummy[(128 / S) * 64];
+};
+
+// Per-lane setup shared by both codecs: the 64-row window, this lane's row extent and
+// its scaled gradients (zero past the last row and past n_vec).
+template <int NV>
+struct BwdRows {
+    uint32_t row, wbeg, wend, rcur, rend;
+    bool valid;
+    float gs[NV];
+    __device__ __forceinline__ void init(const BwdJob &J, uint32_t rb, uint32_t lane) {
+        const uint32_t row0 = rb * 64u;
+        row = row0 + lane;
+        valid = row < J.rows;
+        const uint32_t last = row0 + 63u < J.rows ? row0 + 63u : J.rows - 1u;
+        wbeg = J.cursors[row0];
+        wend = last + 1 < J.rows ? J.cursors[last + 1] : J.payload_size;
+        if (wend > J.payload_size) wend = J.payload_size;
+        if (wend < wbeg) wend = wbeg;
+        wbeg = __builtin_amdgcn_readfirstlane(wbeg);
+        wend = __builtin_amdgcn_readfirstlane(wend);
+        rcur = valid ? J.cursors[row] : 0u;
+        rend = valid ? (row + 1 < J.rows ? J.cursors[row + 1] : J.payload_size) : 0u;
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+            gs[v] = valid && v < int(J.n_vec) ? J.s_w[row] * J.g[size_t(v) * J.g_stride + row]
+                                              : 0.0f;
+    }
+};
+
+// Butterfly of one decode quad: packed holds the four int8 symbols of this lane's row (in
+// column order), gsA / gsB the scaled gradients of rows (lane & 31) and (lane & 31) + 32,
+// sh = 0 for lanes 0..31 and 8 for lanes 32..63.  After the v_permlane32_swap, X holds
+// row (lane & 31)'s symbols and Y row (lane & 31) + 32's; shifted by sh, bytes 0 and 2 are
+// columns (0, 2) in lanes 0..31 and (1, 3) in lanes 32..63.  a, b are those two columns'
+// pair sums; v_permlane16_swap + add leaves the rows of 16 lanes with columns
+// (0, 2, 1, 3), summed over 32 row pairs by row_sum_fixed.  Lane 16k + d keeps the total
+// of its row k for quad d.
+template <int NV>
+__device__ __forceinline__ void bwd_quad(uint32_t packed, const float (&gsA)[NV],
+                                         const float (&gsB)[NV], uint32_t sh, uint32_t lane,
+                                         uint32_t d, float (&outv)[NV]) {
+    auto sw = __builtin_amdgcn_permlane32_swap(packed, packed, false, false);
+    const uint32_t X = sw[0] >> sh, Y = sw[1] >> sh;
+    const float xa = cvt_f32_sbyte0(X), xb = cvt_f32_sbyte2(X);
+    const float ya = cvt_f32_sbyte0(Y), yb = cvt_f32_sbyte2(Y);
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        const float a = fmaf(ya, gsB[v], xa * gsA[v]);
+        const float b = fmaf(yb, gsB[v], xb * gsA[v]);
+        auto t = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false,
+                                                  false);
+        const float tot = row_sum_fixed(__uint_as_float(t[0]) + __uint_as_float(t[1]));
+        outv[v] = (lane & 15) == d ? tot : outv[v];
+    }
+}
+
+// Per-lane constants of the butterfly and the group store.
+template <int NV>
+struct BwdLane {
+    float gsA[NV], gsB[NV];
+    uint32_t sh;      // 0 for lanes 0..31, 8 for lanes 32..63
+    uint32_t col;     // column of this lane's group total within the 16-column group
+    bool writer;      // lanes 16k + d, d < 4, hold a total
+    __device__ __forceinline__ void init(const float (&gs)[NV], uint32_t lane) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(gs[v]),
+                                                       __float_as_uint(gs[v]), false, false);
+            gsA[v] = __uint_as_float(sw[0]);
+            gsB[v] = __uint_as_float(sw[1]);
+        }
+        sh = lane < 32 ? 0u : 8u;
+        const uint32_t d = lane & 15, k = lane >> 4;   // rows of 16 lanes: columns 0, 2, 1, 3
+        col = 4 * d + ((k & 1) * 2 + (k >> 1));
+        writer = d < 4;
+    }
+    __device__ __forceinline__ void store(const float (&outv)[NV], float *out, size_t plane,
+                                          uint32_t n_vec) const {
+        if (writer) {
+#pragma unroll
+            for (int v = 0; v < NV; v++)
+                if (v < int(n_vec)) out[size_t(v) * plane + col] = outv[v];
+        }
+    }
+};
+
+// .bin v0 unit: row block rb, streams 8 sg .. 8 sg + 7.
+template <int NV>
+__device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32_t sg,
+                                            uint32_t nrb, V0BwdSmem &sm) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wave = tid >> 6, lane = tid & 63;
+    if (tid < 256) {   // packed table f | bias << 8 | sym << 24
+        uint32_t f, sym, bias;
+        v0_table_entry(tid, J.min_value, J.G, J.cdf, J.ppf, f, sym, bias);
+        sm.lut[tid] = f | (bias << 8) | (sym << 24);
+    }
+    const uint32_t stream = sg * kBwdWaves + wave;   // 0..31
+    BwdRows<NV> R;
+    R.init(J, rb, lane);
+
+    V0Lane st;
+    st.init(J.payload, R.wbeg, R.wend, R.valid, R.rcur, R.rend, stream, sm.ring, sm.dummy, wave,
+            lane);
+    st.prologue();
+    __syncthreads();
+    st.start();
+
+    const uint32_t cols = J.cols;
+    const uint32_t groups = cols / 512u;
+    const size_t plane = size_t(nrb) * cols;
+    float *base = J.partial + size_t(rb) * cols + stream * 16u;
+    BwdLane<NV> L;
+    L.init(R.gs, lane);
+    auto group = [&](uint32_t t) {
+        uint32_t wbefore = st.waddr;
+        float outv[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) outv[v] = 0.0f;
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            uint32_t e[4];
+            st.pair_packed(sm.lut, e[0], e[1]);
+            st.pair_packed(sm.lut, e[2], e[3]);
+            // symbols (byte 3 of each entry) packed in column order
+            const uint32_t packed = __builtin_amdgcn_perm(e[1], e[0], 0x0c0c0703u) |
+                                    __builtin_amdgcn_perm(e[3], e[2], 0x07030c0cu);
+            bwd_quad<NV>(packed, L.gsA, L.gsB, L.sh, lane, d, outv);
+        }
+        L.store(outv, base + t * 512u, plane, J.n_vec);
+        st.end_group(wbefore);
+    };
+    uint32_t t = 0;
+    for (; t + 1 < groups; t += 2) {
+        st.refill_a();
+        group(t);
+        st.refill_b();
+        group(t + 1);
+    }
+    if (t < groups) {
+        st.refill_a();
+        group(t);
+    }
+}
+
+// Quad-coder unit (S = 32 or 64 streams): row block rb, streams 8 sg .. 8 sg + 7.
+template <int NV, int S>
+__device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint32_t sg,
+                                              uint32_t nrb, QuadBwdSmem<S> &sm) {
+    using QL = v1::QuadLane<128 / S>;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wave = tid >> 6, lane = tid & 63;
+    // packed table T[q] = f | r << 8 | sym << 24 from the A / B tables (A = f | sym << 24,
+    // B = r)
+    if (tid < 256)
+        sm.tab[tid] = (J.glut[tid] & 0xFF0000FFu) | ((J.glut[256 + tid] & 0xFFu) << 8);
+    const uint32_t stream = sg * kBwdWaves + wave;   // 0..S-1
+    BwdRows<NV> R;
+    R.init(J, rb, lane);
+
+    QL st;
+    st.init(J.payload, R.wbeg, R.wend, R.valid, R.rcur, R.rend, stream, sm.ring, sm.dummy, wave,
+            lane);
+    st.prologue();
+    __syncthreads();
+    st.start();
+
+    const uint32_t cols = J.cols;
+    const uint32_t groups = cols / (16u * S);
+    const size_t plane = size_t(nrb) * cols;
+    float *base = J.partial + size_t(rb) * cols + stream * 16u;
+    BwdLane<NV> L;
+    L.init(R.gs, lane);
+    auto group = [&](uint32_t t) {
+        uint32_t wbefore = st.waddr;
+        float outv[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) outv[v] = 0.0f;
+#pragma unroll
+        for (int d = 0; d < 4; d++)
+            bwd_quad<NV>(st.quad(sm.tab), L.gsA, L.gsB, L.sh, lane, d, outv);
+        L.store(outv, base + t * (16u * S), plane, J.n_vec);
+        st.end_group(wbefore);
+    };
+    uint32_t t = 0;
+    for (; t + 1 < groups; t += 2) {
+        st.refill_a();
+        group(t);
+        st.refill_b();
+        group(t + 1);
+    }
+    if (t < groups) {
+        st.refill_a();
+        group(t);
+    }
+}
+
+// dx[v][c] = alpha[v] * sum over row blocks of partial[v][b][c] for the 64 columns of
+// column block cb, with 1024 threads (16 per column): thread part p sums the row blocks
+// [p*chunk, (p+1)*chunk) in ascending order and the 16 parts are added in ascending
+// order, so the result does not depend on scheduling.
+__device__ __forceinline__ void reduce_tile(const BwdJob &J, uint32_t nrb, uint32_t v,
+                                            uint32_t cb, float (&part)[kRedParts][64]) {
+    const uint32_t cl = threadIdx.x & 63, p = threadIdx.x >> 6;
+    const uint32_t cols = J.cols;
+    const uint32_t c = cb * 64 + cl;
+    const uint32_t chunk = (nrb + kRedParts - 1) / kRedParts;
+    float s = 0.0f;
+    if (c < cols) {
+        const float *src = J.partial + size_t(v) * nrb * cols + c;
+        uint32_t b0 = p * chunk, b1 = b0 + chunk < nrb ? b0 + chunk : nrb;
+        for (uint32_t b = b0; b < b1; b++) s += src[size_t(b) * cols];
+    }
+    part[p][cl] = s;
+    __syncthreads();
+    if (p == 0 && c < cols) {
+        float tot = part[0][cl];
+#pragma unroll
+        for (int q = 1; q < kRedParts; q++) tot += part[q][cl];
+        J.dx[size_t(v) * cols + c] = J.alpha[v] * tot;
+    }
+}
+
+}  // namespace blk
+}  // namespace mv

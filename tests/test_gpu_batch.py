@@ -148,3 +148,83 @@ def test_argument_errors(cuda):
     with pytest.raises(MatvansError) as e:       # 9 vectors
         Batch([(c, torch.zeros((9, 2048), dtype=torch.int8, device="cuda"))])
     assert e.value.status == 1
+
+
+# ---- batched backward: bitwise equal to each matrix's own backward
+
+def bwd_inputs(m, n_vec, seed):
+    import torch
+    rng = np.random.default_rng(seed)
+    g = torch.from_numpy(rng.standard_normal((n_vec, m.shape[0])).astype(np.float32)).cuda()
+    s_w = torch.from_numpy((rng.random(m.shape[0]) * 0.02 + 0.001).astype(np.float32)).cuda()
+    alpha = torch.from_numpy((rng.random(n_vec) + 0.5).astype(np.float32)).cuda()
+    return g, s_w, alpha
+
+
+@pytest.mark.parametrize("fmt", ["v0", "q32", "q64"])
+@pytest.mark.parametrize("n_vec", [1, 8])
+def test_backward_batch_block(cuda, fmt, n_vec):
+    import torch
+    from matvans_amd.batch import BatchBackward
+    note(f"backward batch decoder block {fmt} n_vec={n_vec}")
+    mats = [gaussian_int8(r, c, 6.0, seed=r + 7 * c + i)[0] for i, (r, c) in enumerate(BLOCK)]
+    cms = [make(fmt, m) for m in mats]
+    ins = [bwd_inputs(m, n_vec, i) for i, m in enumerate(mats)]
+    singles = [c.backward(g, s, a).cpu().numpy() for c, (g, s, a) in zip(cms, ins)]
+    b = BatchBackward([(c, g, s, a) for c, (g, s, a) in zip(cms, ins)])
+    for cfg in [None, (1, 2, 4, 0), (4, 0, 2, 1), (2, 1, 8, 0)]:
+        if cfg:
+            b.configure(*cfg)
+        for rep in range(2):
+            for o in b.outputs:
+                o.zero_()
+            b.launch()
+            torch.cuda.synchronize()
+            for o, want in zip(b.outputs, singles):
+                assert o.cpu().numpy().tobytes() == want.tobytes(), (cfg, rep)
+
+
+def test_backward_batch_mixed_and_graph(cuda):
+    import torch
+    from matvans_amd.batch import BatchBackward
+    note("backward batch mixed uneven + graph replays")
+    rng = np.random.default_rng(11)
+    specs = [("v0", 65, 1024), ("q32", 1000, 2048), ("q64", 100, 1024), ("v0", 300, 512),
+             ("q32", 7, 512), ("q64", 2049, 3072)]
+    entries, singles, dense = [], [], []
+    for i, (fmt, r, c) in enumerate(specs):
+        m = gaussian_int8(r, c, [2.0, 6.0, 16.0][i % 3], seed=50 + i)[0]
+        cm = make(fmt, m)
+        n_vec = int(rng.integers(1, 9))
+        g, s_w, alpha = bwd_inputs(m, n_vec, 100 + i)
+        entries.append((cm, g, s_w, alpha))
+        singles.append(cm.backward(g, s_w, alpha).cpu().numpy())
+        ref = alpha.cpu().numpy()[:, None].astype(np.float64) * (
+            (g.cpu().numpy().astype(np.float64) * s_w.cpu().numpy()[None, :]) @ m.astype(np.float64))
+        dense.append(ref)
+    b = BatchBackward(entries)
+    b.autotune(reps=2)
+    b.launch()
+    torch.cuda.synchronize()
+    for o, want, ref in zip(b.outputs, singles, dense):
+        got = o.cpu().numpy()
+        assert got.tobytes() == want.tobytes()
+        assert np.linalg.norm(got - ref) <= 1e-5 * np.linalg.norm(ref)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    gph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        b.launch(stream=s)
+        s.synchronize()
+        with torch.cuda.graph(gph, stream=s):
+            b.launch(stream=s)
+    torch.cuda.synchronize()
+    for rep in range(3):
+        for o in b.outputs:
+            o.zero_()
+        torch.cuda.synchronize()
+        gph.replay()
+        torch.cuda.synchronize()
+        assert float(torch.ones(100, device="cuda").sum()) == 100.0   # eager work in between
+        for o, want in zip(b.outputs, singles):
+            assert o.cpu().numpy().tobytes() == want.tobytes(), rep
