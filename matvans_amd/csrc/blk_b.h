@@ -17,11 +17,14 @@
 // reference's register cache and shuffles: a ballot of the coders that need a word,
 // v_mbcnt for each lane's rank in its half, and one LDS read.  Stream word n (state
 // words excluded) lives in ring slot (n + 64) & 511, so a chunk never wraps.
-// Tables in LDS: A[q] = prob | symbol << 24 and B[q] = q - cum for the 128 quantiles.
+// Table in LDS: one packed dword prob | (q - cum) << 8 | symbol << 24 per quantile, so a
+// symbol costs one lookup and the products take the byte fields by SDWA selects (alu.h).
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+
+#include "alu.h"
 
 namespace mv {
 namespace blk {
@@ -33,15 +36,15 @@ constexpr uint32_t kOOB = 0x80000000u;
 __device__ __forceinline__ uint32_t byte_of(uint32_t w, uint32_t k) { return (w >> (8 * k)) & 0xFFu; }
 
 // Packed byte tables (ppf, cum, prob: entry q is byte q%4 of word q/4) to the fused
-// dword tables in LDS.
+// dword table in LDS: T[q] = prob | (q - cum) << 8 | symbol << 24 (prob <= 128 and
+// q - cum < prob, so both fit a byte).
 __device__ __forceinline__ void build_tables(uint32_t *lut, const uint32_t *ppf,
                                              const uint32_t *cum, const uint32_t *prob,
                                              uint32_t tid) {
     if (tid < 128) {
         uint32_t w = tid >> 2, k = tid & 3;
         uint32_t sym = byte_of(ppf[w], k), c = byte_of(cum[w], k), f = byte_of(prob[w], k);
-        lut[tid] = f | (sym << 24);
-        lut[128 + tid] = (tid - c) & 0xFFu;
+        lut[tid] = f | (((tid - c) & 0xFFu) << 8) | (sym << 24);
     }
 }
 
@@ -180,7 +183,6 @@ __device__ __forceinline__ void b_block(const BJob &J, uint32_t bid, BSmem &sm) 
     for (int v = 0; v < NV; v++) acc[v] = 0;
 
     const uint32_t n_iter = LAYOUT == 0 ? in_dim / 128u : in_dim / 4u;
-    const uint32_t *vec32 = reinterpret_cast<const uint32_t *>(vecs);
 
     auto step = [&](uint32_t t, const uint32_t *vq) {
         // refill: coders whose state is below 2^32 take the next words in lane order
@@ -195,74 +197,99 @@ __device__ __forceinline__ void b_block(const BJob &J, uint32_t bid, BSmem &sm) 
         lo = need ? w : lo;
         cur_a += plo;
         cur_b += phi;
-        // decode a quad
-        uint32_t ea[4], eb[4];
+        // decode a quad: four lookups of the packed table, the byte fields taken by SDWA
+        // operand selects: P = p0 p1 p2 p3, R = r0 + p0 (r1 + p1 (r2 + p2 r3))
+        uint32_t e[4];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            uint32_t q = (lo >> (7 * i)) & 0x7Fu;
-            ea[i] = lut[q];
-            eb[i] = lut[128 + q];
-        }
-        uint32_t p01 = __umul24(ea[0], ea[1]);
-        uint32_t r01 = __umul24(eb[1], ea[0]) + eb[0];
-        uint32_t p23 = __umul24(ea[2], ea[3]);
-        uint32_t r23 = __umul24(eb[3], ea[2]) + eb[2];
-        uint32_t P = __umul24(p01, p23);
-        uint32_t R = __umul24(r23, p01) + r01;
+        for (int i = 0; i < 4; i++) e[i] = lut[(lo >> (7 * i)) & 0x7Fu];
+        const uint32_t p01 = mul_b0_b0(e[0], e[1]);
+        const uint32_t r01 = add_b1(mul_b1_b0(e[1], e[0]), e[0]);
+        const uint32_t p23 = mul_b0_b0(e[2], e[3]);
+        const uint32_t r23 = add_b1(mul_b1_b0(e[3], e[2]), e[2]);
+        const uint32_t P = mul_u24(p01, p23);
+        const uint32_t R = mad_u24(r23, p01, r01);
         uint64_t x = ((uint64_t(hi) << 32) | lo) >> 28;
         uint64_t ns = x * P + R;
         lo = uint32_t(ns);
         hi = uint32_t(ns >> 32);
-        uint32_t packed = __builtin_amdgcn_perm(ea[1], ea[0], 0x0c0c0703u) |
-                          __builtin_amdgcn_perm(ea[3], ea[2], 0x07030c0cu);
+        uint32_t packed = __builtin_amdgcn_perm(e[1], e[0], 0x0c0c0703u) |
+                          __builtin_amdgcn_perm(e[3], e[2], 0x07030c0cu);
 #pragma unroll
         for (int v = 0; v < NV; v++)
             acc[v] = __builtin_amdgcn_sdot4(int(packed), int(vq[v]), acc[v], false);
         (void)t;
     };
 
-    // vector words of step t: rows: word 32t + lane; stripes: word t (same for all lanes)
-    auto vword = [&](uint32_t t, int v) -> uint32_t {
-        uint32_t tt = t < n_iter ? t : n_iter - 1;
-        uint32_t idx = LAYOUT == 0 ? tt * 32u + lane : tt;
-        return vec32[(size_t(v < int(J.n_vec) ? v : 0) * vec_stride >> 2) + idx];
+    // Vector words of a 4-step group gg, loaded with the chunk loads two groups ahead
+    // and kept in registers: rows: words 32t + lane of each step t; stripes: word t (the
+    // same for all lanes).  Buffer loads over each vector's in_dim bytes return zeros past
+    // the end; they count on the vector-memory counter, so the LDS waits of the decode
+    // never wait for them (a scalar load would share the LDS counter).
+    __amdgpu_buffer_rsrc_t vr[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+        vr[v] = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<int8_t *>(vecs) + size_t(v < int(J.n_vec) ? v : 0) * vec_stride, 0,
+            int(in_dim), 0x00020000);
+    auto vload_group = [&](uint32_t gg, uint32_t (&w)[4][NV]) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t t = gg * 4 + k;
+            const uint32_t off = LAYOUT == 0 ? (t * 32u + lane) * 4u : t * 4u;
+#pragma unroll
+            for (int v = 0; v < NV; v++)
+                w[k][v] = __builtin_amdgcn_raw_buffer_load_b32(vr[v], t < n_iter ? off : kOOB, 0, 0);
+        }
     };
 
-    // groups of four steps; chunk writes and loads alternate A/B between groups
+    // groups of four steps; chunk writes and loads alternate A/B between groups, and so do
+    // the vector-word registers
     const uint32_t groups = (n_iter + 3) / 4;
-    uint32_t vq[4][NV];
+    uint32_t va[4][NV], vb[4][NV];
+    vload_group(0, va);
+    vload_group(1, vb);
+    // full groups in pairs (A, B buffers), then at most one full group on A and one partial
+    // group; the loop body has no conditional steps, so the read counters stay scalar
+    const uint32_t full = n_iter / 4;
+    uint32_t wk[4][NV];
+    auto take = [&](uint32_t (&w)[4][NV]) {
 #pragma unroll
-    for (int k = 0; k < 4; k++)
+        for (int k = 0; k < 4; k++)
 #pragma unroll
-        for (int v = 0; v < NV; v++) vq[k][v] = vword(k, v);
-    for (uint32_t g = 0; g < groups; g++) {
-        uint32_t cur = half_sel ? cur_b : cur_a;
-        if ((g & 1) == 0) {
-            st.write_chunk(st.pa, st.ga);
-            st.issue(st.pa, st.ga, cur);
-        } else {
-            st.write_chunk(st.pb, st.gb);
-            st.issue(st.pb, st.gb, cur);
-        }
-        const uint32_t t0 = g * 4;
-        if (t0 + 4 <= n_iter) {
+            for (int v = 0; v < NV; v++) wk[k][v] = w[k][v];
+    };
+    uint32_t g = 0;
+    for (; g + 1 < full; g += 2) {
+        st.write_chunk(st.pa, st.ga);
+        st.issue(st.pa, st.ga, half_sel ? cur_b : cur_a);
+        take(va);
+        vload_group(g + 2, va);
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                uint32_t cur_vq[NV];
+        for (int k = 0; k < 4; k++) step(g * 4 + k, wk[k]);
+        st.write_chunk(st.pb, st.gb);
+        st.issue(st.pb, st.gb, half_sel ? cur_b : cur_a);
+        take(vb);
+        vload_group(g + 3, vb);
 #pragma unroll
-                for (int v = 0; v < NV; v++) cur_vq[v] = vq[k][v];
+        for (int k = 0; k < 4; k++) step(g * 4 + 4 + k, wk[k]);
+    }
+    if (g < groups) {   // group g on A (full or partial)
+        st.write_chunk(st.pa, st.ga);
+        st.issue(st.pa, st.ga, half_sel ? cur_b : cur_a);
+        take(va);
+        const uint32_t n = n_iter - g * 4 < 4 ? n_iter - g * 4 : 4;
 #pragma unroll
-                for (int v = 0; v < NV; v++) vq[k][v] = vword(t0 + 4 + k, v);
-                step(t0 + k, cur_vq);
-            }
-        } else {
-            for (uint32_t k = 0; t0 + k < n_iter; k++) {
-                uint32_t cur_vq[NV];
+        for (int k = 0; k < 4; k++)
+            if (uint32_t(k) < n) step(g * 4 + k, wk[k]);
+        g++;
+    }
+    if (g < groups) {   // group g on B (partial)
+        st.write_chunk(st.pb, st.gb);
+        take(vb);
+        const uint32_t n = n_iter - g * 4;
 #pragma unroll
-                for (int v = 0; v < NV; v++) cur_vq[v] = vword(t0 + k, v);
-                step(t0 + k, cur_vq);
-            }
-        }
+        for (int k = 0; k < 4; k++)
+            if (uint32_t(k) < n) step(g * 4 + k, wk[k]);
     }
 
     if constexpr (LAYOUT == 0) {
