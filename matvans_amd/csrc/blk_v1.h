@@ -193,10 +193,12 @@ __device__ __forceinline__ void v1_block(const V1Job &J, uint32_t bid, V1Smem<NV
                                                     tt * (16u * S) + sl * 16u);
     };
     uint4 vc0[NV], vc1[NV];
+    // load order vc0, granules A, granules B, vc1 as in the loop body (A, vc0, B, vc1), so
+    // that the loop's wait before group A leaves the two newest loads (B, vc1) in flight
     vload(0, vc0);
-    vload(1, vc1);
     __syncthreads();
     st.start();
+    vload(1, vc1);
 
     int32_t acc[NV];
 #pragma unroll
@@ -340,10 +342,12 @@ __device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid, QuadSme
                                                     tt * (16u * S) + sl * 16u);
     };
     uint4 vc0[NV], vc1[NV];
+    // load order vc0, granules A, granules B, vc1 as in the loop body (A, vc0, B, vc1), so
+    // that the loop's wait before group A leaves the two newest loads (B, vc1) in flight
     vload(0, vc0);
-    vload(1, vc1);
     __syncthreads();
     st.start();
+    vload(1, vc1);
 
     int32_t acc[NV];
 #pragma unroll
