@@ -42,10 +42,23 @@ def last_json(out):
     return json.loads(out.strip().splitlines()[-1])
 
 
+@pytest.fixture
+def gpu_ready(cuda):
+    """The test process initialises its own GPU runtimes (torch's and the library's, with
+    one small device call) before any child process uses the GPU: when children ran
+    first, this process's later GPU calls failed ("no ROCm-capable device") or crashed."""
+    import torch
+    import matvans_amd as mv
+    w = torch.ones((4, 16), dtype=torch.int8, device=cuda)
+    x = torch.ones((1, 16), dtype=torch.int8, device=cuda)
+    assert int(mv.int8_matvec(w, x)[0, 0]) == 16
+    torch.cuda.synchronize()
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("fx", ["compressed_matrices_512_5bit_count2.bin",
                                 "compressed_matrices_1024_5bit_count1.bin"])
-def test_driver_chain_on_golden_bin(fx):
+def test_driver_chain_on_golden_bin(gpu_ready, fx):
     r = run(os.path.join(GOLDEN, fx), "--iters", "3")
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.count("Hashes match!") == 3
@@ -55,7 +68,7 @@ def test_driver_chain_on_golden_bin(fx):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("streams,precision,coder", [(32, 8, 1), (32, 12, 0), (32, 8, 0)])
-def test_driver_chain_on_mvx1(tmp_path, streams, precision, coder):
+def test_driver_chain_on_mvx1(gpu_ready, tmp_path, streams, precision, coder):
     """The golden raw matrices re-encoded as MVX1: same chain, same result_hash."""
     from matvans_amd.mvx import CompressedMatrixV1, write_mvx1
     from oracle import oracle as O
