@@ -62,6 +62,7 @@ FORMATS = {
     "q32": (32, 8, 0, 1),
     "q64": (64, 8, 0, 1),
     "q128": (128, 8, 0, 1),
+    "q64-rowcdf": (64, 8, 1, 1),
     "b-rows": "rows",
     "b-stripes": "stripes",
 }
@@ -75,11 +76,13 @@ FORMAT_TEXT = {
     "q32": "MVX1 quad coder: 32 streams/row, 64-bit state, global 8-bit CDF",
     "q64": "MVX1 quad coder: 64 streams/row, 64-bit state, global 8-bit CDF",
     "q128": "MVX1 quad coder: 128 streams/row, 64-bit state, global 8-bit CDF",
+    "q64-rowcdf": "MVX1 quad coder: 64 streams/row, 64-bit state, per-row 8-bit CDF",
     "b-rows": "family B (buffered) ans-warp-rows: quad rANS, 7-bit CDF, 32 coders/row",
     "b-stripes": "family B (buffered) ans-warp-stripes: quad rANS, 7-bit CDF, 32 rows/warp",
 }
 BACKWARD_FORMATS = ("v0", "q32", "q64")
-DEFAULT_FORMATS = ["v0", "q32", "q64", "q128", "s128-p8", "s64-p8", "s64-p12", "s64-p8-rowcdf",
+DEFAULT_FORMATS = ["v0", "q32", "q64", "q128", "q64-rowcdf", "s128-p8", "s64-p8", "s64-p12",
+                   "s64-p8-rowcdf",
                    "s32-p12", "b-rows"]
 
 

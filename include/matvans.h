@@ -226,9 +226,10 @@ int matvans_encode_rows(const int8_t *mat, uint32_t rows, uint32_t cols, int8_t 
  *   precision  8 or 12 bit probabilities (decode q = x & (2^P-1);
  *              x = (x >> P)*f + q - cdf[s]; renormalise with a 16-bit word below 2^16).
  *   table_mode 0: one table per matrix, u16 cdf[G+1] (cdf[G] = 2^P);
- *              1: one 8-bit table per row (P = 8 only), u8 freq[rows][G] summing to 256.
+ *              1: one 8-bit table per row (P = 8 only), u8 freq[rows][G] summing to 256;
+ *                 decoded with S = 64 (one row per wave builds its own table).
  *   coder      0: one symbol per step (32-bit state, 16-bit words, as the reference);
- *              1: quad coder (P = 8, table_mode 0): 64-bit state x, 32-bit words, a step
+ *              1: quad coder (P = 8, either table mode): 64-bit state x, 32-bit words, a step
  *              decodes the four bytes of the low word of x (if x < 2^32 first
  *              x = x << 32 | next word; then x = (x >> 32)*f0 f1 f2 f3 + r0 + f0(r1 +
  *              f1(r2 + f2 r3)), r_i = q_i - cdf[s_i]) -- the family-B quad coder of

@@ -70,14 +70,66 @@ struct V1Smem {
     int32_t part[WPR > 1 ? kWaves * NV : 1];
 };
 
-template <int NV, int S>
+template <int NV, int S, int MODE = 0>
 struct QuadSmem {
     static constexpr int WPR = S > 64 ? S / 64 : 1;
     uint32_t ring[kWaves * 16 * 64];
-    uint32_t tab[256];
+    uint32_t tab[MODE == 0 ? 256 : 1];
+    uint32_t rowtab[MODE == 1 ? kWaves * 448 : 1];   // per wave: packed table + scratch
     uint32_t dummy[4 * 64];
     int32_t part[WPR > 1 ? kWaves * NV : 1];
 };
+
+// One 8-bit table per row (table mode 1), built by the row's wave (S = 64: 64 lanes, four
+// quantiles each): an exclusive scan of the row's frequencies gives cdf, every present
+// symbol marks its first quantile, and an inclusive max-scan over the quantiles gives
+// ppf.  scratch: 768 bytes of the wave's LDS.  put(q, f, q - cdf[s], (int8)(min + s)) is
+// called for each of the lane's four quantiles.
+template <class Put>
+__device__ __forceinline__ void build_row_table(const uint8_t *__restrict__ freq, uint32_t row,
+                                                uint32_t G, bool valid, int min_value,
+                                                uint32_t lane, uint8_t *scratch, Put put) {
+    uint8_t *mark = scratch, *sf = scratch + 256, *sc = scratch + 512;
+    uint32_t f[4], c[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        uint32_t s = lane * 4 + k;
+        f[k] = valid && s < G ? freq[size_t(row) * G + s] : 0u;
+    }
+    c[0] = 0; c[1] = f[0]; c[2] = f[0] + f[1]; c[3] = c[2] + f[2];
+    uint32_t e = wave_excl_sum(c[3] + f[3], lane);
+    uint32_t fw = 0, cw = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        c[k] = (c[k] + e) & 0xFFu;
+        fw |= f[k] << (8 * k);
+        cw |= c[k] << (8 * k);
+    }
+    reinterpret_cast<uint32_t *>(mark)[lane] = 0;
+    reinterpret_cast<uint32_t *>(sf)[lane] = fw;
+    reinterpret_cast<uint32_t *>(sc)[lane] = cw;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (f[k]) mark[c[k]] = static_cast<uint8_t>(lane * 4 + k);
+    __syncthreads();
+    uint32_t m = reinterpret_cast<const uint32_t *>(mark)[lane];
+    uint32_t a[4];
+    a[0] = m & 0xFFu;
+#pragma unroll
+    for (int k = 1; k < 4; k++) {
+        uint32_t b = (m >> (8 * k)) & 0xFFu;
+        a[k] = b > a[k - 1] ? b : a[k - 1];
+    }
+    uint32_t pm = wave_excl_max(a[3], lane);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        uint32_t q = lane * 4 + k;
+        uint32_t s = a[k] > pm ? a[k] : pm;
+        put(q, uint32_t(sf[s]), (q - sc[s]) & 0xFFu,
+            uint32_t(static_cast<uint8_t>(min_value + int(s))));
+    }
+}
 
 // Single-symbol coder (MVX1 coder 0; the reference's .bin v0 layout is S = 32, P = 8).
 template <int NV, int S, int P, int MODE>
@@ -120,49 +172,15 @@ __device__ __forceinline__ void v1_block(const V1Job &J, uint32_t bid, V1Smem<NV
         uint4 *d = reinterpret_cast<uint4 *>(lut);
         for (uint32_t i = tid; i < uint32_t(kLut / 4); i += 256) d[i] = src[i];
     } else {
-        // one table per row; S == 64, so the wave owns exactly this row
+        // one table per row; S == 64, so the wave owns exactly this row: three byte tables
+        // f / bias / symbol (one dword per LDS bank each, conflict-free lookups)
         uint8_t *tab = reinterpret_cast<uint8_t *>(rowtab + wave * 384);
-        uint8_t *mark = tab + 768, *sf = tab + 1024, *sc = tab + 1280;
-        uint32_t f[4], c[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            uint32_t s = lane * 4 + k;
-            f[k] = valid && s < G ? freq[size_t(row) * G + s] : 0u;
-        }
-        c[0] = 0; c[1] = f[0]; c[2] = f[0] + f[1]; c[3] = c[2] + f[2];
-        uint32_t e = wave_excl_sum(c[3] + f[3], lane);
-        uint32_t fw = 0, cw = 0;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            c[k] = (c[k] + e) & 0xFFu;
-            fw |= f[k] << (8 * k);
-            cw |= c[k] << (8 * k);
-        }
-        reinterpret_cast<uint32_t *>(mark)[lane] = 0;
-        reinterpret_cast<uint32_t *>(sf)[lane] = fw;
-        reinterpret_cast<uint32_t *>(sc)[lane] = cw;
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < 4; k++)
-            if (f[k]) mark[c[k]] = static_cast<uint8_t>(lane * 4 + k);
-        __syncthreads();
-        uint32_t m = reinterpret_cast<const uint32_t *>(mark)[lane];
-        uint32_t a[4];
-        a[0] = m & 0xFFu;
-#pragma unroll
-        for (int k = 1; k < 4; k++) {
-            uint32_t b = (m >> (8 * k)) & 0xFFu;
-            a[k] = b > a[k - 1] ? b : a[k - 1];
-        }
-        uint32_t pm = wave_excl_max(a[3], lane);
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            uint32_t q = lane * 4 + k;
-            uint32_t s = a[k] > pm ? a[k] : pm;
-            tab[q] = sf[s];
-            tab[256 + q] = static_cast<uint8_t>(q - sc[s]);
-            tab[512 + q] = static_cast<uint8_t>(min_value + int(s));
-        }
+        build_row_table(freq, row, G, valid, min_value, lane, tab + 768,
+                        [&](uint32_t q, uint32_t f, uint32_t b, uint32_t sy) {
+                            tab[q] = static_cast<uint8_t>(f);
+                            tab[256 + q] = static_cast<uint8_t>(b);
+                            tab[512 + q] = static_cast<uint8_t>(sy);
+                        });
     }
 
     // wave window: rows row0 .. row0 + RPW - 1
@@ -287,14 +305,15 @@ __device__ __forceinline__ void v1_block(const V1Job &J, uint32_t bid, V1Smem<NV
 
 // Quad coder (MVX1 coder 1, P = 8, one table per matrix): same mapping, per-lane 64-bit
 // state; a 16-symbol group is four quads, each one v_dot4 per vector.
-template <int NV, int S>
-__device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid, QuadSmem<NV, S> &sm) {
+template <int NV, int S, int MODE = 0>
+__device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid,
+                                           QuadSmem<NV, S, MODE> &sm) {
+    static_assert(MODE == 0 || S == 64, "one table per row needs one row per wave");
     constexpr int RPW = S < 64 ? 64 / S : 1;
     constexpr int WPR = S > 64 ? S / 64 : 1;
     constexpr int G4 = 128 / S;
     using QL = v1::QuadLane<G4>;
     uint32_t *ring = sm.ring;
-    uint32_t *tab = sm.tab;
     uint32_t *dummy = sm.dummy;
     int32_t *part = sm.part;
     const uint32_t rows = J.rows, cols = J.cols, vec_stride = J.vec_stride;
@@ -312,9 +331,20 @@ __device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid, QuadSme
                                   : (bid * kWaves + wave) * RPW;
     const uint32_t row = row0 + (S >= 64 ? 0u : lane / S);
     const bool valid = row < rows;
-    // packed table T[q] = f | r << 8 | sym << 24 from the A / B tables (A = f | sym << 24,
-    // B = r); f <= 255 and r < f for every table the encoder accepts
-    tab[tid] = (glut[tid] & 0xFF0000FFu) | ((glut[256 + tid] & 0xFFu) << 8);
+    // packed table T[q] = f | r << 8 | sym << 24 (r = q - cdf[s]); f <= 255 and r < f for
+    // every table the encoder accepts
+    uint32_t *tab;
+    if constexpr (MODE == 0) {   // from the A / B tables (A = f | sym << 24, B = r)
+        tab = sm.tab;
+        tab[tid] = (glut[tid] & 0xFF0000FFu) | ((glut[256 + tid] & 0xFFu) << 8);
+    } else {                     // this wave's row's own table
+        tab = sm.rowtab + wave * 448;
+        build_row_table(J.freq, row, J.G, valid, J.min_value, lane,
+                        reinterpret_cast<uint8_t *>(tab + 256),
+                        [&](uint32_t q, uint32_t f, uint32_t b, uint32_t sy) {
+                            tab[q] = f | (b << 8) | (sy << 24);
+                        });
+    }
     uint32_t wbeg = 0, wend = 0;
     if (row0 < rows) {
         wbeg = cursors[row0];

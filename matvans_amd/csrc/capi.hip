@@ -108,8 +108,8 @@ extern "C" int matvans_v1_decompress_and_multiply(const matvans_v1_dev_matrix *m
     if (m->table_mode == 1 && (m->streams != 64 || m->precision != 8))
         return fail(MATVANS_E_UNSUPPORTED, "one table per row needs 64 streams and 8-bit tables");
     if (m->coder > 1) return fail(MATVANS_E_INVALID, "coder %u", m->coder);
-    if (m->coder == 1 && (m->precision != 8 || m->table_mode != 0))
-        return fail(MATVANS_E_UNSUPPORTED, "the quad coder needs 8-bit probabilities and one table");
+    if (m->coder == 1 && m->precision != 8)
+        return fail(MATVANS_E_UNSUPPORTED, "the quad coder needs 8-bit probabilities");
     if (m->table_mode == 0 && !m->lut) return fail(MATVANS_E_INVALID, "null decode table");
     if (m->table_mode == 1 && (!m->freq || m->G < 2 || m->G > 256))
         return fail(MATVANS_E_INVALID, "per-row tables need freq and 2 <= G <= 256");

@@ -36,10 +36,10 @@ __global__ __launch_bounds__(256) void v1_fwd(V1Job job) {
     blk::v1_block<NV, S, P, MODE>(job, blockIdx.x, sm);
 }
 
-template <int NV, int S>
+template <int NV, int S, int MODE>
 __global__ __launch_bounds__(256) void v1q_fwd(V1Job job) {
-    __shared__ blk::QuadSmem<NV, S> sm;
-    blk::quad_block<NV, S>(job, blockIdx.x, sm);
+    __shared__ blk::QuadSmem<NV, S, MODE> sm;
+    blk::quad_block<NV, S, MODE>(job, blockIdx.x, sm);
 }
 
 V1Job job_of(const V1Args &a) {
@@ -47,11 +47,11 @@ V1Job job_of(const V1Args &a) {
                  a.G, a.lut, a.freq, a.payload, a.payload_size};
 }
 
-template <int NV, int S>
+template <int NV, int S, int MODE = 0>
 hipError_t launch_quad(const V1Args &a, hipStream_t s) {
     constexpr uint32_t rows_per_block = S > 64 ? kWaves / (S / 64) : kWaves * (64 / S);
     dim3 grid((a.rows + rows_per_block - 1) / rows_per_block), block(256);
-    hipLaunchKernelGGL((v1q_fwd<NV, S>), grid, block, 0, s, job_of(a));
+    hipLaunchKernelGGL((v1q_fwd<NV, S, MODE>), grid, block, 0, s, job_of(a));
     return hipGetLastError();
 }
 
@@ -66,7 +66,9 @@ hipError_t launch_cfg(const V1Args &a, hipStream_t s) {
 template <int NV>
 hipError_t launch_v1_nv(const V1Args &a, hipStream_t s) {
     if (a.coder == 1) {
-        if (a.precision != 8 || a.table_mode != 0) return hipErrorInvalidValue;
+        if (a.precision != 8) return hipErrorInvalidValue;
+        if (a.table_mode == 1)
+            return a.streams == 64 ? launch_quad<NV, 64, 1>(a, s) : hipErrorInvalidValue;
         if (a.streams == 128) return launch_quad<NV, 128>(a, s);
         if (a.streams == 64) return launch_quad<NV, 64>(a, s);
         return launch_quad<NV, 32>(a, s);

@@ -64,7 +64,7 @@ int quantize_counts(const uint64_t *counts, uint32_t n_sym, uint32_t total, uint
 bool valid_params(uint32_t streams, uint32_t precision, uint32_t table_mode, uint32_t coder = 0) {
     return (streams == 32 || streams == 64 || streams == 128) && (precision == 8 || precision == 12) &&
            (table_mode == 0 || (table_mode == 1 && precision == 8)) &&
-           (coder == 0 || (coder == 1 && precision == 8 && table_mode == 0));
+           (coder == 0 || (coder == 1 && precision == 8));
 }
 
 // Quad coder (coder 1): 64-bit state, 32-bit words, four 8-bit quantiles per step (the

@@ -29,7 +29,7 @@ GW = 256 / S consecutive words (8 for S = 32, as in the reference layout; 4 for 
 2 for S = 128),
 so stream l's word j sits at u16 index cursor + 256*(j // GW) + GW*l + j % GW.
 
-Quad coder (coder = 1; P = 8, one table): 64-bit state x, 32-bit words.  A stream's
+Quad coder (coder = 1; P = 8, one table or one per row): 64-bit state x, 32-bit words.  A stream's
 symbols go in quads (decode order); a quad is the four bytes of the low word of x:
   decode: if x < 2^32: x = (x << 32) | next word
           q_i = byte i of x; s_i = ppf[q_i]; F = f0*f1*f2*f3; R = r0 + f0*(r1 + f1*(r2 + f2*r3))
@@ -202,17 +202,18 @@ def encode_stream_quad(symbols, cdf):
     return [x & 0xFFFFFFFF, x >> 32] + bulk[::-1]
 
 
-def encode_rows_quad(matrix, streams, min_value, cdf):
-    """Quad-coder encode of a matrix.  Returns (cursors uint32[rows] in u16 units,
-    payload uint16)."""
+def encode_rows_quad(matrix, streams, min_value, cdf=None, freq=None):
+    """Quad-coder encode of a matrix with one table (``cdf``) or one per row (``freq``).
+    Returns (cursors uint32[rows] in u16 units, payload uint16)."""
     m = np.asarray(matrix, np.int8)
     rows, cols = m.shape
     assert streams in (32, 64, 128) and cols % (16 * streams) == 0
     G4 = 128 // streams
     out, cursors, pos = [], [], 0
     for r in range(rows):
+        c = cdf if freq is None else cdf_of(freq[r])
         sym = m[r].astype(np.int64) - min_value
-        lanes = [encode_stream_quad(sym[stream_columns(cols, streams, l)], cdf)
+        lanes = [encode_stream_quad(sym[stream_columns(cols, streams, l)], c)
                  for l in range(streams)]
         K = (max(len(s) for s in lanes) + G4 - 1) // G4
         row = np.zeros(K * 128, np.uint32)
@@ -225,12 +226,16 @@ def encode_rows_quad(matrix, streams, min_value, cdf):
     return np.asarray(cursors, np.uint32), np.concatenate(out)
 
 
-def decode_rows_quad(rows, cols, streams, cursors, payload, min_value, cdf):
-    """Decode a quad-coder matrix, vectorised over the rows x streams streams."""
+def decode_rows_quad(rows, cols, streams, cursors, payload, min_value, cdf=None, freq=None):
+    """Decode a quad-coder matrix (one table ``cdf`` or one per row ``freq``), vectorised
+    over the rows x streams streams."""
     pay32 = np.asarray(payload, np.uint16).view(np.uint32)
     S, G4 = streams, 128 // streams
-    cdf = np.asarray(cdf, np.int64)
-    ppf = ppf_of(cdf, 8)
+    if freq is None:
+        cdfs = np.tile(np.asarray(cdf, np.int64), (rows, 1))
+    else:
+        cdfs = np.stack([cdf_of(freq[r]) for r in range(rows)])
+    ppfs = np.stack([ppf_of(c, 8) for c in cdfs])
     n = rows * S
     row = np.repeat(np.arange(rows), S)
     lane = np.tile(np.arange(S), rows)
@@ -256,12 +261,12 @@ def decode_rows_quad(rows, cols, streams, cursors, payload, min_value, cdf):
         R = np.zeros(n, np.uint64)
         mul = np.ones(n, np.uint64)
         for i in range(4):
-            s = ppf[q[i]]
+            s = ppfs[row, q[i]]
             k = 4 * j + i
             col = 16 * S * (k // 16) + 16 * lane + k % 16
             out[row, col] = ((min_value + s) & 0xFF).astype(np.uint8).view(np.int8)
-            f = (cdf[s + 1] - cdf[s]).astype(np.uint64)
-            R = R + mul * (q[i] - cdf[s]).astype(np.uint64)
+            f = (cdfs[row, s + 1] - cdfs[row, s]).astype(np.uint64)
+            R = R + mul * (q[i] - cdfs[row, s]).astype(np.uint64)
             mul = mul * f
         x = (x >> np.uint64(32)) * mul + R
     return out
@@ -397,7 +402,7 @@ class RecordV1:
     def decode(self):
         if self.coder == 1:
             return decode_rows_quad(self.rows, self.cols, self.streams, self.cursors,
-                                    self.payload, self.min_value, self.cdf)
+                                    self.payload, self.min_value, self.cdf, self.freq)
         return decode_rows(self.rows, self.cols, self.streams, self.precision, self.cursors,
                            self.payload, self.min_value, self.cdf, self.freq)
 
@@ -405,10 +410,15 @@ class RecordV1:
 def encode_record(matrix, streams=64, precision=8, table_mode=0, coder=0):
     m = np.asarray(matrix, np.int8)
     if coder == 1:
-        assert precision == 8 and table_mode == 0
-        mn, G, cdf = global_model(m, 8)
-        cur, pay = encode_rows_quad(m, streams, mn, cdf)
-        return RecordV1(m.shape[0], m.shape[1], streams, 8, 0, mn, G, cur, pay, cdf=cdf,
+        assert precision == 8
+        if table_mode == 0:
+            mn, G, cdf = global_model(m, 8)
+            cur, pay = encode_rows_quad(m, streams, mn, cdf=cdf)
+            return RecordV1(m.shape[0], m.shape[1], streams, 8, 0, mn, G, cur, pay, cdf=cdf,
+                            coder=1)
+        mn, G, freq = per_row_model(m)
+        cur, pay = encode_rows_quad(m, streams, mn, freq=freq)
+        return RecordV1(m.shape[0], m.shape[1], streams, 8, 1, mn, G, cur, pay, freq=freq,
                         coder=1)
     if table_mode == 0:
         mn, G, cdf = global_model(m, precision)

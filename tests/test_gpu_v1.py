@@ -16,7 +16,8 @@ from .helpers import gaussian_int8, gaussian_vec, laplace_int8, uniform_vec
 pytestmark = pytest.mark.gpu
 
 VARIANTS = [(32, 8, 0, 0), (64, 8, 0, 0), (128, 8, 0, 0), (32, 12, 0, 0), (64, 12, 0, 0),
-            (128, 12, 0, 0), (64, 8, 1, 0), (32, 8, 0, 1), (64, 8, 0, 1), (128, 8, 0, 1)]
+            (128, 12, 0, 0), (64, 8, 1, 0), (32, 8, 0, 1), (64, 8, 0, 1), (128, 8, 0, 1),
+            (64, 8, 1, 1)]
 
 
 def note(msg):

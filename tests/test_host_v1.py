@@ -37,15 +37,19 @@ def test_encoder_matches_oracle(S, P, mode):
         assert np.array_equal(r.decode(), m)
 
 
-@pytest.mark.parametrize("S", [32, 64, 128])
-def test_quad_encoder_matches_oracle(S):
+@pytest.mark.parametrize("S,mode", [(32, 0), (64, 0), (128, 0), (64, 1)])
+def test_quad_encoder_matches_oracle(S, mode):
     for m in _mats():
         if m.shape[1] % (16 * S):
             continue
-        c = CompressedMatrixV1.encode(m, S, 8, 0, n_threads=3, coder=1)
-        r = E.encode_record(m, S, 8, 0, coder=1)
+        c = CompressedMatrixV1.encode(m, S, 8, mode, n_threads=3, coder=1)
+        r = E.encode_record(m, S, 8, mode, coder=1)
         assert np.array_equal(c.cursors, r.cursors) and np.array_equal(c.payload, r.payload)
-        assert np.array_equal(c.cdf, r.cdf) and np.array_equal(r.decode(), m)
+        if mode == 0:
+            assert np.array_equal(c.cdf, r.cdf)
+        else:
+            assert np.array_equal(c.freq, r.freq)
+        assert np.array_equal(r.decode(), m)
 
 
 def test_reference_special_case_matches_v0_library():
@@ -59,12 +63,13 @@ def test_reference_special_case_matches_v0_library():
 def test_container_round_trip(tmp_path):
     ms = _mats()
     cs = [CompressedMatrixV1.encode(ms[0], 64, 12, 0), CompressedMatrixV1.encode(ms[2], 64, 8, 1),
-          CompressedMatrixV1.encode(ms[1], 32, 8, 0), CompressedMatrixV1.encode(ms[1], 64, coder=1)]
+          CompressedMatrixV1.encode(ms[1], 32, 8, 0), CompressedMatrixV1.encode(ms[1], 64, coder=1),
+          CompressedMatrixV1.encode(ms[2], 64, 8, 1, coder=1)]
     vec = np.arange(7, dtype=np.int8)
     p = tmp_path / "a.mvx"
     write_mvx1(p, cs, vec, 77)
     hdr, v, back = read_mvx1(p)
-    assert hdr == {"num_matrices": 4, "result_hash": 77} and np.array_equal(v, vec)
+    assert hdr == {"num_matrices": 5, "result_hash": 77} and np.array_equal(v, vec)
     ohdr, ov, orecs = E.read_mvx1(p)
     for c, b, o in zip(cs, back, orecs):
         assert np.array_equal(c.payload, b.payload) and np.array_equal(c.cursors, b.cursors)

@@ -97,15 +97,35 @@ def test_textbook_decode_of_single_streams(S, P, mode):
             assert syms == want.tolist()
 
 
-@pytest.mark.parametrize("S", [32, 64, 128])
+@pytest.mark.parametrize("S,mode", [(32, 0), (64, 0), (128, 0), (64, 1)])
 @pytest.mark.parametrize("name", list(CASES))
-def test_quad_round_trip(S, name):
+def test_quad_round_trip(S, mode, name):
     m = CASES[name]
     if m.shape[1] % (16 * S):
         pytest.skip("cols not a multiple of 16 * streams")
-    r = E.encode_record(m, S, 8, 0, coder=1)
+    r = E.encode_record(m, S, 8, mode, coder=1)
     assert r.payload.size % 256 == 0 and np.all(r.cursors % 256 == 0)
     assert np.array_equal(r.decode(), m)
+
+
+def test_quad_per_row_tables_textbook():
+    """Quad coder with one table per row: each stream decodes with plain 8-bit pops over
+    its own row's table (and not with another row's)."""
+    m = CASES["laplace4"][:, :2048]
+    r = E.encode_record(m, 64, 8, 1, coder=1)
+    p32 = r.payload.view(np.uint32)
+    for row in range(m.shape[0]):
+        end = (r.cursors[row + 1] if row + 1 < m.shape[0] else len(r.payload)) // 2
+        words, j = [], 0
+        while True:
+            p = int(r.cursors[row]) // 2 + 128 * (j // 2) + 2 * 5 + j % 2
+            if p >= end:
+                break
+            words.append(int(p32[p]))
+            j += 1
+        syms = E.decode_stream_quad_textbook(words, m.shape[1] // 64 // 4, E.cdf_of(r.freq[row]))
+        want = m[row, E.stream_columns(m.shape[1], 64, 5)].astype(np.int64) - r.min_value
+        assert syms == want.tolist()
 
 
 @pytest.mark.parametrize("S", [32, 64, 128])
