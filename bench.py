@@ -43,9 +43,10 @@ def parse():
                     help="run one kernel only (for profiling); prints its timing")
     ap.add_argument("--formats", default=",".join(DEFAULT_FORMATS),
                     help="comma list of encodings to time; the fastest is the headline")
-    ap.add_argument("--batch", default="block,uneven",
-                    help="batched workloads to time beside the headline: block, uneven "
-                         "(comma list, or 'none')")
+    ap.add_argument("--batch", default="block,uneven,shapes,small",
+                    help="workloads to time beside the headline: block, uneven, shapes "
+                         "(4096x4096, 14336x4096 and the backward at both LLM shapes), small "
+                         "(the 1024x1024 plain case) (comma list, or 'none')")
     ap.add_argument("--batch-scale", type=float, default=1.0,
                     help="scale every batched shape by this factor (1 = the full sizes)")
     return ap.parse_args()
@@ -379,6 +380,128 @@ def bench_uneven(dev, steps, warmup, scale=1.0, n_mat=32):
             "rows_layout_matrices": len(rows_only)}
 
 
+def bench_shapes(dev, steps, warmup, nvecs=(1, 4, 8)):
+    """The other LLM shapes (4096x4096, 14336x4096; Laplace(0,4)): q32 and .bin v0 fused
+    decode-matvec against the int8 and fp32 comparators, one vector; and the backward
+    dx = W^T (s_w * g) at 14336x4096 and 4096x14336 (sigma=6, g ~ N(0,1)) for 1, 4 and 8
+    gradient vectors, one-matrix launches, q32."""
+    import torch
+    import matvans_amd as mv
+    from matvans_amd.mvx import CompressedMatrixV1
+    out = {}
+    for rows, cols in [(4096, 4096), (14336, 4096)]:
+        w, x = gen(rows, cols, "laplace4")
+        d_x = torch.from_numpy(x[None]).to(dev)
+        want = exact_product(w, x[None])
+        r = {}
+        for fmt in ("q32", "v0"):
+            if fmt == "q32":
+                c = CompressedMatrixV1.encode(w, 32, 8, 0, coder=1)
+                stored, make = c.stored_bytes, _v1_copy
+            else:
+                c = mv.CompressedMatrix.encode(w)
+                stored, make = c.payload_size * 2 + 4 * rows + c.G + 1 + 256, _v0_copy
+            n = min(64, max(2, -(-2 * MALL_BYTES // stored)))
+            cps = [make(c).to_device(dev) for _ in range(n)]
+            o = torch.empty((1, rows), dtype=torch.int32, device=dev)
+            if not np.array_equal(cps[0].multiply(d_x, out=o).cpu().numpy(), want):
+                print(f"{rows}x{cols} {fmt}: result differs", file=sys.stderr)
+                sys.exit(1)
+            ms = time_launches(lambda i: cps[i].multiply(d_x, out=o), n, steps, warmup)
+            r[f"{fmt}_us"] = round(ms * 1e3, 3)
+            r[f"{fmt}_bits_per_entry"] = round(8 * stored / (rows * cols), 4)
+            del cps
+        w8 = [torch.from_numpy(w).to(dev) for _ in range(max(2, -(-2 * MALL_BYTES // w.nbytes)))]
+        o = torch.empty((1, rows), dtype=torch.int32, device=dev)
+        r["int8_us"] = round(1e3 * time_launches(lambda i: mv.int8_matvec(w8[i], d_x, out=o),
+                                                 len(w8), steps, warmup), 3)
+        del w8
+        wf = torch.from_numpy(w.astype(np.float32)).to(dev)
+        wfs = [wf] + [wf.clone() for _ in range(max(1, -(-2 * MALL_BYTES // (w.size * 4)) - 1))]
+        xf = torch.from_numpy(x.astype(np.float32)).to(dev)
+        of = torch.empty(rows, dtype=torch.float32, device=dev)
+        r["f32_us"] = round(1e3 * time_launches(lambda i: mv.f32_gemv(wfs[i], xf, out=of),
+                                                len(wfs), steps, warmup), 3)
+        del wfs, wf
+        r["speedup_q32_vs_int8"] = round(r["int8_us"] / r["q32_us"], 3)
+        out[f"{rows}x{cols}"] = r
+    rng = np.random.default_rng(20250313)
+    bwd = {}
+    for rows, cols in [(14336, 4096), (4096, 14336)]:
+        w = sample_int8(rng, (rows, cols), 6.0)
+        c = CompressedMatrixV1.encode(w, 32, 8, 0, coder=1)
+        n = min(16, max(2, -(-2 * MALL_BYTES // c.stored_bytes)))
+        cps = [_v1_copy(c).to_device(dev) for _ in range(n)]
+        s_w = torch.full((rows,), 0.01, dtype=torch.float32, device=dev)
+        r = {}
+        for nv in nvecs:
+            g = torch.from_numpy(rng.standard_normal((nv, rows)).astype(np.float32)).to(dev)
+            alpha = torch.ones(nv, dtype=torch.float32, device=dev)
+            ws = torch.empty(int(mv.lib().matvans_backward_workspace_size(rows, cols, nv)) // 4,
+                             dtype=torch.float32, device=dev)
+            dx = torch.empty((nv, cols), dtype=torch.float32, device=dev)
+            msb = time_launches(lambda i: cps[i].backward(g, s_w, alpha, out=dx, workspace=ws),
+                                n, steps, warmup)
+            d_v = torch.from_numpy(act_vectors(rng, nv, cols)).to(dev)
+            o = torch.empty((nv, rows), dtype=torch.int32, device=dev)
+            msf = time_launches(lambda i: cps[i].multiply(d_v, out=o), n, steps, warmup)
+            r[f"n_vec={nv}"] = {"fwd_us": round(msf * 1e3, 3), "bwd_us": round(msb * 1e3, 3),
+                                "fwd_plus_bwd_over_fwd": round((msf + msb) / msf, 3)}
+            del ws
+        bwd[f"{rows}x{cols}"] = r
+        del cps
+    out["backward_q32"] = bwd
+    return out
+
+
+def bench_small(dev, steps, warmup):
+    """The reference's plain CPU case: one 1024x1024 int8 matrix, entries
+    clip(rint(N(0, 6^2))), per-row scale s_w = rowmax|W_float| / 127, x uniform int8,
+    .bin v0 encoding: one matvec with int32 and fp32-scaled outputs on the GPU (decode-
+    matvec + scale, two launches) and the C oracle decode+matvec on one host core."""
+    import torch
+    import matvans_amd as mv
+    from oracle import oracle as O
+    rng = np.random.default_rng(20250310)
+    wf = rng.normal(0, 6.0, size=(1024, 1024))
+    w = np.clip(np.rint(wf), -127, 127).astype(np.int8)
+    s_w = (np.abs(wf).max(axis=1) / 127).astype(np.float32)
+    x = rng.integers(-127, 128, size=1024).astype(np.int8)
+    c = mv.CompressedMatrix.encode(w)
+    cps = [_v0_copy(c).to_device(dev) for _ in range(2)]
+    d_x = torch.from_numpy(x[None]).to(dev)
+    d_sw = torch.from_numpy(s_w).to(dev)
+    s_x = torch.ones(1, dtype=torch.float32, device=dev)
+    acc = torch.empty((1, 1024), dtype=torch.int32, device=dev)
+    y = torch.empty((1, 1024), dtype=torch.float32, device=dev)
+    cps[0].multiply(d_x, out=acc)
+    mv.scale_output(acc, d_sw, s_x, out=y)
+    want = exact_product(w, x[None])
+    yref = s_w.astype(np.float64) * want[0].astype(np.float64)
+    if not np.array_equal(acc.cpu().numpy(), want) or \
+            np.max(np.abs(y.cpu().numpy()[0] - yref) / np.maximum(np.abs(yref), 1e-30)) > 1e-6:
+        print("1024x1024: result differs", file=sys.stderr)
+        sys.exit(1)
+
+    def step(i):
+        cps[i].multiply(d_x, out=acc)
+        mv.scale_output(acc, d_sw, s_x, out=y)
+    ms = time_launches(step, 2, steps, warmup)
+    rec = O.Record(c.rows, c.cols, 1.0, c.cursors, c.payload_size, c.min_value, c.G, c.cdf_data,
+                   c.ppf_data, c.payload)
+    t0 = time.perf_counter()
+    reps = 20
+    for _ in range(reps):
+        O.c_matvec(rec, x, 1)
+    cpu_us = (time.perf_counter() - t0) / reps * 1e6
+    return {"gpu_us": round(ms * 1e3, 3),
+            "cpu_baseline": {"value": round(cpu_us, 1), "unit": "us/matvec", "cores": 1,
+                             "kind": "port", "sample": f"full 1024x1024 matrix, {reps} repetitions"},
+            "bits_per_entry": round(8 * (c.payload_size * 2) / (1024 * 1024), 4),
+            "note": "GPU time covers the decode-matvec and the fp32 scaled output (launch-bound "
+                    "at this size)"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -551,6 +674,12 @@ def main():
         torch.cuda.empty_cache()
     if "uneven" in batch:
         line["uneven"] = bench_uneven(dev, max(1, bsteps // 5), bwarm, args.batch_scale)
+        torch.cuda.empty_cache()
+    if "shapes" in batch:
+        line["shapes"] = bench_shapes(dev, bsteps, bwarm)
+        torch.cuda.empty_cache()
+    if "small" in batch:
+        line["small"] = bench_small(dev, bsteps, bwarm)
     print(json.dumps(line))
 
 

@@ -77,11 +77,20 @@ class Batch:
         call("matvans_batch_create", arr, len(entries), ctypes.byref(h))
         self._h = h
 
-    def __del__(self):
+    def close(self):
+        """Release the plan's device memory (also done when the object is collected)."""
         h = getattr(self, "_h", None)
         if h is not None and h.value:
             lib().matvans_batch_destroy(h)
-            self._h = None
+        self._h = None
+
+    def __del__(self):
+        # at interpreter exit the module globals may already be gone; the process exit
+        # releases the device memory then
+        try:
+            self.close()
+        except (TypeError, AttributeError):
+            pass
 
     def configure(self, units_per_tile=1, order=0, blocks_per_cu=8, schedule=0):
         """schedule 0: blocks take tiles from the device work queue; 1: block b takes
