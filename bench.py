@@ -11,7 +11,10 @@ rotate through enough distinct device copies (> 256 MB) that the MALL cannot ser
 repeat.  Inputs are resident in HBM before timing starts.  The same-GPU comparators
 (uncompressed int8 sdot4 matvec, fp32 GEMV) are timed the same way, the int32 result is
 checked bit-exact before timing, and the CPU baseline (the C oracle decode+matvec on the
-host cores) is timed on a bounded sample.  Prints one JSON line.
+host cores) is timed on a bounded sample.  Beside the headline (--batch): the decoder
+block in one persistent launch per encoding and vector count, with its batched backward;
+the 32-matrix uneven family-B batch; the other LLM shapes and the backward at both of
+them; the 1024 x 1024 plain case.  Prints one JSON line.
 """
 from __future__ import annotations
 
