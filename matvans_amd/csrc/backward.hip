@@ -23,13 +23,13 @@ using blk::kRedParts;
 
 template <int NV>
 __global__ __launch_bounds__(512) void v0_bwd_partial(BwdJob job) {
-    __shared__ blk::V0BwdSmem sm;
+    __shared__ blk::V0BwdSmem<NV> sm;
     blk::v0_bwd_unit<NV>(job, blockIdx.x, blockIdx.y, gridDim.x, sm);
 }
 
 template <int NV, int S>
 __global__ __launch_bounds__(512) void quad_bwd_partial(BwdJob job) {
-    __shared__ blk::QuadBwdSmem<S> sm;
+    __shared__ blk::QuadBwdSmem<NV, S> sm;
     blk::quad_bwd_unit<NV, S>(job, blockIdx.x, blockIdx.y, gridDim.x, sm);
 }
 

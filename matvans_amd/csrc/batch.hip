@@ -156,11 +156,11 @@ struct NoSmem {};
 template <bool On, class T>
 using Opt = typename std::conditional<On, T, NoSmem>::type;
 
-template <uint32_t MASK>
+template <int NV, uint32_t MASK>
 union BwdSmem {
-    Opt<(MASK & bit(kBwdV0)) != 0, blk::V0BwdSmem> v0;
-    Opt<(MASK & bit(kBwdQ32)) != 0, blk::QuadBwdSmem<32>> q32;
-    Opt<(MASK & bit(kBwdQ64)) != 0, blk::QuadBwdSmem<64>> q64;
+    Opt<(MASK & bit(kBwdV0)) != 0, blk::V0BwdSmem<NV>> v0;
+    Opt<(MASK & bit(kBwdQ32)) != 0, blk::QuadBwdSmem<NV, 32>> q32;
+    Opt<(MASK & bit(kBwdQ64)) != 0, blk::QuadBwdSmem<NV, 64>> q64;
 };
 
 // Same work queue as batch_fwd (sharded counters, static first tile, last-ticket reset);
@@ -169,7 +169,7 @@ template <int NV, uint32_t MASK>
 __global__ __launch_bounds__(512) void batch_bwd(const BwdItem *__restrict__ items,
                                                  const Tile *__restrict__ tiles, uint32_t n_tiles,
                                                  uint32_t *counters, uint32_t queue) {
-    __shared__ BwdSmem<MASK> sm;
+    __shared__ BwdSmem<NV, MASK> sm;
     __shared__ uint32_t s_pos;
     const uint32_t tid = threadIdx.x;
     const uint32_t g = blockIdx.x % kShards;

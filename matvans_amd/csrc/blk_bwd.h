@@ -13,6 +13,9 @@
 // in a fixed order and applies alpha.  Both stages are deterministic bit for bit, so one-matrix and batched launches
 // give identical results.
 //
+// With two or more gradient vectors the group sums run on the matrix cores instead
+// (bwd_group_mfma), one MFMA sequence for all vectors.
+//
 // .bin v0 streams (create_matrices_WarpPerRow.py:361-366): stream l covers columns
 // 512*(k/16) + 16*l + k%16.  Quad-coder streams (S = 32 or 64): stream l's group t covers
 // columns 16*S*t + 16*l .. +15 and a quad step yields four consecutive ones.
@@ -63,17 +66,24 @@ struct BwdJob {
     const uint32_t *glut;        // quad: A / B decode tables (2 x 256 dwords)
 };
 
+// From this many gradient vectors on, a group's sums are computed on the matrix cores
+// (bwd_group_mfma) instead of one butterfly per vector.
+constexpr int kMfmaNV = 2;
+
+template <int NV>
 struct V0BwdSmem {
     uint16_t ring[kBwdWaves * kRingWords * 64];
     uint32_t lut[256];
     uint16_t dummy[kDummyBytes / 2];
+    uint32_t wstage[NV >= kMfmaNV ? kBwdWaves * 256 : 1];   // per wave: 64 x 16 int8
 };
 
-template <int S>
+template <int NV, int S>
 struct QuadBwdSmem {
     uint32_t ring[kBwdWaves * v1::QuadLane<128 / S>::R4 * 64];
     uint32_t tab[256];
     uint32_t dummy[(128 / S) * 64];
+    uint32_t wstage[NV >= kMfmaNV ? kBwdWaves * 256 : 1];
 };
 
 // Per-lane setup shared by both codecs: the 64-row window, this lane's row extent and
@@ -160,10 +170,51 @@ struct BwdLane {
     }
 };
 
+// Matrix-core path for several gradient vectors.  A group's 16 columns of the 64 rows
+// form a 64 x 16 int8 block W_g (lane r holds row r: four packed quads).  The group sums
+// out[c][v] = sum_r W_g[r][c] gs[r][v] are W_g^T gs, a 16 x 64 by 64 x 16 product:
+// sixteen v_mfma_f32_16x16x4f32 steps over slabs of four rows.  In slab s, lane l supplies
+// A[l%16][l/16] = W_g[4s + l/16][l%16] (read back from the wave's LDS stage) and
+// B[l/16][l%16] = gs[4s + l/16][v = l%16] (MfmaGrad); the result D[i][j] = out[i][j] is
+// held by lane 16 (i/4) + j, register i%4, so each lane stores four consecutive columns of
+// one vector.  The hardware's summation order is fixed: deterministic like the butterfly.
+struct MfmaGrad {
+    float b[16];   // slab s: s_w[r] g[l%16][r], r = 4s + l/16 (0 past the rows / n_vec)
+    __device__ __forceinline__ void init(const BwdJob &J, uint32_t rb, uint32_t lane) {
+        const uint32_t v = lane & 15;
+#pragma unroll
+        for (int s = 0; s < 16; s++) {
+            const uint32_t r = rb * 64u + 4u * s + (lane >> 4);
+            b[s] = r < J.rows && v < J.n_vec ? J.s_w[r] * J.g[size_t(v) * J.g_stride + r] : 0.0f;
+        }
+    }
+};
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void bwd_group_mfma(const uint32_t (&P)[4], const float (&b)[16],
+                                               uint32_t lane, uint32_t *wst, float *out,
+                                               size_t plane, uint32_t n_vec) {
+    // stage row r = lane at bytes 16r .. 16r + 15 (the wave's LDS accesses run in order, so
+    // the reads below see this write and the next group's write follows these reads)
+    *reinterpret_cast<uint4 *>(wst + 4 * lane) = make_uint4(P[0], P[1], P[2], P[3]);
+    const int8_t *w8 = reinterpret_cast<const int8_t *>(wst);
+    const uint32_t i = lane & 15, k = lane >> 4;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        const float a = static_cast<float>(w8[(4 * s + k) * 16 + i]);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[s], acc, 0, 0, 0);
+    }
+    if (i < n_vec)   // lane l: columns 4 (l/16) .. +3 of vector l%16
+        *reinterpret_cast<float4 *>(out + size_t(i) * plane + 4 * k) =
+            make_float4(acc[0], acc[1], acc[2], acc[3]);
+}
+
 // .bin v0 unit: row block rb, streams 8 sg .. 8 sg + 7.
 template <int NV>
 __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32_t sg,
-                                            uint32_t nrb, V0BwdSmem &sm) {
+                                            uint32_t nrb, V0BwdSmem<NV> &sm) {
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid >> 6, lane = tid & 63;
     if (tid < 256) {   // packed table f | bias << 8 | sym << 24
@@ -186,24 +237,34 @@ __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32
     const uint32_t groups = cols / 512u;
     const size_t plane = size_t(nrb) * cols;
     float *base = J.partial + size_t(rb) * cols + stream * 16u;
+    // four symbols (byte 3 of each table entry) packed in column order
+    auto next4 = [&]() {
+        uint32_t e[4];
+        st.pair_packed(sm.lut, e[0], e[1]);
+        st.pair_packed(sm.lut, e[2], e[3]);
+        return __builtin_amdgcn_perm(e[1], e[0], 0x0c0c0703u) |
+               __builtin_amdgcn_perm(e[3], e[2], 0x07030c0cu);
+    };
     BwdLane<NV> L;
-    L.init(R.gs, lane);
+    MfmaGrad M;
+    if constexpr (NV >= kMfmaNV) M.init(J, rb, lane);
+    else L.init(R.gs, lane);
     auto group = [&](uint32_t t) {
         uint32_t wbefore = st.waddr;
-        float outv[NV];
+        if constexpr (NV >= kMfmaNV) {
+            uint32_t P[4];
 #pragma unroll
-        for (int v = 0; v < NV; v++) outv[v] = 0.0f;
+            for (int d = 0; d < 4; d++) P[d] = next4();
+            bwd_group_mfma(P, M.b, lane, sm.wstage + wave * 256, base + t * 512u, plane,
+                           J.n_vec);
+        } else {
+            float outv[NV];
 #pragma unroll
-        for (int d = 0; d < 4; d++) {
-            uint32_t e[4];
-            st.pair_packed(sm.lut, e[0], e[1]);
-            st.pair_packed(sm.lut, e[2], e[3]);
-            // symbols (byte 3 of each entry) packed in column order
-            const uint32_t packed = __builtin_amdgcn_perm(e[1], e[0], 0x0c0c0703u) |
-                                    __builtin_amdgcn_perm(e[3], e[2], 0x07030c0cu);
-            bwd_quad<NV>(packed, L.gsA, L.gsB, L.sh, lane, d, outv);
+            for (int v = 0; v < NV; v++) outv[v] = 0.0f;
+#pragma unroll
+            for (int d = 0; d < 4; d++) bwd_quad<NV>(next4(), L.gsA, L.gsB, L.sh, lane, d, outv);
+            L.store(outv, base + t * 512u, plane, J.n_vec);
         }
-        L.store(outv, base + t * 512u, plane, J.n_vec);
         st.end_group(wbefore);
     };
     uint32_t t = 0;
@@ -222,7 +283,7 @@ __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32
 // Quad-coder unit (S = 32 or 64 streams): row block rb, streams 8 sg .. 8 sg + 7.
 template <int NV, int S>
 __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint32_t sg,
-                                              uint32_t nrb, QuadBwdSmem<S> &sm) {
+                                              uint32_t nrb, QuadBwdSmem<NV, S> &sm) {
     using QL = v1::QuadLane<128 / S>;
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid >> 6, lane = tid & 63;
@@ -246,16 +307,26 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
     const size_t plane = size_t(nrb) * cols;
     float *base = J.partial + size_t(rb) * cols + stream * 16u;
     BwdLane<NV> L;
-    L.init(R.gs, lane);
+    MfmaGrad M;
+    if constexpr (NV >= kMfmaNV) M.init(J, rb, lane);
+    else L.init(R.gs, lane);
     auto group = [&](uint32_t t) {
         uint32_t wbefore = st.waddr;
-        float outv[NV];
+        if constexpr (NV >= kMfmaNV) {
+            uint32_t P[4];
 #pragma unroll
-        for (int v = 0; v < NV; v++) outv[v] = 0.0f;
+            for (int d = 0; d < 4; d++) P[d] = st.quad(sm.tab);
+            bwd_group_mfma(P, M.b, lane, sm.wstage + wave * 256, base + t * (16u * S), plane,
+                           J.n_vec);
+        } else {
+            float outv[NV];
 #pragma unroll
-        for (int d = 0; d < 4; d++)
-            bwd_quad<NV>(st.quad(sm.tab), L.gsA, L.gsB, L.sh, lane, d, outv);
-        L.store(outv, base + t * (16u * S), plane, J.n_vec);
+            for (int v = 0; v < NV; v++) outv[v] = 0.0f;
+#pragma unroll
+            for (int d = 0; d < 4; d++)
+                bwd_quad<NV>(st.quad(sm.tab), L.gsA, L.gsB, L.sh, lane, d, outv);
+            L.store(outv, base + t * (16u * S), plane, J.n_vec);
+        }
         st.end_group(wbefore);
     };
     uint32_t t = 0;
