@@ -30,7 +30,7 @@ typedef enum {
     MATVANS_E_IO = 5,          /* file cannot be opened / written */
     MATVANS_E_HIP = 6,         /* HIP runtime error (launch, ...) */
     MATVANS_E_RANGE = 7,       /* symbol outside the entropy model */
-    MATVANS_E_UNSUPPORTED = 8, /* valid request this build does not implement */
+    MATVANS_E_UNSUPPORTED = 8, /* valid request the library does not implement */
     MATVANS_E_CAPACITY = 9     /* caller buffer too small */
 } matvans_status;
 
