@@ -11,7 +11,7 @@ from .buffered import BufferedMatrices
 from .capi import BatchBwdItem, BatchConfig, BatchItem, call, lib
 from .mvx import CompressedMatrixV1
 
-__all__ = ["Batch", "BatchBackward"]
+__all__ = ["Batch", "BatchBackward", "tune"]
 
 V0, QUAD, B_ROWS, B_STRIPES = 0, 1, 2, 3
 
@@ -160,3 +160,42 @@ class BatchBackward(Batch):
         h = ctypes.c_void_p()
         call("matvans_batch_create_backward", arr, len(entries), ctypes.byref(h))
         self._h = h
+
+
+def tune(matrices, vecs, encodings=("q32", "q64", "v0"), reps=5, stream=None):
+    """Pick the encoding (streams per row, and with it rows per wavefront: two rows per
+    wave at 32 streams, one at 64) and the plan configuration (tile size, tile order,
+    persistent grid, schedule) for a batch of raw int8 matrices.  matrices: int8 numpy
+    arrays [rows, cols]; vecs: int8 device tensors [n_vec, cols], one per matrix.  Each
+    candidate encoding is encoded on the host, uploaded, and its plan autotuned; the
+    fastest plan is returned with a report {encoding: ms per launch}.  The other
+    candidates' device memory is released."""
+    from . import CompressedMatrix
+    import torch
+    report, best = {}, None
+    dev = vecs[0].device
+    for enc in encodings:
+        if enc == "v0":
+            mats = [CompressedMatrix.encode(m).to_device(dev) for m in matrices]
+        elif enc in ("q32", "q64"):
+            S = 32 if enc == "q32" else 64
+            if any(m.shape[1] % (16 * S) for m in matrices):
+                continue
+            mats = [CompressedMatrixV1.encode(m, S, 8, 0, coder=1).to_device(dev)
+                    for m in matrices]
+        else:
+            raise ValueError(f"unknown encoding {enc}")
+        b = Batch(list(zip(mats, vecs)))
+        _, ms = b.autotune(reps=reps, stream=stream)
+        report[enc] = ms
+        if best is None or ms < best[1]:
+            if best is not None:
+                best[0].close()
+            best = (b, ms, enc)
+        else:
+            b.close()
+        torch.cuda.synchronize()
+    if best is None:
+        raise ValueError("no candidate encoding fits the matrices' shapes")
+    best[0].encoding = best[2]
+    return best[0], report

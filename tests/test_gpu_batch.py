@@ -228,3 +228,20 @@ def test_backward_batch_mixed_and_graph(cuda):
         assert float(torch.ones(100, device="cuda").sum()) == 100.0   # eager work in between
         for o, want in zip(b.outputs, singles):
             assert o.cpu().numpy().tobytes() == want.tobytes(), rep
+
+
+def test_tune_picks_an_exact_plan(cuda):
+    """The streams-per-row / plan tuner returns a plan whose outputs are exact."""
+    import torch
+    from matvans_amd.batch import tune
+    note("batch tune over encodings")
+    mats = [gaussian_int8(r, c, 6.0, seed=7 + i)[0] for i, (r, c) in enumerate(BLOCK[:4])]
+    vs = [vecs_for(m.shape[1], 2, 40 + i) for i, m in enumerate(mats)]
+    dvs = [torch.from_numpy(v).cuda() for v in vs]
+    b, report = tune(mats, dvs, reps=2)
+    assert b.encoding in report and set(report) <= {"q32", "q64", "v0"}
+    assert report[b.encoding] == min(report.values())
+    b.launch()
+    torch.cuda.synchronize()
+    for m, v, o in zip(mats, vs, b.outputs):
+        assert np.array_equal(o.cpu().numpy(), want(m, v))
