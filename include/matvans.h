@@ -297,6 +297,17 @@ typedef struct {
     const uint16_t *payload;    /* 16-byte aligned */
 } matvans_v1_dev_matrix;
 
+/* Upload (matvansWarpPerRow.cu:287-313, the per-matrix cudaMalloc + cudaMemcpy of
+ * cursors, cdf, ppf and payload): device copies of a host matrix owned by the returned
+ * handle, and the device view to pass to the decode entry points.  The payload gets 1 KiB
+ * of zero slack.  Synchronous; call matvans_upload_free to release the device memory. */
+typedef struct matvans_upload matvans_upload;
+int matvans_matrix_upload(const matvans_host_matrix *m, matvans_upload **out,
+                          matvans_dev_matrix *view);
+int matvans_v1_matrix_upload(const matvans_v1_host *m, matvans_upload **out,
+                             matvans_v1_dev_matrix *view);
+void matvans_upload_free(matvans_upload *u);
+
 /* dst[v][r] = sum_c W[r,c] * vecs[v][c] for v < n_vec (1..8), W decoded from an MVX1
  * matrix (int32, bit-exact).  vecs: n_vec vectors vec_stride bytes apart, 16-byte
  * aligned.  The fused decode-matvec of A1 for the extended formats. */

@@ -120,6 +120,11 @@ SIGNATURES = {
     "matvans_v1_matrix": (i32, [P, u32, ctypes.POINTER(V1Host)]),
     "matvans_v1_write": (i32, [ctypes.c_char_p, u32, P, u32, u32, ctypes.POINTER(V1Host)]),
     "matvans_v1_decompress_and_multiply": (i32, [ctypes.POINTER(V1DevMatrix), P, P, u32, u32, P]),
+    "matvans_matrix_upload": (i32, [ctypes.POINTER(HostMatrix), ctypes.POINTER(P),
+                                    ctypes.POINTER(DevMatrix)]),
+    "matvans_v1_matrix_upload": (i32, [ctypes.POINTER(V1Host), ctypes.POINTER(P),
+                                       ctypes.POINTER(V1DevMatrix)]),
+    "matvans_upload_free": (None, [P]),
     "matvans_v1_decompress_backward": (i32, [ctypes.POINTER(V1DevMatrix), P, P, u32, u32, P, P,
                                              P, ctypes.c_size_t, P]),
     "matvans_b_rows_matvec": (i32, [P, u64, P, u32, P, P, P, P, u32, u32, u32, u32, f32, P, P, P]),

@@ -13,11 +13,12 @@
 // matched, 1 otherwise, 2 on an error.
 //
 // Differences from the reference harness, on purpose: device buffers are allocated once
-// (the reference allocates and frees them in every iteration, :287-313,377-383); "with
-// upload" re-copies the payloads host -> device inside the timed region instead; the
-// chain is stream-ordered with one synchronisation at its end (the reference copies each
-// intermediate vector to the host, :344); throughput counts rows x cols MACs per matrix
-// (the reference's rows * rows assumes square matrices, :386).
+// with the library's upload handles (matvans_matrix_upload; the reference allocates and
+// frees them in every iteration, :287-313,377-383); "with upload" re-copies the payloads
+// host -> device inside the timed region instead; the chain is stream-ordered with one
+// synchronisation at its end (the reference copies each intermediate vector to the host,
+// :344); throughput counts rows x cols MACs per matrix (the reference's rows * rows
+// assumes square matrices, :386).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -62,16 +63,16 @@ struct Arena {
 };
 
 template <class T>
-T *upload(Arena &a, const T *host, size_t count, size_t slack = 0) {
-    T *d = a.alloc<T>(count + slack);
-    if (slack) hip_check(hipMemset(d, 0, (count + slack) * sizeof(T)), "hipMemset");
+T *upload(Arena &a, const T *host, size_t count) {
+    T *d = a.alloc<T>(count);
     if (count) hip_check(hipMemcpy(d, host, count * sizeof(T), hipMemcpyHostToDevice), "upload");
     return d;
 }
 
-// One matrix of the chain, either container.
+// One matrix of the chain, either container: its library-owned device copy and view.
 struct Step {
     bool v1 = false;
+    matvans_upload *up = nullptr;
     matvans_dev_matrix m0{};
     matvans_v1_dev_matrix m1{};
     const uint16_t *host_payload = nullptr;
@@ -79,8 +80,6 @@ struct Step {
     uint32_t payload_words = 0;
     uint32_t rows = 0, cols = 0;
 };
-
-constexpr size_t kSlackWords = 512;   // zero words after each payload
 
 bool is_mvx1(const char *path) {
     FILE *f = std::fopen(path, "rb");
@@ -143,26 +142,8 @@ int main(int argc, char **argv) {
             s.v1 = true;
             s.rows = h.rows;
             s.cols = h.cols;
-            s.m1.rows = h.rows;
-            s.m1.cols = h.cols;
-            s.m1.streams = h.streams;
-            s.m1.precision = h.precision;
-            s.m1.table_mode = h.table_mode;
-            s.m1.coder = h.coder;
-            s.m1.min_value = h.min_value;
-            s.m1.G = h.G;
-            s.m1.payload_size = h.payload_size;
-            s.m1.cursors = upload(arena, h.cursors, h.rows);
-            if (h.table_mode == 0) {
-                std::vector<uint32_t> lut(size_t(2) << h.precision);
-                mv_check(matvans_v1_lut(h.cdf, h.G, h.min_value, h.precision, lut.data()),
-                         "decode table");
-                s.m1.lut = upload(arena, lut.data(), lut.size());
-            } else {
-                s.m1.freq = upload(arena, h.freq, size_t(h.rows) * h.G);
-            }
-            s.dev_payload = upload(arena, h.payload, h.payload_size, kSlackWords);
-            s.m1.payload = s.dev_payload;
+            mv_check(matvans_v1_matrix_upload(&h, &s.up, &s.m1), "upload");
+            s.dev_payload = const_cast<uint16_t *>(s.m1.payload);
             s.host_payload = h.payload;
             s.payload_words = h.payload_size;
             steps.push_back(s);
@@ -178,13 +159,8 @@ int main(int argc, char **argv) {
             Step s;
             s.rows = h.rows;
             s.cols = h.cols;
-            std::vector<uint8_t> cdf(256, 0);
-            std::memcpy(cdf.data(), h.cdf, size_t(h.G) + 1);
-            s.m0 = {h.rows, h.cols, h.grid_spacing, h.min_value, h.G, 0, h.payload_size,
-                    upload(arena, h.cursors, h.rows), upload(arena, cdf.data(), cdf.size()),
-                    upload(arena, h.ppf, 256), nullptr};
-            s.dev_payload = upload(arena, h.payload, h.payload_size, kSlackWords);
-            s.m0.payload = s.dev_payload;
+            mv_check(matvans_matrix_upload(&h, &s.up, &s.m0), "upload");
+            s.dev_payload = const_cast<uint16_t *>(s.m0.payload);
             s.host_payload = h.payload;
             s.payload_words = h.payload_size;
             steps.push_back(s);
@@ -275,6 +251,7 @@ int main(int argc, char **argv) {
         path, mvx ? "MVX1" : "bin v0", num, iters, matched, result_hash, best1, best2, macs);
     for (hipEvent_t e : {start1, start2, stop1, stop2}) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(stream);
+    for (Step &st : steps) matvans_upload_free(st.up);
     if (bin) matvans_bin_free(bin);
     if (mvx) matvans_v1_free(mvx);
     return matched == iters ? 0 : 1;

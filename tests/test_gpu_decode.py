@@ -192,3 +192,49 @@ def test_argument_errors_do_not_launch(cuda):
     with pytest.raises(MatvansError) as e:   # 9 vectors
         cm.multiply(torch.zeros((9, 512), dtype=torch.int8, device="cuda"))
     assert e.value.status == 1
+
+
+def test_library_upload_matches_python_upload(cuda, golden_dir):
+    """matvans_matrix_upload / matvans_v1_matrix_upload (library-owned device copies)
+    give the same decode as the Python-side upload."""
+    import ctypes
+    import torch
+    from matvans_amd.capi import DevMatrix, HostMatrix, V1DevMatrix, call, lib
+    from matvans_amd.mvx import CompressedMatrixV1
+    hdr, vec, mats = mv.read_bin(os.path.join(golden_dir, FIXTURES[0]))
+    m = mats[0]
+    keep = [m.cursors, m.cdf_data, m.ppf_data, m.payload]
+    h = HostMatrix(m.rows, m.cols, 1.0, m.min_value, m.G, 0, m.payload_size,
+                   m.cursors.ctypes.data, m.cdf_data.ctypes.data, m.ppf_data.ctypes.data,
+                   m.payload.ctypes.data)
+    up, view = ctypes.c_void_p(), DevMatrix()
+    call("matvans_matrix_upload", ctypes.byref(h), ctypes.byref(up), ctypes.byref(view))
+    try:
+        v = torch.from_numpy(vec[:m.cols].copy()).cuda()
+        out = torch.empty(m.rows, dtype=torch.int32, device="cuda")
+        call("matvans_decompress_and_multiply", ctypes.c_void_p(out.data_ptr()),
+             ctypes.c_void_p(v.data_ptr()), view.rows, view.cols, view.grid_spacing,
+             ctypes.c_void_p(view.cursors), view.min_value, view.G, ctypes.c_void_p(view.cdf),
+             ctypes.c_void_p(view.ppf), ctypes.c_void_p(view.payload), view.payload_size, None)
+        torch.cuda.synchronize()
+        ref = m.to_device("cuda").multiply(v).cpu().numpy()
+        assert np.array_equal(out.cpu().numpy(), ref)
+    finally:
+        lib().matvans_upload_free(up)
+    del keep
+    w = np.clip(np.rint(np.random.default_rng(3).normal(0, 6, (100, 2048))), -127, 127).astype(np.int8)
+    c = CompressedMatrixV1.encode(w, 64, 8, 1, coder=1)
+    keep = []
+    hv = c.host_struct(keep)
+    up, view = ctypes.c_void_p(), V1DevMatrix()
+    call("matvans_v1_matrix_upload", ctypes.byref(hv), ctypes.byref(up), ctypes.byref(view))
+    try:
+        x = torch.from_numpy(np.arange(2048, dtype=np.int64).astype(np.int8)).cuda()
+        out = torch.empty(100, dtype=torch.int32, device="cuda")
+        call("matvans_v1_decompress_and_multiply", ctypes.byref(view),
+             ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(x.data_ptr()), 1, 2048, None)
+        torch.cuda.synchronize()
+        want = (w.astype(np.int64) @ np.arange(2048).astype(np.int8).astype(np.int64)).astype(np.int32)
+        assert np.array_equal(out.cpu().numpy(), want)
+    finally:
+        lib().matvans_upload_free(up)
