@@ -367,8 +367,10 @@ int matvans_b_stripes_matvec(const uint32_t *compressed, uint64_t compressed_wor
 /* 7-bit model (fit_128bit_encoder_model, encoder.py:268-306, float32 rounding and
  * first-maximum adjustment as numpy) and the decoder tables
  * (decoder_model_from_128bit_encoder_model, :309-320) packed 4 bytes per word
- * (pack_4bytes, :465-471).  cdf receives n_symbols + 1 <= 129 bytes.  More than 128
- * distinct values return MATVANS_E_RANGE (the reference divides by zero there). */
+ * (pack_4bytes, :465-471).  n_symbols = max - min + 1 spans the value range (gaps
+ * included), so cdf receives n_symbols + 1 <= 257 bytes; ppf, ppf_cum and ppf_prob 32
+ * words each.  More than 128 distinct values return MATVANS_E_RANGE (the reference
+ * divides by zero there). */
 int matvans_b_model(const int8_t *data, uint64_t n, int32_t *min_value, uint32_t *n_symbols,
                     uint8_t *cdf, uint32_t *ppf, uint32_t *ppf_cum, uint32_t *ppf_prob);
 /* Upper bound on the words matvans_b_encode writes; layout 0 rows, 1 stripes. */
