@@ -37,6 +37,8 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--replays", type=int, default=5,
+                    help="timed replays of the K-launch graph per format (median reported)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--rows", type=int, default=4096)
     ap.add_argument("--cols", type=int, default=14336)
@@ -107,12 +109,14 @@ def gen(rows, cols, dist, seed=20250310):
     return w, x
 
 
-def time_launches(fn, n_copies, steps, warmup):
-    """Average ms per launch of fn(copy_index) over `steps` back-to-back launches.
+def time_launches(fn, n_copies, steps, warmup, stats=None, replays=1):
+    """ms per launch of fn(copy_index) over `steps` back-to-back launches.
 
     The launches are captured once into a HIP graph and the graph is replayed between
     two events on the replay stream, so the figure is device time per launch (host
-    launch overhead excluded).  Warm-up launches run eagerly first."""
+    launch overhead excluded).  Warm-up launches run eagerly first.  With replays > 1
+    the graph is replayed that many times, each replay timed on its own, and the median
+    per-launch time is returned; `stats` (a dict) then receives the median, p10 and p90."""
     import torch
     for i in range(warmup):
         fn(i % n_copies)
@@ -127,14 +131,22 @@ def time_launches(fn, n_copies, steps, warmup):
     torch.cuda.synchronize()
     g.replay()                      # one untimed replay
     torch.cuda.synchronize()
-    with torch.cuda.stream(s):
-        start = torch.cuda.Event(enable_timing=True)
-        stop = torch.cuda.Event(enable_timing=True)
-        start.record(s)
-        g.replay()
-        stop.record(s)
-    torch.cuda.synchronize()
-    return start.elapsed_time(stop) / steps
+    times = []
+    for _ in range(max(1, replays)):
+        with torch.cuda.stream(s):
+            start = torch.cuda.Event(enable_timing=True)
+            stop = torch.cuda.Event(enable_timing=True)
+            start.record(s)
+            g.replay()
+            stop.record(s)
+        torch.cuda.synchronize()
+        times.append(start.elapsed_time(stop) / steps)
+    ms = float(np.median(times))
+    if stats is not None:
+        stats.update({"ms_median": ms, "ms_p10": float(np.percentile(times, 10)),
+                      "ms_p90": float(np.percentile(times, 90)), "replays": len(times),
+                      "launches_per_replay": steps})
+    return ms
 
 
 def cpu_baseline(cm, x, budget_s=10.0):
@@ -580,10 +592,13 @@ def main():
         if not np.array_equal(got, want):
             print(f"{fmt}: fused decode-matvec differs from the integer product", file=sys.stderr)
             sys.exit(1)
+        st = {}
         ms = time_launches(lambda i: copies[i].multiply(d_vecs, out=out), n_copies, args.steps,
-                           args.warmup)
+                           args.warmup, stats=st, replays=args.replays)
         alg = stored + nvec * cols + 4 * rows * nvec
-        r = {"us": round(ms * 1e3, 3), "bits_per_entry": round(8 * stored / (rows * cols), 4),
+        r = {"us": round(ms * 1e3, 3), "us_p10": round(st["ms_p10"] * 1e3, 3),
+             "us_p90": round(st["ms_p90"] * 1e3, 3),
+             "bits_per_entry": round(8 * stored / (rows * cols), 4),
              "alg_bytes": alg, "GBps": round(alg / (ms * 1e-3) / 1e9, 1), "copies": n_copies}
         if fmt in BACKWARD_FORMATS and not args.no_compare:
             r["bwd_us"] = round(1e3 * time_backward(copies, rows, cols, nvec), 3)
@@ -644,6 +659,15 @@ def main():
     tpath = os.path.join(ROOT, "profiles", f"traffic_fused_{best}_{rows}x{cols}_{args.dist}.json")
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("bytes_per_launch")
+    # VALU / LDS operations per symbol and VALU utilisation of the same kernel and workload,
+    # from its committed rocprofv3 SQ counter run (counters are not collected live)
+    valu = None
+    vpath = os.path.join(ROOT, "profiles", f"valu_fused_{best}_{rows}x{cols}_{args.dist}.json")
+    if os.path.exists(vpath) and nvec == 1:
+        v = json.load(open(vpath))
+        valu = {k: v[k] for k in ("valu_ops_per_symbol", "lds_ops_per_symbol",
+                                  "valu_utilisation", "kernel_avg_ns")}
+        valu["source"] = os.path.relpath(vpath, ROOT)
     line = {
         "metric": "µs/matvec, speedup vs uncompressed int8 (bit-exact), compressed HBM GB/s",
         "value": round(us, 3),
@@ -652,6 +676,11 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_fused, 6),
+        "timing": {"us_median": per_format[best]["us"], "us_p10": per_format[best]["us_p10"],
+                   "us_p90": per_format[best]["us_p90"], "replays": args.replays,
+                   "launches_per_replay": args.steps,
+                   "method": "graph of `steps` launches over distinct matrix copies, replayed "
+                             "`replays` times between HIP events on its stream"},
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
@@ -675,6 +704,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                      "traffic": traffic, "alg_bytes": alg_bytes},
+        "counters": valu,
         "cpu_baseline": None if args.no_cpu else cpu_baseline(cm, x),
     }
     torch.cuda.empty_cache()
