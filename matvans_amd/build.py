@@ -9,6 +9,7 @@ library is built in-tree so that it travels with the source tree.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import concurrent.futures as cf
 import os
 import shutil
@@ -31,6 +32,22 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC)
                   if f.endswith((".hip", ".cpp")))
+
+
+STAMP = os.path.join(LIBDIR, "libmatvans.sources")
+HEADER = os.path.join(HERE, "..", "include", "matvans.h")
+
+
+def source_digest() -> str:
+    """sha256 over the library's sources (csrc/*.hip, *.cpp, *.h and include/matvans.h),
+    the stamp that ties a built libmatvans.so to the sources it was built from."""
+    h = hashlib.sha256()
+    files = sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp", ".h")))
+    for path in [os.path.join(CSRC, f) for f in files] + [HEADER]:
+        h.update(os.path.basename(path).encode() + b"\0")
+        with open(path, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
 
 
 def _compile(src, asm_dir=None):
@@ -67,6 +84,8 @@ def build(jobs: int | None = None, asm_dir: str | None = None, verbose: bool = F
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")
     os.replace(tmp, LIB)
+    with open(STAMP, "w") as fh:
+        fh.write(source_digest() + "\n")
     build_driver()
     return LIB
 

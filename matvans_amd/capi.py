@@ -152,12 +152,34 @@ SIGNATURES = {
 _LIB = None
 
 
+_DEFAULT_LIB_PATH = LIB_PATH
+
+
+def _check_stamp():
+    """The in-tree library must have been built from the sources beside it: build() writes
+    a digest of the sources next to it, and a library whose digest differs (sources edited
+    or reverted without a rebuild) is refused rather than run."""
+    from . import build as B
+    try:
+        with open(B.STAMP) as fh:
+            built = fh.read().strip()
+    except FileNotFoundError:
+        raise RuntimeError(f"{LIB_PATH} has no source stamp ({B.STAMP}); rebuild with "
+                           "python -m matvans_amd.build") from None
+    if built != B.source_digest():
+        raise RuntimeError(f"{LIB_PATH} was built from other sources than matvans_amd/csrc "
+                           "and include/matvans.h hold now; rebuild with "
+                           "python -m matvans_amd.build")
+
+
 def lib():
-    """The loaded library; raises if it is not built."""
+    """The loaded library; raises if it is not built or was built from other sources."""
     global _LIB
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is not built; run python -m matvans_amd.build")
+        if LIB_PATH == _DEFAULT_LIB_PATH:
+            _check_stamp()
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)
