@@ -265,8 +265,8 @@ def bench_block(dev, steps, warmup, scale=1.0, nvecs=(1, 4, 8), fmts=("v0", "q32
                                   round(ms1 * 1e3, 2), "tile_units_order_bpc": list(cfg),
                                   "compressed_GBps": round(alg / (ms * 1e-3) / 1e9, 1)}
             del batches
-        # backward of every matrix from the same compressed copies
-        for nv in (1, 8):
+        # backward of every matrix from the same compressed copies (one and eight vectors)
+        for nv in [n for n in (1, 8) if n in nvecs]:
             g = [torch.from_numpy(np.random.default_rng(5).standard_normal((nv, r))
                                   .astype(np.float32)).to(dev) for _, r, _ in shapes]
             s_w = [torch.full((r,), 0.01, dtype=torch.float32, device=dev)

@@ -6,12 +6,14 @@
 // rows; a wave decodes one stream of the 64 rows, one row per lane.  A stream covers the
 // same columns in every row, so at each decode step all 64 lanes contribute to the same
 // columns.  Every four columns (one decode quad, its int8 symbols packed in one dword) a
-// fixed butterfly sums the 64 rows (bwd_quad): one v_permlane32_swap of the packed
-// dword, a multiply and an fma per pair of rows, one v_permlane16_swap and four DPP row
-// adds, in the same order on every run.  A 16-column group's totals are collected in one
-// register and stored once, to partial[v][row_block][c]; reduce_tile sums the row blocks
-// in a fixed order and applies alpha.  Both stages are deterministic bit for bit, so one-matrix and batched launches
-// give identical results.
+// fixed pair step (bwd_pair) folds the 64 rows to 16 lanes per column: one
+// v_permlane32_swap, a multiply and an fma per pair of rows, one v_permlane16_swap and an
+// add.  After a group's four quads, bwd_group_reduce sums each quad's 16 lanes for all
+// four quads at once (a transposed reduction over DPP rows: row_mirror, row_half_mirror
+// and two quad_perm adds), in the same order on every run.  The group's 16 totals are
+// stored once, to partial[v][row_block][c]; reduce_tile sums the row blocks in a fixed
+// order and applies alpha.  Both stages are deterministic bit for bit, so one-matrix and
+// batched launches give identical results.
 //
 // With two or more gradient vectors the group sums run on the matrix cores instead
 // (bwd_group_mfma), one MFMA sequence for all vectors.
@@ -40,15 +42,6 @@ __device__ __forceinline__ float dpp_f(float v) {
     return __int_as_float(r);
 }
 
-// Sum over the 16 lanes of each DPP row in a fixed order (every lane gets its row sum).
-__device__ __forceinline__ float row_sum_fixed(float v) {
-    v += dpp_f<0xB1, 0xF>(v);    // quad_perm [1,0,3,2]
-    v += dpp_f<0x4E, 0xF>(v);    // quad_perm [2,3,0,1]
-    v += dpp_f<0x141, 0xF>(v);   // row_half_mirror
-    v += dpp_f<0x140, 0xF>(v);   // row_mirror
-    return v;
-}
-
 // One matrix of a backward (device pointers).
 struct BwdJob {
     float *partial;              // workspace [n_vec][nrb][cols], nrb = ceil(rows / 64)
@@ -67,7 +60,7 @@ struct BwdJob {
 };
 
 // From this many gradient vectors on, a group's sums are computed on the matrix cores
-// (bwd_group_mfma) instead of one butterfly per vector.
+// (bwd_group_mfma) instead of pair steps and row reductions per vector.
 constexpr int kMfmaNV = 2;
 
 template <int NV>
@@ -113,20 +106,18 @@ struct BwdRows {
     }
 };
 
-// Butterfly of one decode quad: packed holds the four int8 symbols of this lane's row (in
-// column order), gsA / gsB the scaled gradients of rows (lane & 31) and (lane & 31) + 32,
-// sh = 0 for lanes 0..31 and 8 for lanes 32..63.  After the v_permlane32_swap, X holds
-// row (lane & 31)'s symbols and Y row (lane & 31) + 32's; shifted by sh, bytes 0 and 2 are
-// columns (0, 2) in lanes 0..31 and (1, 3) in lanes 32..63.  a, b are those two columns'
-// pair sums; v_permlane16_swap + add leaves the rows of 16 lanes with columns
-// (0, 2, 1, 3), summed over 32 row pairs by row_sum_fixed.  Lane 16k + d keeps the total
-// of its row k for quad d.
+// Pair step of one decode quad: packed holds the four int8 symbols of this lane's row (in
+// column order), gsA / gsB the scaled gradients of rows (lane & 31) and (lane & 31) + 32.
+// Swapping packed against packed >> 8 across the wave halves leaves, in bytes 0 and 2,
+// columns (0, 2) in lanes 0..31 and (1, 3) in lanes 32..63, of row (lane & 31) in X and of
+// row (lane & 31) + 32 in Y.  a, b are those two columns' pair sums; v_permlane16_swap +
+// add leaves the 16-lane rows k = 0..3 of the wave with columns (0, 2, 1, 3), each lane
+// holding the sum over four rows.
 template <int NV>
-__device__ __forceinline__ void bwd_quad(uint32_t packed, const float (&gsA)[NV],
-                                         const float (&gsB)[NV], uint32_t sh, uint32_t lane,
-                                         uint32_t d, float (&outv)[NV]) {
-    auto sw = __builtin_amdgcn_permlane32_swap(packed, packed, false, false);
-    const uint32_t X = sw[0] >> sh, Y = sw[1] >> sh;
+__device__ __forceinline__ void bwd_pair(uint32_t packed, const float (&gsA)[NV],
+                                         const float (&gsB)[NV], float (&s)[NV]) {
+    auto sw = __builtin_amdgcn_permlane32_swap(packed, packed >> 8, false, false);
+    const uint32_t X = sw[0], Y = sw[1];
     const float xa = cvt_f32_sbyte0(X), xb = cvt_f32_sbyte2(X);
     const float ya = cvt_f32_sbyte0(Y), yb = cvt_f32_sbyte2(Y);
 #pragma unroll
@@ -135,18 +126,36 @@ __device__ __forceinline__ void bwd_quad(uint32_t packed, const float (&gsA)[NV]
         const float b = fmaf(yb, gsB[v], xb * gsA[v]);
         auto t = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false,
                                                   false);
-        const float tot = row_sum_fixed(__uint_as_float(t[0]) + __uint_as_float(t[1]));
-        outv[v] = (lane & 15) == d ? tot : outv[v];
+        s[v] = __uint_as_float(t[0]) + __uint_as_float(t[1]);
     }
 }
 
-// Per-lane constants of the butterfly and the group store.
+// Sums s[d] over the 16 lanes of each DPP row for the four quads d of a group at once:
+// lanes i < 8 of a row keep quads 0, 1 and hand quads 2, 3 to lane 15 - i (row_mirror),
+// then lanes with bit 2 clear keep the lower of their two quads and hand the other to
+// lane i ^ 7 (row_half_mirror), then two quad_perm adds finish the row.  Lane i of row k
+// ends with the total of quad i >> 2, column 4 (i >> 2) + (0, 2, 1, 3)[k].
+template <int NV>
+__device__ __forceinline__ void bwd_group_reduce(const float (&s)[4][NV], uint32_t lane,
+                                                 float (&tot)[NV]) {
+    const bool hi8 = (lane & 8u) != 0, hi4 = (lane & 4u) != 0;
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        const float u0 = (hi8 ? s[2][v] : s[0][v]) + dpp_f<0x140, 0xF>(hi8 ? s[0][v] : s[2][v]);
+        const float u1 = (hi8 ? s[3][v] : s[1][v]) + dpp_f<0x140, 0xF>(hi8 ? s[1][v] : s[3][v]);
+        float w = (hi4 ? u1 : u0) + dpp_f<0x141, 0xF>(hi4 ? u0 : u1);
+        w += dpp_f<0xB1, 0xF>(w);    // quad_perm [1,0,3,2]
+        w += dpp_f<0x4E, 0xF>(w);    // quad_perm [2,3,0,1]
+        tot[v] = w;
+    }
+}
+
+// Per-lane constants of the pair step and the group store.
 template <int NV>
 struct BwdLane {
     float gsA[NV], gsB[NV];
-    uint32_t sh;      // 0 for lanes 0..31, 8 for lanes 32..63
     uint32_t col;     // column of this lane's group total within the 16-column group
-    bool writer;      // lanes 16k + d, d < 4, hold a total
+    bool writer;      // lanes 4q of each 16-lane row hold quad q's total
     __device__ __forceinline__ void init(const float (&gs)[NV], uint32_t lane) {
 #pragma unroll
         for (int v = 0; v < NV; v++) {
@@ -155,18 +164,33 @@ struct BwdLane {
             gsA[v] = __uint_as_float(sw[0]);
             gsB[v] = __uint_as_float(sw[1]);
         }
-        sh = lane < 32 ? 0u : 8u;
-        const uint32_t d = lane & 15, k = lane >> 4;   // rows of 16 lanes: columns 0, 2, 1, 3
-        col = 4 * d + ((k & 1) * 2 + (k >> 1));
-        writer = d < 4;
+        const uint32_t i = lane & 15, k = lane >> 4;   // rows of 16 lanes: columns 0, 2, 1, 3
+        col = 4 * (i >> 2) + ((k & 1) * 2 + (k >> 1));
+        writer = (i & 3) == 0;
     }
-    __device__ __forceinline__ void store(const float (&outv)[NV], float *out, size_t plane,
+    __device__ __forceinline__ void store(const float (&tot)[NV], float *out, size_t plane,
                                           uint32_t n_vec) const {
         if (writer) {
 #pragma unroll
             for (int v = 0; v < NV; v++)
-                if (v < int(n_vec)) out[size_t(v) * plane + col] = outv[v];
+                if (v < int(n_vec)) out[size_t(v) * plane + col] = tot[v];
         }
+    }
+    // a group's four packed quads: pair steps, then the transposed row reduction
+    template <class Next>
+    __device__ __forceinline__ void group(Next next, uint32_t lane, float (&tot)[NV]) const {
+        float s[4][NV];
+        // quad d's pair step follows quad d+1's decode: independent work between the
+        // decode's dependent steps and the swaps' operand hazards
+        uint32_t prev = next();
+#pragma unroll
+        for (int d = 1; d < 4; d++) {
+            const uint32_t cur = next();
+            bwd_pair<NV>(prev, gsA, gsB, s[d - 1]);
+            prev = cur;
+        }
+        bwd_pair<NV>(prev, gsA, gsB, s[3]);
+        bwd_group_reduce<NV>(s, lane, tot);
     }
 };
 
@@ -258,12 +282,9 @@ __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32
             bwd_group_mfma(P, M.b, lane, sm.wstage + wave * 256, base + t * 512u, plane,
                            J.n_vec);
         } else {
-            float outv[NV];
-#pragma unroll
-            for (int v = 0; v < NV; v++) outv[v] = 0.0f;
-#pragma unroll
-            for (int d = 0; d < 4; d++) bwd_quad<NV>(next4(), L.gsA, L.gsB, L.sh, lane, d, outv);
-            L.store(outv, base + t * 512u, plane, J.n_vec);
+            float tot[NV];
+            L.group(next4, lane, tot);
+            L.store(tot, base + t * 512u, plane, J.n_vec);
         }
         st.end_group(wbefore);
     };
@@ -319,13 +340,9 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
             bwd_group_mfma(P, M.b, lane, sm.wstage + wave * 256, base + t * (16u * S), plane,
                            J.n_vec);
         } else {
-            float outv[NV];
-#pragma unroll
-            for (int v = 0; v < NV; v++) outv[v] = 0.0f;
-#pragma unroll
-            for (int d = 0; d < 4; d++)
-                bwd_quad<NV>(st.quad(sm.tab), L.gsA, L.gsB, L.sh, lane, d, outv);
-            L.store(outv, base + t * (16u * S), plane, J.n_vec);
+            float tot[NV];
+            L.group([&]() { return st.quad(sm.tab); }, lane, tot);
+            L.store(tot, base + t * (16u * S), plane, J.n_vec);
         }
         st.end_group(wbefore);
     };
