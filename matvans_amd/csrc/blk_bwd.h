@@ -373,7 +373,16 @@ __device__ __forceinline__ void reduce_tile(const BwdJob &J, uint32_t nrb, uint3
     if (c < cols) {
         const float *src = J.partial + size_t(v) * nrb * cols + c;
         uint32_t b0 = p * chunk, b1 = b0 + chunk < nrb ? b0 + chunk : nrb;
-        for (uint32_t b = b0; b < b1; b++) s += src[size_t(b) * cols];
+        // eight loads in flight, then the adds in ascending row-block order
+        for (uint32_t b = b0; b < b1; b += 8) {
+            float x[8];
+#pragma unroll
+            for (uint32_t k = 0; k < 8; k++)   // past b1: row block b1 - 1 again (not added)
+                x[k] = src[size_t(b + k < b1 ? b + k : b1 - 1) * cols];
+#pragma unroll
+            for (uint32_t k = 0; k < 8; k++)
+                if (b + k < b1) s += x[k];
+        }
     }
     part[p][cl] = s;
     __syncthreads();
