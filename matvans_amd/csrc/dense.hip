@@ -148,10 +148,19 @@ __global__ __launch_bounds__(1024) void absmax_requant_kernel(const int32_t *__r
                                                               int clamp) {
     __shared__ uint32_t red[16];
     uint32_t m = 0;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        int32_t a = acc[i];
-        uint32_t u = a < 0 ? 0u - uint32_t(a) : uint32_t(a);
-        m = u > m ? u : m;
+    // four loads in flight per thread; an index past n re-reads element i (max unchanged)
+    for (uint32_t i = threadIdx.x; i < n; i += 4 * blockDim.x) {
+        int32_t a[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint32_t j = i + k * blockDim.x;
+            a[k] = acc[j < n ? j : i];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint32_t u = a[k] < 0 ? 0u - uint32_t(a[k]) : uint32_t(a[k]);
+            m = u > m ? u : m;
+        }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -174,13 +183,23 @@ __global__ __launch_bounds__(1024) void absmax_requant_kernel(const int32_t *__r
     const float a = static_cast<float>(red[0]);
     const float d = a / 127.0f;   // IEEE fp32 division (v_delta = abs_max / 127)
     if (threadIdx.x == 0) *delta = d;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        float q = rintf(static_cast<float>(acc[i]) / d);   // __float2int_rn(afl / scalar)
-        int32_t v = (q == q) ? static_cast<int32_t>(fminf(fmaxf(q, -2147483648.0f),
-                                                          2147483520.0f))
-                             : 0;
-        if (clamp) v = v < -128 ? -128 : (v > 127 ? 127 : v);
-        out8[i] = static_cast<int8_t>(v);
+    for (uint32_t i = threadIdx.x; i < n; i += 4 * blockDim.x) {
+        int32_t a[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint32_t j = i + k * blockDim.x;
+            a[k] = acc[j < n ? j : i];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint32_t j = i + k * blockDim.x;
+            float q = rintf(static_cast<float>(a[k]) / d);   // __float2int_rn(afl / scalar)
+            int32_t v = (q == q) ? static_cast<int32_t>(fminf(fmaxf(q, -2147483648.0f),
+                                                              2147483520.0f))
+                                 : 0;
+            if (clamp) v = v < -128 ? -128 : (v > 127 ? 127 : v);
+            if (j < n) out8[j] = static_cast<int8_t>(v);
+        }
     }
 }
 

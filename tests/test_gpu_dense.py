@@ -47,6 +47,17 @@ def test_f32_gemv(cuda, rows, cols):
     assert np.all(np.abs(got.cpu().numpy() - want) <= 1e-5 * scale)
 
 
+def edge_max_cases(rng):
+    """the extreme value in the last, partial stride of the kernel's loop (n past a
+    multiple of the block) and a negative extreme"""
+    out = []
+    for n, at, sign in [(4097, 4096, 1), (5001, 5000, -1), (1025, 1, 1)]:
+        a = rng.integers(-1000, 1000, n).astype(np.int32)
+        a[at] = sign * 123457
+        out.append(a)
+    return out
+
+
 @pytest.mark.parametrize("clamp", [False, True])
 def test_absmax_requant(cuda, clamp):
     import torch
@@ -55,7 +66,7 @@ def test_absmax_requant(cuda, clamp):
     for acc in [rng.integers(-2**20, 2**20, 4096).astype(np.int32),
                 np.array([254, 1, 3, 5, -5, -3], np.int32),            # ties
                 rng.integers(-2**30, 2**30, 14336).astype(np.int32),    # > 2^24
-                np.zeros(7, np.int32)]:
+                np.zeros(7, np.int32)] + edge_max_cases(rng):
         out8, d = mv.absmax_requant(torch.from_numpy(acc).cuda(), clamp=clamp)
         torch.cuda.synchronize()
         ref8, refd = O.requant_f32(acc)
