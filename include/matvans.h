@@ -415,7 +415,8 @@ typedef struct {
     uint32_t rows, cols;
     uint32_t n_vec, vec_stride;  /* 1..8 vectors, vec_stride bytes apart */
     const int8_t *vecs;          /* device */
-    int32_t *dst;                /* int32 [n_vec][rows] (v0, quad); family B: use acc32/out8 */
+    int32_t *dst;                /* v0, quad: int32 [n_vec][rows], or NULL with out8 set;
+                                    family B: use acc32 */
     /* v0 and quad */
     const uint32_t *cursors;
     const uint16_t *payload;
@@ -431,8 +432,10 @@ typedef struct {
     const uint32_t *warp_offsets;
     uint32_t end_offset;
     const uint32_t *b_ppf, *b_cum, *b_prob;
+    /* every format: the next layer's int8 input, out8[v*rows + r] =
+       clamp(rint(float(sum) * scale), -128, 127) (ans-warp-rows-kernel.cu:212-220), or NULL */
     float scale;
-    int32_t *acc32;
+    int32_t *acc32;              /* family B: int32 sums, or NULL */
     int8_t *out8;
 } matvans_batch_item;
 

@@ -21,11 +21,14 @@ def _p(t):
 
 
 class Batch:
-    """entries: list of (matrix, vecs) where matrix is a CompressedMatrix (.bin v0), a
-    CompressedMatrixV1 with coder=1 and 32 or 64 streams, or a (BufferedMatrices, k)
-    pair; vecs is an int8 device tensor [n_vec, cols] (or [cols]).  All matrices must be
-    on the device.  ``outputs[i]`` is the int32 [n_vec, rows] result of entry i (family-B
-    entries also fill ``out8[i]`` = clamp(rint(acc * scale)))."""
+    """entries: list of (matrix, vecs), (matrix, vecs, scale) or (matrix, vecs, scale,
+    int32) where matrix is a CompressedMatrix (.bin v0), a CompressedMatrixV1 with
+    coder=1 and 32 or 64 streams, or a (BufferedMatrices, k) pair; vecs is an int8 device
+    tensor [n_vec, cols] (or [cols]).  All matrices must be on the device.
+    ``outputs[i]`` is the int32 [n_vec, rows] result of entry i (None when int32 is
+    False); ``out8[i]`` the next layer's int8 input clamp(rint(float(acc) * scale)),
+    computed in the same launch, for entries with a scale (family-B entries: always, with
+    the file's scale unless one is given), else None."""
 
     def __init__(self, entries):
         import torch
@@ -33,7 +36,12 @@ class Batch:
         self._keep = []
         self.outputs, self.out8 = [], []
         arr = (BatchItem * len(entries))()
-        for i, (m, v) in enumerate(entries):
+        for i, entry in enumerate(entries):
+            m, v, *rest = entry
+            scale = rest[0] if rest else None
+            want32 = bool(rest[1]) if len(rest) > 1 else True
+            if not want32 and scale is None:
+                raise ValueError("batch entry %d: no output (int32 off and no scale)" % i)
             v = v.unsqueeze(0) if v.dim() == 1 else v
             self._keep.append(v)
             it = arr[i]
@@ -51,14 +59,20 @@ class Batch:
                 it.b_ppf = bm.d["ppf"].data_ptr() + 128 * k
                 it.b_cum = bm.d["cum"].data_ptr() + 128 * k
                 it.b_prob = bm.d["prob"].data_ptr() + 128 * k
-                it.scale = float(bm.scales[k])
-                it.acc32, it.out8 = acc.data_ptr(), o8.data_ptr()
-                it.dst = acc.data_ptr()
+                it.scale = float(bm.scales[k] if scale is None else scale)
+                if not want32:
+                    acc = None
+                it.acc32, it.out8 = _p(acc), o8.data_ptr()
+                it.dst = _p(acc)
             else:
                 rows, cols = m.rows, m.cols
-                acc = torch.empty((v.shape[0], rows), dtype=torch.int32, device=v.device)
+                acc = (torch.empty((v.shape[0], rows), dtype=torch.int32, device=v.device)
+                       if want32 else None)
                 o8 = None
-                it.dst, it.cursors = acc.data_ptr(), _p(m.d_cursors)
+                if scale is not None:
+                    o8 = torch.empty((v.shape[0], rows), dtype=torch.int8, device=v.device)
+                    it.out8, it.scale = o8.data_ptr(), float(scale)
+                it.dst, it.cursors = _p(acc), _p(m.d_cursors)
                 it.payload, it.payload_size = _p(m.d_payload), m.payload_size
                 if isinstance(m, CompressedMatrix):
                     it.format = V0

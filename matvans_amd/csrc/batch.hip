@@ -268,7 +268,8 @@ hipError_t build_tiles(matvans_batch *p) {
 
 int check_item(const matvans_batch_item &it, uint32_t k) {
     const bool b = it.format == MATVANS_BATCH_B_ROWS || it.format == MATVANS_BATCH_B_STRIPES;
-    if (!it.vecs || (!b && !it.dst)) return fail(MATVANS_E_INVALID, "item %u: null vecs or dst", k);
+    if (!it.vecs || (!b && !it.dst && !it.out8))
+        return fail(MATVANS_E_INVALID, "item %u: null vecs, or neither dst nor out8", k);
     if (it.n_vec < 1 || it.n_vec > 8) return fail(MATVANS_E_INVALID, "item %u: n_vec %u", k, it.n_vec);
     if (!it.rows || !it.cols) return fail(MATVANS_E_SHAPE, "item %u: empty matrix", k);
     uint32_t stride = it.n_vec == 1 && it.vec_stride < it.cols ? it.cols : it.vec_stride;
@@ -398,7 +399,8 @@ extern "C" int matvans_batch_create(const matvans_batch_item *items, uint32_t n,
                 p->chain.push_back(it.cols / it.streams / 4);
             }
             m.v1 = mv::blk::V1Job{it.dst, it.vecs, stride, it.n_vec, it.rows, it.cols, it.cursors,
-                                  it.min_value, it.G, lut, nullptr, it.payload, it.payload_size};
+                                  it.min_value, it.G, lut, nullptr, it.payload, it.payload_size,
+                                  it.out8, it.scale};
         } else {
             const bool rows = it.format == MATVANS_BATCH_B_ROWS;
             m.path = rows ? mv::kPathBRows : mv::kPathBStripes;

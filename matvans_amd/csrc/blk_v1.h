@@ -39,10 +39,10 @@ __device__ __forceinline__ uint32_t wave_excl_max(uint32_t v, uint32_t lane) {
     return lane ? prev : 0u;
 }
 
-// One MVX1 matrix with its vectors and output (device pointers).  n_vec <= the kernel's
-// NV: vectors past n_vec are neither read nor written.
+// One MVX1 matrix with its vectors and outputs (device pointers).  n_vec <= the kernel's
+// NV: vectors past n_vec are neither read nor written.  dst and out8 may each be null.
 struct V1Job {
-    int32_t *dst;            // [n_vec][rows]
+    int32_t *dst;            // [n_vec][rows] int32 sums
     const int8_t *vecs;      // [n_vec][vec_stride]
     uint32_t vec_stride, n_vec;
     uint32_t rows, cols;
@@ -53,7 +53,20 @@ struct V1Job {
     const uint8_t *freq;     // per-row frequencies, table mode 1
     const uint16_t *payload;
     uint32_t payload_size;
+    int8_t *out8;            // [n_vec][rows] clamp(rint(float(sum) * scale), -128, 127)
+    float scale;
 };
+
+// Output of row `row`, vector v: the int32 sum and / or the next layer's int8 input with
+// the epilogue of the reference's family-B kernel (ans-warp-rows-kernel.cu:212-220).
+__device__ __forceinline__ void put_row(const V1Job &J, int v, uint32_t row, int32_t a) {
+    if (J.dst) J.dst[size_t(v) * J.rows + row] = a;
+    if (J.out8) {
+        float y = rintf(float(a) * J.scale);
+        y = y > 127.0f ? 127.0f : (y < -128.0f ? -128.0f : y);
+        J.out8[size_t(v) * J.rows + row] = static_cast<int8_t>(y);
+    }
+}
 
 // Rows a block covers for S streams per row.
 template <int S>
@@ -152,7 +165,6 @@ __device__ __forceinline__ void v1_block(const V1Job &J, uint32_t bid, V1Smem<NV
     const uint8_t *__restrict__ freq = J.freq;
     const uint16_t *__restrict__ payload = J.payload;
     const int8_t *__restrict__ vecs = J.vecs;
-    int32_t *__restrict__ dst = J.dst;
 
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid >> 6, lane = tid & 63;
@@ -293,13 +305,13 @@ __device__ __forceinline__ void v1_block(const V1Job &J, uint32_t bid, V1Smem<NV
                 int32_t a = 0;
 #pragma unroll
                 for (int w = 0; w < WPR; w++) a += part[(wave + w) * NV + v];
-                if (v < int(J.n_vec)) dst[size_t(v) * rows + row] = a;
+                if (v < int(J.n_vec)) put_row(J, v, row, a);
             }
         }
     } else if ((lane & (S - 1)) == 0 && valid) {
 #pragma unroll
         for (int v = 0; v < NV; v++)
-            if (v < int(J.n_vec)) dst[size_t(v) * rows + row] = acc[v];
+            if (v < int(J.n_vec)) put_row(J, v, row, acc[v]);
     }
 }
 
@@ -322,7 +334,6 @@ __device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid,
     const uint32_t *__restrict__ glut = J.glut;
     const uint16_t *__restrict__ payload = J.payload;
     const int8_t *__restrict__ vecs = J.vecs;
-    int32_t *__restrict__ dst = J.dst;
 
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid >> 6, lane = tid & 63;
@@ -432,13 +443,13 @@ __device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid,
                 int32_t a = 0;
 #pragma unroll
                 for (int w = 0; w < WPR; w++) a += part[(wave + w) * NV + v];
-                if (v < int(J.n_vec)) dst[size_t(v) * rows + row] = a;
+                if (v < int(J.n_vec)) put_row(J, v, row, a);
             }
         }
     } else if ((lane & (S - 1)) == 0 && valid) {
 #pragma unroll
         for (int v = 0; v < NV; v++)
-            if (v < int(J.n_vec)) dst[size_t(v) * rows + row] = acc[v];
+            if (v < int(J.n_vec)) put_row(J, v, row, acc[v]);
     }
 }
 

@@ -40,6 +40,8 @@ def create(fn, items):
 
 @pytest.mark.parametrize("kw,status", [
     ({"vecs": 0}, 1),                        # null vectors
+    ({"dst": 0}, 1),                         # quad: neither int32 nor int8 output
+    ({"format": V0, "dst": 0}, 1),           # .bin v0: the same
     ({"n_vec": 9}, 1),                       # more than 8 vectors
     ({"n_vec": 0}, 1),
     ({"rows": 0}, 2),                        # empty matrix

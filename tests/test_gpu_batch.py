@@ -105,6 +105,74 @@ def test_mixed_uneven_batch(cuda):
             assert np.array_equal(o.cpu().numpy(), want(m, v)), (cfg, fmt)
 
 
+def test_int8_epilogue_every_format(cuda):
+    """The next layer's int8 input computed in the launch, for .bin v0, quad and family-B
+    items: out8 = clamp(rint(float(acc) * scale)) (the reference's family-B epilogue,
+    restated by the oracle), with and without the int32 output, and a two-layer chain
+    whose second plan reads the first plan's out8 as its vectors."""
+    import torch
+    note("batch int8 epilogue, every format")
+    specs = [("v0", 300, 1024, 0.003), ("q32", 77, 2048, 0.0125), ("q64", 129, 1024, 0.05),
+             ("brows", 90, 384, 0.02), ("q32", 1024, 4096, 0.004)]
+    entries, checks = [], []
+    for i, (fmt, rows, cols, scale) in enumerate(specs):
+        m = gaussian_int8(rows, cols, 6.0, seed=50 + i)[0]
+        n_vec = 1 + i % 3
+        v = vecs_for(cols, n_vec, 300 + 10 * i)
+        dv = torch.from_numpy(v).cuda()
+        want32 = fmt != "q64"          # the q64 item writes only its int8 output
+        if fmt == "brows":
+            bm = BufferedMatrices.encode([m], "rows", scales=np.array([0.01], np.float32))
+            entries.append(((bm.to_device(), 0), dv, scale))   # overrides the file's scale
+        else:
+            entries.append((make(fmt, m), dv, scale, want32))
+        checks.append((m, v, scale, want32))
+    b = Batch(entries)
+    for rep in range(2):
+        b.launch()
+        torch.cuda.synchronize()
+        for (m, v, scale, want32), o, o8 in zip(checks, b.outputs, b.out8):
+            acc = want(m, v)
+            assert np.array_equal(o8.cpu().numpy(), B.requant_b(acc, scale)), (rep, scale)
+            if want32:
+                assert np.array_equal(o.cpu().numpy(), acc)
+            else:
+                assert o is None
+    # two layers: 1024 int8 outputs of the last item feed a 512 x 1024 quad matrix
+    m2 = gaussian_int8(512, 1024, 6.0, seed=99)[0]
+    first = b.out8[-1]                 # [n_vec, 1024]
+    b2 = Batch([(make("q32", m2), first, 0.01)])
+    b.launch()
+    b2.launch()
+    torch.cuda.synchronize()
+    x1 = B.requant_b(want(checks[-1][0], checks[-1][1]), checks[-1][2])
+    assert np.array_equal(b2.outputs[0].cpu().numpy(), want(m2, x1))
+    assert np.array_equal(b2.out8[0].cpu().numpy(), B.requant_b(want(m2, x1), 0.01))
+    # the same chain captured in one graph and replayed
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            b.launch(stream=s)
+            b2.launch(stream=s)
+    for rep in range(3):
+        b2.outputs[0].zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(b2.outputs[0].cpu().numpy(), want(m2, x1)), rep
+
+
+def test_entry_without_output_is_refused(cuda):
+    import torch
+    note("batch entry without output")
+    m = gaussian_int8(64, 1024, 6.0, seed=1)[0]
+    dv = torch.from_numpy(vecs_for(1024, 1, 1)).cuda()
+    with pytest.raises(ValueError):
+        Batch([(make("q32", m), dv, None, False)])
+
+
 def test_graph_replays_and_autotune(cuda):
     import torch
     note("batch graph replays + autotune")
