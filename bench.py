@@ -525,8 +525,8 @@ def bench_small(dev, steps, warmup):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the row-sharded "
-              "multi-GPU design is described in DESIGN.md and not run here.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); sharding rows "
+              "across GPUs is a design, not run here.", file=sys.stderr)
         sys.exit(2)
     import torch
     import matvans_amd as mv

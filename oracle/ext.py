@@ -2,7 +2,7 @@
 P-bit probabilities (8 or 12) and per-row frequency tables.
 
 TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` may import it, and only as the checker.
+``cpu_baseline`` leg of ``bench.py`` may import it, and only to check results.
 
 The extensions have no reference implementation; they generalise the reference's
 WarpPerRow codec (create_matrices_WarpPerRow.py:131-272,353-430 and

@@ -3,7 +3,7 @@ probabilities, 64-bit state, 32-bit words) and its two layouts, ans-warp-rows an
 ans-warp-stripes.
 
 TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` may import it, and only as the checker.
+``cpu_baseline`` leg of ``bench.py`` may import it, and only to check results.
 
 Restates buffered/encoder.py and the decoders of buffered/ans-warp-rows-kernel.cu and
 buffered/ans-warp-stripes-kernel.cu; each function cites the lines it follows.

@@ -3,7 +3,7 @@
  *
  * Plain-C restatement of the reference decoder, encoder and integer matvec.  Only
  * tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py load this
- * library, and only as the checker; the product (matvans_amd) never links it.
+ * library, and only to check results; the product (matvans_amd) never links it.
  *
  *   oracle_fxhash      create_matrices_WarpPerRow.py:58-70, matvansWarpPerRow.cu:31-42
  *   oracle_decode_v0   matvansWarpPerRow-kernel.cu:95-174 (word order of the per-lane

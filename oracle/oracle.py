@@ -2,7 +2,7 @@
 
 TEST INFRASTRUCTURE ONLY.  Nothing under ``oracle/`` is part of the product: only
 ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py``
-may import it, and only as the checker.  The product path (``matvans_amd``) never
+may import it, and only to check results.  The product path (``matvans_amd``) never
 calls into it and fails loudly when its HIP library is missing.
 
 This module is a numpy restatement of the reference algorithm.  Each function cites

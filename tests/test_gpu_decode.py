@@ -1,7 +1,7 @@
 """GPU parity of the fused decode-matvec against the oracle (bit-exact int32).
 
-Every launch goes through the C ABI of libmatvans.so; the oracle (oracle/) is only the
-checker.  Cases: the golden files written by the reference generator, the reference
+Every launch goes through the C ABI of libmatvans.so; the oracle (oracle/) only checks
+the results.  Cases: the golden files written by the reference generator, the reference
 chain hash, small ragged shapes, 1..8 vectors, and the full LLM shapes."""
 import ctypes
 import os
