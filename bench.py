@@ -53,10 +53,11 @@ def parse():
     ap.add_argument("--headline", default="q32",
                     help="encoding whose time is the line's value (the fastest is reported "
                          "beside it; if the headline encoding is not timed, the fastest is used)")
-    ap.add_argument("--batch", default="block,uneven,shapes,small",
+    ap.add_argument("--batch", default="block,uneven,shapes,small,chain",
                     help="workloads to time beside the headline: block, uneven, shapes "
                          "(4096x4096, 14336x4096 and the backward at both LLM shapes), small "
-                         "(the 1024x1024 plain case) (comma list, or 'none')")
+                         "(the 1024x1024 plain case), chain (the reference's chained "
+                         "harness, 8 x 4096x4096) (comma list, or 'none')")
     ap.add_argument("--batch-scale", type=float, default=1.0,
                     help="scale every batched shape by this factor (1 = the full sizes)")
     return ap.parse_args()
@@ -474,6 +475,68 @@ def bench_shapes(dev, steps, warmup, nvecs=(1, 4, 8)):
     return out
 
 
+def bench_chain(dev, steps, warmup, d=4096, n_mat=8):
+    """The reference's own benchmark workload (matvansWarpPerRow.cu:280-385): a chain of
+    n_mat d x d matrices from its generator (5-bit grid, create_matrices_WarpPerRow.py:35-50,
+    restated by the oracle), v_k = requant(W_k v_(k-1)), each step the host composite
+    (fused decode-matvec, then the device absmax + requant, no host round trip), for the
+    .bin v0 layout and q32.  One timed step is one whole chain on one of several distinct
+    device copies (together more than twice the MALL).  The last vector is checked
+    bit-exact against the oracle's chain with the same fp32 requant."""
+    import torch
+    import matvans_amd as mv
+    from matvans_amd.mvx import CompressedMatrixV1
+    from oracle import oracle as O
+    mats, _deltas, vec = O.gen_reference_matrices(d, n_mat, 5)
+    v = vec.copy()
+    for m in mats:                     # oracle chain with the device epilogue
+        v, _ = O.requant_f32(exact_product(m, v[None])[0])
+    want_last = v
+    d_vec = torch.from_numpy(vec).to(dev)
+    out = {"d": d, "matrices": n_mat, "grid_bits": 5,
+           "reference_fig2_mac_per_s_d4096": 7.7e10,
+           "reference_note": "whitepaper Fig. 2, unnamed NVIDIA GPU (sm_61/sm_75 build), chain "
+                             "with a device-to-host copy and a thrust sync per matrix"}
+    acc = torch.empty(d, dtype=torch.int32, device=dev)
+    bufs = [torch.empty(d, dtype=torch.int8, device=dev) for _ in range(2)]
+    delta = torch.empty(1, dtype=torch.float32, device=dev)
+    for fmt in ("v0", "q32"):
+        if fmt == "v0":
+            enc = [mv.CompressedMatrix.encode(m) for m in mats]
+            stored = sum(c.payload_size * 2 + 4 * d + c.G + 1 + 256 for c in enc)
+            make = _v0_copy
+        else:
+            enc = [CompressedMatrixV1.encode(m, 32, 8, 0, coder=1) for m in mats]
+            stored = sum(c.stored_bytes for c in enc)
+            make = _v1_copy
+        n_copies = min(16, max(2, -(-2 * MALL_BYTES // stored)))
+        chains = [[make(c).to_device(dev) for c in enc] for _ in range(n_copies)]
+
+        def run(i, chain=None):
+            x = d_vec
+            for k, c in enumerate(chain if chain is not None else chains[i]):
+                y = bufs[k % 2]
+                if fmt == "v0":
+                    c.decompress_and_mult(y, acc, x, delta)
+                else:
+                    c.multiply(x[None], out=acc[None])
+                    mv.absmax_requant(acc, out8=y, delta=delta)
+                x = y
+            return x
+        last = run(0).cpu().numpy()
+        if not np.array_equal(last, want_last):
+            print(f"chain {fmt}: last vector differs from the oracle chain", file=sys.stderr)
+            sys.exit(1)
+        ms = time_launches(run, n_copies, steps, warmup)
+        out[fmt] = {"us_per_chain": round(ms * 1e3, 2),
+                    "us_per_matvec": round(ms * 1e3 / n_mat, 3),
+                    "mac_per_s": float(f"{n_mat * d * d / (ms * 1e-3):.4g}"),
+                    "bits_per_entry": round(8 * stored / (n_mat * d * d), 4),
+                    "copies": n_copies, "last_vector_matches_oracle": True}
+        del chains
+    return out
+
+
 def bench_small(dev, steps, warmup):
     """The reference's plain CPU case: one 1024x1024 int8 matrix, entries
     clip(rint(N(0, 6^2))), per-row scale s_w = rowmax|W_float| / 127, x uniform int8,
@@ -721,6 +784,8 @@ def main():
         torch.cuda.empty_cache()
     if "small" in batch:
         line["small"] = bench_small(dev, bsteps, bwarm)
+    if "chain" in batch:
+        line["chain"] = bench_chain(dev, max(1, bsteps // 5), bwarm)
     print(json.dumps(line))
 
 
