@@ -75,7 +75,7 @@ template <int NV, int S>
 struct QuadBwdSmem {
     uint32_t ring[kBwdWaves * v1::QuadLane<128 / S>::R4 * 64];
     uint32_t tab[256];
-    uint32_t dummy[(128 / S) * 64];
+    uint32_t dummy[v1::QuadLane<128 / S>::M * 64];
     uint32_t wstage[NV >= kMfmaNV ? kBwdWaves * 256 : 1];
 };
 

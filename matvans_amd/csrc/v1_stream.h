@@ -260,27 +260,31 @@ struct Lane {
 // Quad coder (coder 1): per-lane 64-bit state {lo, hi}, 32-bit words in a ring of
 // R4 = 16 u32 words per lane (lane-interleaved: word slot w of lane i at (w*64 + i)*4
 // bytes of the wave's 4 KiB ring).  A 16-symbol group is four quads and takes at most
-// M = 4 words; NG = M / G4 granules (G4 = 128/S words per stream per granule) are written
-// per group, loaded two groups ahead, with the same fit rule and no-unwritten-slot bound
-// as Lane (R4 = 16 >= 3M + G4).
+// M = 4 words.  The ring is filled in sets of M words per lane: NG = M / G4 consecutive
+// granules (G4 = 128/S words per stream per granule), loaded two groups ahead under one
+// fit check, one ring position and one validity marker per set whatever S is.  The fit
+// rule and the no-unwritten-slot bound are those of Lane with a set as the unit
+// (R4 = 16 >= 3M + M).  A set may run past the row's last granule: those loads read the
+// next row's words inside the window or return zeros past it (buffer range check), into
+// ring slots the fit check has shown to be free, and the lane never consumes them.
 template <int G4>
 struct QuadLane {
     static constexpr int R4 = 16;
     static constexpr uint32_t kRingWaveBytes = R4 * 64 * 4;
     static constexpr int M = 4;
-    static_assert(R4 >= 3 * M + G4, "ring too small");
     static constexpr int NG = M / G4;
-    static constexpr int kPre = (R4 * 3 / 4) / G4;
+    static_assert(R4 >= 3 * M + M, "ring too small");
+    static constexpr int kPreSets = (R4 * 3 / 4) / M;
 
     __amdgpu_buffer_rsrc_t rsrc;
-    uint32_t lane_off, K, gl, j;
+    uint32_t lane_off, K, gs, j;   // K granules in the row; gs next set to load
     uint32_t lo, hi;       // state
     uint32_t W;            // next unread word
     uint32_t waddr;        // LDS byte offset of that word within the ring array
     uint32_t wave_base;
-    Chunk<2 * G4> pa[NG], pb[NG];   // a chunk of 2*G4 u16 = G4 u32 words
-    uint32_t ga[NG], gb[NG];
-    uint32_t *ring, *dummy;
+    Chunk<2 * G4> pa[NG], pb[NG];   // a set: NG chunks of 2*G4 u16 = G4 u32 words
+    uint32_t sa, sb;                // their set index, or ~0 when not loaded
+    uint32_t *ring, *dummy;         // dummy: M slots (M*64 words), written, never read
     uint32_t lane;
 
     __device__ void init(const uint16_t *payload, uint32_t win_beg, uint32_t win_end, bool valid,
@@ -303,36 +307,44 @@ struct QuadLane {
         dummy = dummy_rows;
         lane = lane_in_wave;
         wave_base = wave * kRingWaveBytes;
-#pragma unroll
-        for (int i = 0; i < NG; i++) ga[i] = gb[i] = 0xFFFFFFFFu;
+        sa = sb = 0xFFFFFFFFu;
     }
 
     __device__ __forceinline__ uint32_t &ring_at(uint32_t w) {
         return ring[(wave_base >> 2) + w * 64 + lane];
     }
 
-    __device__ __forceinline__ void write_granule(const Chunk<2 * G4> &d, uint32_t g) {
-        uint32_t s = (g * G4) & (R4 - 1);
-        uint32_t *p = g != 0xFFFFFFFFu ? &ring_at(s) : dummy + lane;
+    // loads set `set` (granules set*NG ..): one byte offset, the chunks at +512 apart
+    __device__ __forceinline__ void load_set(Chunk<2 * G4> (&p)[NG], uint32_t base) {
 #pragma unroll
-        for (int i = 0; i < G4; i++) p[i * 64] = d.d[i];
+        for (int i = 0; i < NG; i++) p[i] = load_chunk<2 * G4>(rsrc, base + uint32_t(i) * 512u);
     }
 
-    __device__ __forceinline__ void issue(Chunk<2 * G4> &p, uint32_t &gp) {
-        bool ok = gl < K && (gl + 1) * G4 - j <= uint32_t(R4);
-        p = load_chunk<2 * G4>(rsrc, ok ? lane_off + gl * 512u : kOOB);
-        gp = ok ? gl : 0xFFFFFFFFu;
-        gl += ok ? 1u : 0u;
+    __device__ __forceinline__ void write_set(const Chunk<2 * G4> (&p)[NG], uint32_t set) {
+        uint32_t *q = set != 0xFFFFFFFFu ? &ring_at((set * M) & (R4 - 1)) : dummy + lane;
+#pragma unroll
+        for (int i = 0; i < NG; i++)
+#pragma unroll
+            for (int w = 0; w < G4; w++) q[(i * G4 + w) * 64] = p[i].d[w];
+    }
+
+    __device__ __forceinline__ void issue(Chunk<2 * G4> (&p)[NG], uint32_t &sp) {
+        const uint32_t g0 = gs * NG;
+        const bool ok = g0 < K && (gs + 1) * M - j <= uint32_t(R4);
+        load_set(p, ok ? lane_off + g0 * 512u : kOOB);
+        sp = ok ? gs : 0xFFFFFFFFu;
+        gs += ok ? 1u : 0u;
     }
 
     __device__ void prologue() {
-        Chunk<2 * G4> pre[kPre];
+        Chunk<2 * G4> pre[kPreSets][NG];
 #pragma unroll
-        for (uint32_t g = 0; g < uint32_t(kPre); g++)
-            pre[g] = load_chunk<2 * G4>(rsrc, g < K ? lane_off + g * 512u : kOOB);
+        for (uint32_t t = 0; t < uint32_t(kPreSets); t++)
+            load_set(pre[t], t * NG < K ? lane_off + t * NG * 512u : kOOB);
 #pragma unroll
-        for (uint32_t g = 0; g < uint32_t(kPre); g++) write_granule(pre[g], g < K ? g : 0xFFFFFFFFu);
-        gl = K < uint32_t(kPre) ? K : uint32_t(kPre);
+        for (uint32_t t = 0; t < uint32_t(kPreSets); t++) write_set(pre[t], t);
+        const uint32_t k_sets = (K + NG - 1) / NG;
+        gs = k_sets < uint32_t(kPreSets) ? k_sets : uint32_t(kPreSets);
     }
 
     __device__ void start() {
@@ -341,23 +353,17 @@ struct QuadLane {
         j = 2;
         W = ring_at(2);
         waddr = wave_base | (2u * 256u) | (lane * 4u);
-#pragma unroll
-        for (int i = 0; i < NG; i++) issue(pa[i], ga[i]);
-#pragma unroll
-        for (int i = 0; i < NG; i++) issue(pb[i], gb[i]);
+        issue(pa, sa);
+        issue(pb, sb);
     }
 
     __device__ __forceinline__ void refill_a() {
-#pragma unroll
-        for (int i = 0; i < NG; i++) write_granule(pa[i], ga[i]);
-#pragma unroll
-        for (int i = 0; i < NG; i++) issue(pa[i], ga[i]);
+        write_set(pa, sa);
+        issue(pa, sa);
     }
     __device__ __forceinline__ void refill_b() {
-#pragma unroll
-        for (int i = 0; i < NG; i++) write_granule(pb[i], gb[i]);
-#pragma unroll
-        for (int i = 0; i < NG; i++) issue(pb[i], gb[i]);
+        write_set(pb, sb);
+        issue(pb, sb);
     }
 
     __device__ __forceinline__ void end_group(uint32_t waddr_before) {
