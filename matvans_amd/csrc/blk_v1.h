@@ -79,7 +79,7 @@ struct V1Smem {
     uint16_t ring[kWaves * R * 64];
     uint32_t lut[MODE == 1 ? 1 : 2 << P];
     uint32_t rowtab[MODE == 1 ? kWaves * 384 : 1];   // per wave: f|bias|sym + scratch
-    uint16_t dummy[8 * 64];
+    uint16_t dummy[(P == 8 ? 8 : 16) * 64];   // a ring set: M words per lane
     int32_t part[WPR > 1 ? kWaves * NV : 1];
 };
 

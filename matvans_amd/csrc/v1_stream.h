@@ -76,22 +76,24 @@ struct Lane {
     static constexpr int kRingWords = R;
     static constexpr uint32_t kRingWaveBytes = R * 64 * 2;
     static constexpr int M = (P == 8) ? 8 : 16;     // max words per 16-step group
-    static_assert(R >= 3 * M + GW && (R & (R - 1)) == 0, "ring too small");
-    static constexpr int NG = M / GW;               // granules written per group
-    static constexpr int kPre = (R * 3 / 4) / GW;   // granules loaded by the prologue
+    static constexpr int NG = M / GW;               // granules per set of M words
+    // the ring is filled in sets of M words (NG granules) under one fit check, as the
+    // quad coder's: no-unwritten-slot bound R >= 3M + M
+    static_assert(R >= 4 * M && (R & (R - 1)) == 0, "ring too small");
+    static constexpr int kPreSets = (R * 3 / 4) / M;   // sets loaded by the prologue
 
     __amdgpu_buffer_rsrc_t rsrc;
     uint32_t lane_off;   // byte offset of this lane's slice in granule 0
     uint32_t K;          // granules of this lane's row (0 = inactive lane)
-    uint32_t gl;         // next granule to load
+    uint32_t gs;         // next set to load
     uint32_t x;          // state (rotated for P = 8)
     uint32_t j;          // index of the next unread word
     uint32_t W;          // that word
     uint32_t waddr;      // LDS byte offset of word j within the ring array
     uint32_t wave_base;
     Chunk<GW> pa[NG], pb[NG];
-    uint32_t ga[NG], gb[NG];
-    uint16_t *ring, *dummy;
+    uint32_t sa, sb;     // their set index, or ~0 when not loaded
+    uint16_t *ring, *dummy;   // dummy: M rows of 64 u16, written, never read
     uint32_t lane;
 
     // win_beg/win_end: wave-uniform u16 window holding every row of the wave;
@@ -116,41 +118,50 @@ struct Lane {
         dummy = dummy_rows;
         lane = lane_in_wave;
         wave_base = wave * kRingWaveBytes;
-#pragma unroll
-        for (int i = 0; i < NG; i++) ga[i] = gb[i] = 0xFFFFFFFFu;
+        sa = sb = 0xFFFFFFFFu;
     }
 
     __device__ __forceinline__ uint16_t &ring_at(uint32_t w) {
         return ring[(wave_base >> 1) + w * 64 + lane];
     }
 
-    // Granule g into its ring slots; g == ~0 writes the zeros of an out-of-range load to
-    // the wave's dummy rows instead, so the write is not behind a branch.
-    __device__ __forceinline__ void write_granule(const Chunk<GW> &d, uint32_t g) {
-        uint32_t s = (g * GW) & (kRingWords - 1);
-        uint16_t *p = g != 0xFFFFFFFFu ? &ring_at(s) : dummy + lane;
+    // a set's NG granules from one byte offset, +512 apart; granules past the row read the
+    // next row inside the window or zeros past it, into slots the fit check has freed, and
+    // are never consumed
+    __device__ __forceinline__ void load_set(Chunk<GW> (&p)[NG], uint32_t base) {
 #pragma unroll
-        for (int i = 0; i < GW / 2; i++) {
-            p[(2 * i) * 64] = static_cast<uint16_t>(d.d[i]);
-            p[(2 * i + 1) * 64] = static_cast<uint16_t>(d.d[i] >> 16);
-        }
+        for (int i = 0; i < NG; i++) p[i] = load_chunk<GW>(rsrc, base + uint32_t(i) * 512u);
     }
 
-    __device__ __forceinline__ void issue(Chunk<GW> &p, uint32_t &gp) {
-        bool ok = gl < K && (gl + 1) * GW - j <= uint32_t(kRingWords);
-        p = load_chunk<GW>(rsrc, ok ? lane_off + gl * 512u : kOOB);
-        gp = ok ? gl : 0xFFFFFFFFu;
-        gl += ok ? 1u : 0u;
+    // set ~0 (nothing loaded) writes to the dummy rows, so the write is not behind a branch
+    __device__ __forceinline__ void write_set(const Chunk<GW> (&p)[NG], uint32_t set) {
+        uint16_t *q = set != 0xFFFFFFFFu ? &ring_at((set * M) & (kRingWords - 1)) : dummy + lane;
+#pragma unroll
+        for (int g = 0; g < NG; g++)
+#pragma unroll
+            for (int i = 0; i < GW / 2; i++) {
+                q[(g * GW + 2 * i) * 64] = static_cast<uint16_t>(p[g].d[i]);
+                q[(g * GW + 2 * i + 1) * 64] = static_cast<uint16_t>(p[g].d[i] >> 16);
+            }
+    }
+
+    __device__ __forceinline__ void issue(Chunk<GW> (&p)[NG], uint32_t &sp) {
+        const uint32_t g0 = gs * NG;
+        const bool ok = g0 < K && (gs + 1) * M - j <= uint32_t(kRingWords);
+        load_set(p, ok ? lane_off + g0 * 512u : kOOB);
+        sp = ok ? gs : 0xFFFFFFFFu;
+        gs += ok ? 1u : 0u;
     }
 
     __device__ void prologue() {
-        Chunk<GW> pre[kPre];
+        Chunk<GW> pre[kPreSets][NG];
 #pragma unroll
-        for (uint32_t g = 0; g < uint32_t(kPre); g++)
-            pre[g] = load_chunk<GW>(rsrc, g < K ? lane_off + g * 512u : kOOB);
+        for (uint32_t t = 0; t < uint32_t(kPreSets); t++)
+            load_set(pre[t], t * NG < K ? lane_off + t * NG * 512u : kOOB);
 #pragma unroll
-        for (uint32_t g = 0; g < uint32_t(kPre); g++) write_granule(pre[g], g < K ? g : 0xFFFFFFFFu);
-        gl = K < uint32_t(kPre) ? K : uint32_t(kPre);
+        for (uint32_t t = 0; t < uint32_t(kPreSets); t++) write_set(pre[t], t);
+        const uint32_t k_sets = (K + NG - 1) / NG;
+        gs = k_sets < uint32_t(kPreSets) ? k_sets : uint32_t(kPreSets);
     }
 
     __device__ void start() {
@@ -159,23 +170,17 @@ struct Lane {
         j = 2;
         W = ring_at(2);
         waddr = wave_base | (2u * 128u) | (lane * 2u);
-#pragma unroll
-        for (int i = 0; i < NG; i++) issue(pa[i], ga[i]);
-#pragma unroll
-        for (int i = 0; i < NG; i++) issue(pb[i], gb[i]);
+        issue(pa, sa);
+        issue(pb, sb);
     }
 
     __device__ __forceinline__ void refill_a() {
-#pragma unroll
-        for (int i = 0; i < NG; i++) write_granule(pa[i], ga[i]);
-#pragma unroll
-        for (int i = 0; i < NG; i++) issue(pa[i], ga[i]);
+        write_set(pa, sa);
+        issue(pa, sa);
     }
     __device__ __forceinline__ void refill_b() {
-#pragma unroll
-        for (int i = 0; i < NG; i++) write_granule(pb[i], gb[i]);
-#pragma unroll
-        for (int i = 0; i < NG; i++) issue(pb[i], gb[i]);
+        write_set(pb, sb);
+        issue(pb, sb);
     }
 
     // ring byte offset of the next word; the add may carry into the wave-base bits, the
