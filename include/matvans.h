@@ -481,7 +481,8 @@ int matvans_batch_configure(matvans_batch *plan, const matvans_batch_config *cfg
 int matvans_batch_get_config(const matvans_batch *plan, matvans_batch_config *cfg);
 int matvans_batch_launch(const matvans_batch *plan, void *stream);
 /* Autotuner: times units_per_tile x order x blocks_per_cu x schedule over `reps` launches each,
- * keeps the fastest and returns it (and its ms per launch).  Synchronises. */
+ * in two passes (the second in reverse order), keeps the configuration with the fastest of
+ * its two timings and returns it (and that ms per launch).  Synchronises. */
 int matvans_batch_autotune(matvans_batch *plan, void *stream, uint32_t reps,
                            matvans_batch_config *best, float *best_ms);
 

@@ -585,8 +585,9 @@ extern "C" int matvans_batch_launch(const matvans_batch *p, void *stream) {
 }
 
 // Times every configuration of a small grid (schedule x units per tile x tile order x
-// blocks per CU) over `reps` launches each on `stream` and keeps the fastest.  Synchronises; not for
-// use inside graph capture.  No reference counterpart (the closest is the hand-set
+// blocks per CU) over `reps` launches each on `stream`, twice, and keeps the
+// configuration with the fastest of its two timings.  Synchronises; not for use inside
+// graph capture.  No reference counterpart (the closest is the hand-set
 // warpsPerBlock of matvansWarpPerRow.cu:156).
 extern "C" int matvans_batch_autotune(matvans_batch *p, void *stream, uint32_t reps,
                                       matvans_batch_config *best_out, float *best_ms) {
@@ -605,25 +606,34 @@ extern "C" int matvans_batch_autotune(matvans_batch *p, void *stream, uint32_t r
             for (uint32_t o : {0u, 1u, 2u})
                 for (uint32_t c : {4u, 8u}) grid.push_back({u, o, c, q});
     int st = MATVANS_OK;
-    for (const matvans_batch_config &cfg : grid) {
-        if ((st = matvans_batch_configure(p, &cfg))) goto done;
-        if ((st = matvans_batch_launch(p, stream))) goto done;   // warm-up
-        hipEventRecord(a, s);
-        for (uint32_t r = 0; r < reps && !st; r++) st = matvans_batch_launch(p, stream);
-        hipEventRecord(b, s);
-        if (st) goto done;
-        if ((e = hipEventSynchronize(b)) != hipSuccess) {
-            st = hip_fail(e, "sync");
-            goto done;
-        }
-        float ms = 0;
-        hipEventElapsedTime(&ms, a, b);
-        ms /= float(reps);
-        if (ms < best_t) {
-            best_t = ms;
-            best = cfg;
+    // two passes over the grid, the second in reverse order, keeping each configuration's
+    // faster pass: a transient slow measurement (clock ramp, a neighbour's burst) then
+    // does not decide the choice, and drift along the grid cancels
+    std::vector<float> t(grid.size(), 1e30f);
+    for (int pass = 0; pass < 2; pass++) {
+        for (size_t n = 0; n < grid.size(); n++) {
+            const size_t k = pass == 0 ? n : grid.size() - 1 - n;
+            const matvans_batch_config &cfg = grid[k];
+            if ((st = matvans_batch_configure(p, &cfg))) goto done;
+            if ((st = matvans_batch_launch(p, stream))) goto done;   // warm-up
+            hipEventRecord(a, s);
+            for (uint32_t r = 0; r < reps && !st; r++) st = matvans_batch_launch(p, stream);
+            hipEventRecord(b, s);
+            if (st) goto done;
+            if ((e = hipEventSynchronize(b)) != hipSuccess) {
+                st = hip_fail(e, "sync");
+                goto done;
+            }
+            float ms = 0;
+            hipEventElapsedTime(&ms, a, b);
+            t[k] = std::min(t[k], ms / float(reps));
         }
     }
+    for (size_t k = 0; k < grid.size(); k++)
+        if (t[k] < best_t) {
+            best_t = t[k];
+            best = grid[k];
+        }
 done:
     hipEventDestroy(a);
     hipEventDestroy(b);
