@@ -60,12 +60,13 @@ int matvans_decompress_and_multiply(int32_t *dst, const int8_t *vec, uint32_t ro
                                     const uint16_t *payload, uint32_t payload_size,
                                     void *stream);
 
-/* Decode kernel variant used by the launch entry points (all give the same bits):
- * 0 = reference form of the step, 1 = rotated state with dword tables (default),
- * 2 = rotated state with three conflict-free 256-byte tables, 3 = rotated state with one
- * packed table dword per quantile; -1 restores the default
- * (or MATVANS_DECODE_VARIANT).  No reference counterpart: a tuning control for
- * measurements and the autotuner.  Process-wide. */
+/* Decode kernel variant of the .bin v0 launch entry points (all give the same bits):
+ * 0 = reference form of the step, 1 = rotated state with dword tables, 2 = rotated state
+ * with three conflict-free 256-byte tables, 3 = rotated state with one packed table dword
+ * per quantile.  By default each launch picks 1 or 3 by the waves per SIMD of its grid
+ * (3 from four on); set 0..3 to force one, -1 to restore the default (or
+ * MATVANS_DECODE_VARIANT).  get returns the forced variant, -1 when automatic.  No
+ * reference counterpart: a tuning control for measurements.  Process-wide. */
 int matvans_set_decode_variant(int variant);
 int matvans_get_decode_variant(void);
 

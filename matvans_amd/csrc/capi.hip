@@ -21,20 +21,46 @@ int hip_status(hipError_t e, const char *what) {
 }
 
 // Decode kernel variant, all computing the same bits: 0 = reference form of the step,
-// 1 = rotated state with dword tables (default: the 32-stream layout leaves few waves
-// per SIMD at long rows, so the shorter dependent chain of the read2 form wins),
-// 2 = rotated state with three conflict-free byte tables, 3 = rotated state with one
-// packed dword table (half the LDS accesses, one more dependent op).
-// MATVANS_DECODE_VARIANT overrides the default for A/B runs.
-std::atomic<int> g_variant{-1};
+// 1 = rotated state with dword tables read as one pair, 2 = rotated state with three
+// conflict-free byte tables, 3 = rotated state with one packed dword table (half the LDS
+// accesses, one more dependent op).  Unless one is forced (matvans_set_decode_variant, or
+// MATVANS_DECODE_VARIANT for A/B runs), each launch picks by the waves per SIMD its grid
+// gives: with few (long rows: 4096 x 14336 has two per SIMD) the step's dependent chain
+// decides and 1 wins; from four on the instruction count decides and 3 wins (14336 x 4096,
+// seven per SIMD: 29.8 vs 33.0 us).
+std::atomic<int> g_variant{-2};   // -2 not read yet, -1 automatic, 0..3 forced
 
-int decode_variant() {
+int forced_variant() {
     int v = g_variant.load(std::memory_order_relaxed);
-    if (v >= 0) return v;
+    if (v != -2) return v;
     const char *e = std::getenv("MATVANS_DECODE_VARIANT");
-    v = (e && e[0] >= '0' && e[0] <= '3' && e[1] == 0) ? e[0] - '0' : 1;
+    v = (e && e[0] >= '0' && e[0] <= '3' && e[1] == 0) ? e[0] - '0' : -1;
     g_variant.store(v, std::memory_order_relaxed);
     return v;
+}
+
+// Compute units of the current device, read once per device.
+uint32_t current_device_cus() {
+    static std::atomic<uint32_t> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    uint32_t n = cache[dev].load(std::memory_order_relaxed);
+    if (n == 0) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            v <= 0)
+            v = 256;
+        n = uint32_t(v);
+        cache[dev].store(n, std::memory_order_relaxed);
+    }
+    return n;
+}
+
+int v0_variant_for(uint32_t rows) {
+    const int v = forced_variant();
+    if (v >= 0) return v;
+    const uint64_t waves = (uint64_t(rows) + 1) / 2;   // two rows per wave
+    return waves >= 4ull * 4ull * current_device_cus() ? 3 : 1;
 }
 
 int check_v0(const int32_t *dst, const int8_t *vec, uint32_t n_vec, uint32_t vec_stride,
@@ -57,12 +83,12 @@ int check_v0(const int32_t *dst, const int8_t *vec, uint32_t n_vec, uint32_t vec
 
 extern "C" int matvans_set_decode_variant(int variant) {
     if (variant < -1 || variant > 3) return fail(MATVANS_E_INVALID, "variant %d outside -1..3", variant);
-    g_variant.store(variant, std::memory_order_relaxed);
+    g_variant.store(variant == -1 ? -2 : variant, std::memory_order_relaxed);
     mv::clear_error();
     return MATVANS_OK;
 }
 
-extern "C" int matvans_get_decode_variant(void) { return decode_variant(); }
+extern "C" int matvans_get_decode_variant(void) { return forced_variant(); }
 
 extern "C" int matvans_decompress_and_multiply(int32_t *dst, const int8_t *vec, uint32_t rows,
                                                uint32_t cols, float grid_spacing,
@@ -90,7 +116,7 @@ extern "C" int matvans_decompress_and_multiply_nvec(int32_t *dst, const int8_t *
                       payload);
     if (st) return st;
     mv::V0Args a{dst, vecs, n_vec, vec_stride, rows, cols, cursors, min_value, G,
-                 cdf_data, ppf_data, payload, payload_size, decode_variant()};
+                 cdf_data, ppf_data, payload, payload_size, v0_variant_for(rows)};
     return hip_status(mv::launch_v0_decode_matvec(a, as_stream(stream)), "decode_matvec launch");
 }
 
