@@ -1,5 +1,5 @@
 // Backward of the compressed linear layer on gfx950, from the ".bin v0" streams and from
-// the MVX1 quad-coder streams (32 or 64 per row):
+// the MVX1 quad-coder streams (32, 64 or 128 per row):
 //
 //   dx[v][c] = alpha[v] * sum_r W[r, c] * (s_w[r] * g[v][r])
 //
@@ -104,6 +104,7 @@ hipError_t launch_quad_backward(const QuadBwdArgs &a, hipStream_t s) {
              a.cursors, a.payload, a.payload_size, 0, 0, nullptr, nullptr, a.lut};
     if (a.streams == 32) return quad_bwd_s<32>(j, s);
     if (a.streams == 64) return quad_bwd_s<64>(j, s);
+    if (a.streams == 128) return quad_bwd_s<128>(j, s);
     return hipErrorInvalidValue;
 }
 

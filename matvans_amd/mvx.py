@@ -124,7 +124,7 @@ class CompressedMatrixV1:
 
     def backward(self, g, s_w, alpha=None, out=None, workspace=None, stream=None):
         """dx[v] = alpha[v] * W^T (s_w * g[v]) re-decoded from the quad-coder streams
-        (coder 1, 32 or 64 streams), g [rows] or [n_vec, rows] fp32 (n_vec <= 8); bitwise
+        (coder 1, 32, 64 or 128 streams), g [rows] or [n_vec, rows] fp32 (n_vec <= 8); bitwise
         reproducible."""
         import torch
         s = self.dev_struct()

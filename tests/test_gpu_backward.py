@@ -20,15 +20,15 @@ def note(msg):
     sys.stdout.flush()
 
 
-FORMATS = ["v0", "q32", "q64"]
+FORMATS = ["v0", "q32", "q64", "q128"]
 
 
 def encode(fmt, w):
     """The matrix on the device in one of the encodings with a backward: .bin v0, or the
-    MVX1 quad coder with 32 / 64 streams."""
+    MVX1 quad coder with 32 / 64 / 128 streams."""
     if fmt == "v0":
         return mv.CompressedMatrix.encode(w, n_threads=16).to_device()
-    S = 32 if fmt == "q32" else 64
+    S = int(fmt[1:])
     if w.shape[1] % (16 * S):
         pytest.skip(f"cols {w.shape[1]} not a multiple of {16 * S}")
     return CompressedMatrixV1.encode(w, S, 8, 0, coder=1).to_device()
