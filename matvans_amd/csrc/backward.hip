@@ -5,9 +5,9 @@
 //
 // W is re-decoded from the same row-ordered rANS streams the forward reads; no dense
 // copy is ever built.  The reference has no backward pass.  The block bodies (stream-major
-// decode, fixed-order 64-row butterfly, fixed-order row-block reduce) are in blk_bwd.h;
-// this file holds the one-matrix launches: a grid of (row blocks, stream groups) units,
-// then one reduce kernel.
+// decode, fixed-order 64-row pair steps and row reductions, fixed-order row-block reduce)
+// are in blk_bwd.h; this file holds the one-matrix launches: a grid of (row blocks,
+// stream groups) units, then one reduce kernel.
 
 #include <hip/hip_runtime.h>
 

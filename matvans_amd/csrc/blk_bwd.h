@@ -201,7 +201,7 @@ struct BwdLane {
 // A[l%16][l/16] = W_g[4s + l/16][l%16] (read back from the wave's LDS stage) and
 // B[l/16][l%16] = gs[4s + l/16][v = l%16] (MfmaGrad); the result D[i][j] = out[i][j] is
 // held by lane 16 (i/4) + j, register i%4, so each lane stores four consecutive columns of
-// one vector.  The hardware's summation order is fixed: deterministic like the butterfly.
+// one vector.  The hardware's summation order is fixed: deterministic like the pair steps.
 struct MfmaGrad {
     float b[16];   // slab s: s_w[r] g[l%16][r], r = 4s + l/16 (0 past the rows / n_vec)
     __device__ __forceinline__ void init(const BwdJob &J, uint32_t rb, uint32_t lane) {
