@@ -406,7 +406,8 @@ int matvans_st_tensor(const matvans_st *st, const char *name, const char **dtype
  * ----------------------------------------------------------------------------*/
 enum {
     MATVANS_BATCH_V0 = 0,        /* .bin v0 record: cursors, cdf, ppf, min_value, G, payload */
-    MATVANS_BATCH_QUAD = 1,      /* MVX1 quad coder, 32 or 64 streams: cursors, lut, payload */
+    MATVANS_BATCH_QUAD = 1,      /* MVX1 quad coder, 32, 64 or 128 streams: cursors, lut,
+                                    payload (batched backward: 32 or 64) */
     MATVANS_BATCH_B_ROWS = 2,    /* family-B ans-warp-rows */
     MATVANS_BATCH_B_STRIPES = 3  /* family-B ans-warp-stripes */
 };
@@ -425,7 +426,7 @@ typedef struct {
     int32_t min_value;           /* v0 */
     uint32_t G;                  /* v0 */
     const uint8_t *cdf, *ppf;    /* v0 */
-    uint32_t streams;            /* quad: 32 or 64 */
+    uint32_t streams;            /* quad: 32, 64 or 128 */
     const uint32_t *lut;         /* quad: the 2 * 256 dwords of matvans_v1_lut */
     /* family B (as matvans_b_rows_matvec) */
     const uint32_t *compressed;

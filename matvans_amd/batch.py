@@ -23,7 +23,7 @@ def _p(t):
 class Batch:
     """entries: list of (matrix, vecs), (matrix, vecs, scale) or (matrix, vecs, scale,
     int32) where matrix is a CompressedMatrix (.bin v0), a CompressedMatrixV1 with
-    coder=1 and 32 or 64 streams, or a (BufferedMatrices, k) pair; vecs is an int8 device
+    coder=1 and 32, 64 or 128 streams, or a (BufferedMatrices, k) pair; vecs is an int8 device
     tensor [n_vec, cols] (or [cols]).  All matrices must be on the device.
     ``outputs[i]`` is the int32 [n_vec, rows] result of entry i (None when int32 is
     False); ``out8[i]`` the next layer's int8 input clamp(rint(float(acc) * scale)),
@@ -176,9 +176,9 @@ class BatchBackward(Batch):
         self._h = h
 
 
-def tune(matrices, vecs, encodings=("q32", "q64", "v0"), reps=5, stream=None):
+def tune(matrices, vecs, encodings=("q32", "q64", "q128", "v0"), reps=5, stream=None):
     """Pick the encoding (streams per row, and with it rows per wavefront: two rows per
-    wave at 32 streams, one at 64) and the plan configuration (tile size, tile order,
+    wave at 32 streams, one at 64, half a row at 128) and the plan configuration (tile size, tile order,
     persistent grid, schedule) for a batch of raw int8 matrices.  matrices: int8 numpy
     arrays [rows, cols]; vecs: int8 device tensors [n_vec, cols], one per matrix.  Each
     candidate encoding is encoded on the host, uploaded, and its plan autotuned; the
@@ -191,8 +191,8 @@ def tune(matrices, vecs, encodings=("q32", "q64", "v0"), reps=5, stream=None):
     for enc in encodings:
         if enc == "v0":
             mats = [CompressedMatrix.encode(m).to_device(dev) for m in matrices]
-        elif enc in ("q32", "q64"):
-            S = 32 if enc == "q32" else 64
+        elif enc in ("q32", "q64", "q128"):
+            S = int(enc[1:])
             if any(m.shape[1] % (16 * S) for m in matrices):
                 continue
             mats = [CompressedMatrixV1.encode(m, S, 8, 0, coder=1).to_device(dev)

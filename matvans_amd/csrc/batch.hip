@@ -37,8 +37,10 @@ using mv::fail;
 namespace mv {
 namespace {
 
-// decode path of an item: v0, quad with 32 / 64 streams, family-B rows / stripes
-enum : uint32_t { kPathV0 = 0, kPathQ32 = 1, kPathQ64 = 2, kPathBRows = 3, kPathBStripes = 4 };
+// decode path of an item: v0, quad with 32 / 64 / 128 streams, family-B rows / stripes
+enum : uint32_t {
+    kPathV0 = 0, kPathQ32 = 1, kPathQ64 = 2, kPathBRows = 3, kPathBStripes = 4, kPathQ128 = 5
+};
 
 struct Item {
     uint32_t path;
@@ -56,15 +58,17 @@ union BatchSmem {
     blk::V1Smem<NV, 32, 8, 0> v0;
     blk::QuadSmem<NV, 32> q32;
     blk::QuadSmem<NV, 64> q64;
+    blk::QuadSmem<NV, 128> q128;
     blk::BSmem b;
 };
 
 constexpr uint32_t bit(uint32_t path) { return 1u << path; }
 // kernel instances: the decode paths each one contains (a plan picks the smallest one
 // that covers its items; fewer paths leave fewer registers live)
-constexpr uint32_t kMasks[] = {bit(kPathV0), bit(kPathQ32), bit(kPathQ64),
-                               bit(kPathQ32) | bit(kPathQ64),
-                               bit(kPathBRows) | bit(kPathBStripes), 0x1Fu};
+constexpr uint32_t kQuad = bit(kPathQ32) | bit(kPathQ64) | bit(kPathQ128);
+constexpr uint32_t kMasks[] = {bit(kPathV0), bit(kPathQ32), bit(kPathQ64), bit(kPathQ128),
+                               bit(kPathQ32) | bit(kPathQ64), kQuad,
+                               bit(kPathBRows) | bit(kPathBStripes), 0x3Fu};
 
 // work-queue shards: block b draws from shard b % kShards, whose tiles are t = g, g + 8,
 // g + 16, ...; each shard's counter sits on its own 128-byte line
@@ -102,6 +106,8 @@ __global__ __launch_bounds__(256) void batch_fwd(const Item *__restrict__ items,
                 blk::quad_block<NV, 32>(it.v1, bid, sm.q32);
             else if ((MASK & bit(kPathQ64)) && path == kPathQ64)
                 blk::quad_block<NV, 64>(it.v1, bid, sm.q64);
+            else if ((MASK & bit(kPathQ128)) && path == kPathQ128)
+                blk::quad_block<NV, 128>(it.v1, bid, sm.q128);
             else if ((MASK & bit(kPathBRows)) && path == kPathBRows)
                 blk::b_block<NV, 0>(it.b, bid, sm.b);
             else if ((MASK & bit(kPathBStripes)) && path == kPathBStripes)
@@ -284,8 +290,9 @@ int check_item(const matvans_batch_item &it, uint32_t k) {
                 return fail(MATVANS_E_INVALID, "item %u: null v0 arrays", k);
             break;
         case MATVANS_BATCH_QUAD:
-            if (it.streams != 32 && it.streams != 64)
-                return fail(MATVANS_E_UNSUPPORTED, "item %u: quad items need 32 or 64 streams", k);
+            if (it.streams != 32 && it.streams != 64 && it.streams != 128)
+                return fail(MATVANS_E_UNSUPPORTED,
+                            "item %u: quad items need 32, 64 or 128 streams", k);
             if (it.cols % (16 * it.streams)) return fail(MATVANS_E_SHAPE, "item %u: cols", k);
             if (!it.cursors || !it.lut || !it.payload)
                 return fail(MATVANS_E_INVALID, "item %u: null quad arrays", k);
@@ -323,8 +330,9 @@ int nv_row(uint32_t nv) { return nv == 1 ? 0 : nv == 2 ? 1 : nv == 4 ? 2 : 3; }
 FwdKernel fwd_kernel(const matvans_batch *p) {
 #define MV_ROW(NV) {mv::batch_fwd<NV, mv::kMasks[0]>, mv::batch_fwd<NV, mv::kMasks[1]>, \
                     mv::batch_fwd<NV, mv::kMasks[2]>, mv::batch_fwd<NV, mv::kMasks[3]>, \
-                    mv::batch_fwd<NV, mv::kMasks[4]>, mv::batch_fwd<NV, mv::kMasks[5]>}
-    static const FwdKernel table[4][6] = {MV_ROW(1), MV_ROW(2), MV_ROW(4), MV_ROW(8)};
+                    mv::batch_fwd<NV, mv::kMasks[4]>, mv::batch_fwd<NV, mv::kMasks[5]>, \
+                    mv::batch_fwd<NV, mv::kMasks[6]>, mv::batch_fwd<NV, mv::kMasks[7]>}
+    static const FwdKernel table[4][8] = {MV_ROW(1), MV_ROW(2), MV_ROW(4), MV_ROW(8)};
 #undef MV_ROW
     return table[nv_row(p->nv)][p->kernel];
 }
@@ -394,8 +402,10 @@ extern "C" int matvans_batch_create(const matvans_batch_item *items, uint32_t n,
                 m.units = (it.rows + 7) / 8;
                 p->chain.push_back(it.cols / 32);
             } else {
-                m.path = it.streams == 32 ? mv::kPathQ32 : mv::kPathQ64;
-                m.units = it.streams == 32 ? (it.rows + 7) / 8 : (it.rows + 3) / 4;
+                m.path = it.streams == 32 ? mv::kPathQ32
+                         : it.streams == 64 ? mv::kPathQ64 : mv::kPathQ128;
+                const uint32_t rpb = 256u / it.streams;   // rows per 4-wave block: 8, 4, 2
+                m.units = (it.rows + rpb - 1) / rpb;
                 p->chain.push_back(it.cols / it.streams / 4);
             }
             m.v1 = mv::blk::V1Job{it.dst, it.vecs, stride, it.n_vec, it.rows, it.cols, it.cursors,
@@ -425,7 +435,7 @@ extern "C" int matvans_batch_create(const matvans_batch_item *items, uint32_t n,
     if (e == hipSuccess) e = hipMalloc(&p->d_counters, counter_bytes);
     if (e == hipSuccess) e = hipMemset(p->d_counters, 0, counter_bytes);
     p->max_tiles = max_tiles;
-    for (uint32_t i = 0; i < 6; i++)
+    for (uint32_t i = 0; i < sizeof(mv::kMasks) / sizeof(mv::kMasks[0]); i++)
         if ((mv::kMasks[i] & p->mask) == p->mask) { p->kernel = i; break; }
     if (e == hipSuccess) e = set_max_bpc(p);
     if (e == hipSuccess) e = build_tiles(p);

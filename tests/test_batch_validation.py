@@ -45,7 +45,7 @@ def create(fn, items):
     ({"n_vec": 9}, 1),                       # more than 8 vectors
     ({"n_vec": 0}, 1),
     ({"rows": 0}, 2),                        # empty matrix
-    ({"streams": 128}, 8),                   # quad items: 32 or 64 streams
+    ({"streams": 96}, 8),                    # quad items: 32, 64 or 128 streams
     ({"cols": 1000}, 2),                     # cols not a multiple of 16 * streams
     ({"payload": 18}, 3),                    # payload not 16-byte aligned
     ({"vecs": 20}, 3),                       # vectors not 16-byte aligned
