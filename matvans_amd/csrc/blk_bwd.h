@@ -5,10 +5,10 @@
 // Mapping (stream-major): a unit is 8 waves = 8 consecutive streams of 64 consecutive
 // rows; a wave decodes one stream of the 64 rows, one row per lane.  A stream covers the
 // same columns in every row, so at each decode step all 64 lanes contribute to the same
-// columns.  Every four columns (one decode quad, its int8 symbols packed in one dword) a
-// fixed pair step (bwd_pair) folds the 64 rows to 16 lanes per column: one
-// v_permlane32_swap, a multiply and an fma per pair of rows, one v_permlane16_swap and an
-// add.  After a group's four quads, bwd_group_reduce sums each quad's 16 lanes for all
+// columns.  Every four columns (one decode quad) a fixed pair step (bwd_pair) folds the
+// 64 rows to 16 lanes per column: the quad's symbols as two column pairs (two byte
+// permutes of its table entries), one v_permlane32_swap, a multiply and an fma per pair
+// of rows, one v_permlane16_swap and an add.  After a group's four quads, bwd_group_reduce sums each quad's 16 lanes for all
 // four quads at once (a transposed reduction over DPP rows: row_mirror, row_half_mirror
 // and two quad_perm adds), in the same order on every run.  The group's 16 totals are
 // stored once, to partial[v][row_block][c]; reduce_tile sums the row blocks in a fixed
@@ -106,17 +106,24 @@ struct BwdRows {
     }
 };
 
-// Pair step of one decode quad: packed holds the four int8 symbols of this lane's row (in
-// column order), gsA / gsB the scaled gradients of rows (lane & 31) and (lane & 31) + 32.
-// Swapping packed against packed >> 8 across the wave halves leaves, in bytes 0 and 2,
-// columns (0, 2) in lanes 0..31 and (1, 3) in lanes 32..63, of row (lane & 31) in X and of
-// row (lane & 31) + 32 in Y.  a, b are those two columns' pair sums; v_permlane16_swap +
-// add leaves the 16-lane rows k = 0..3 of the wave with columns (0, 2, 1, 3), each lane
-// holding the sum over four rows.
+// The symbols of a decode quad's four table entries e[0..3] (symbol in byte 3, column
+// order) as the two inputs of the pair step: v1 = columns 0 and 2 in bytes 0 and 2,
+// v2 = columns 1 and 3 in bytes 0 and 2.
+__device__ __forceinline__ void quad_cols(const uint32_t (&e)[4], uint32_t &v1, uint32_t &v2) {
+    v1 = __builtin_amdgcn_perm(e[2], e[0], 0x0c070c03u);
+    v2 = __builtin_amdgcn_perm(e[3], e[1], 0x0c070c03u);
+}
+
+// Pair step of one decode quad of this lane's row (v1, v2 from quad_cols), gsA / gsB the
+// scaled gradients of rows (lane & 31) and (lane & 31) + 32.  Swapping v1 against v2
+// across the wave halves leaves, in bytes 0 and 2, columns (0, 2) in lanes 0..31 and
+// (1, 3) in lanes 32..63, of row (lane & 31) in X and of row (lane & 31) + 32 in Y.  a, b
+// are those two columns' pair sums; v_permlane16_swap + add leaves the 16-lane rows
+// k = 0..3 of the wave with columns (0, 2, 1, 3), each lane holding the sum over four rows.
 template <int NV>
-__device__ __forceinline__ void bwd_pair(uint32_t packed, const float (&gsA)[NV],
+__device__ __forceinline__ void bwd_pair(uint32_t v1, uint32_t v2, const float (&gsA)[NV],
                                          const float (&gsB)[NV], float (&s)[NV]) {
-    auto sw = __builtin_amdgcn_permlane32_swap(packed, packed >> 8, false, false);
+    auto sw = __builtin_amdgcn_permlane32_swap(v1, v2, false, false);
     const uint32_t X = sw[0], Y = sw[1];
     const float xa = cvt_f32_sbyte0(X), xb = cvt_f32_sbyte2(X);
     const float ya = cvt_f32_sbyte0(Y), yb = cvt_f32_sbyte2(Y);
@@ -176,20 +183,24 @@ struct BwdLane {
                 if (v < int(n_vec)) out[size_t(v) * plane + col] = tot[v];
         }
     }
-    // a group's four packed quads: pair steps, then the transposed row reduction
+    // a group's four quads (next(v1, v2) decodes one, see quad_cols): pair steps, then
+    // the transposed row reduction
     template <class Next>
     __device__ __forceinline__ void group(Next next, uint32_t lane, float (&tot)[NV]) const {
         float s[4][NV];
         // quad d's pair step follows quad d+1's decode: independent work between the
         // decode's dependent steps and the swaps' operand hazards
-        uint32_t prev = next();
+        uint32_t p1, p2;
+        next(p1, p2);
 #pragma unroll
         for (int d = 1; d < 4; d++) {
-            const uint32_t cur = next();
-            bwd_pair<NV>(prev, gsA, gsB, s[d - 1]);
-            prev = cur;
+            uint32_t c1, c2;
+            next(c1, c2);
+            bwd_pair<NV>(p1, p2, gsA, gsB, s[d - 1]);
+            p1 = c1;
+            p2 = c2;
         }
-        bwd_pair<NV>(prev, gsA, gsB, s[3]);
+        bwd_pair<NV>(p1, p2, gsA, gsB, s[3]);
         bwd_group_reduce<NV>(s, lane, tot);
     }
 };
@@ -283,7 +294,14 @@ __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32
                            J.n_vec);
         } else {
             float tot[NV];
-            L.group(next4, lane, tot);
+            L.group(
+                [&](uint32_t &v1, uint32_t &v2) {
+                    uint32_t e[4];
+                    st.pair_packed(sm.lut, e[0], e[1]);
+                    st.pair_packed(sm.lut, e[2], e[3]);
+                    quad_cols(e, v1, v2);
+                },
+                lane, tot);
             L.store(tot, base + t * 512u, plane, J.n_vec);
         }
         st.end_group(wbefore);
@@ -341,7 +359,13 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
                            J.n_vec);
         } else {
             float tot[NV];
-            L.group([&]() { return st.quad(sm.tab); }, lane, tot);
+            L.group(
+                [&](uint32_t &v1, uint32_t &v2) {
+                    uint32_t e[4];
+                    st.quad_entries(sm.tab, e);
+                    quad_cols(e, v1, v2);
+                },
+                lane, tot);
             L.store(tot, base + t * (16u * S), plane, J.n_vec);
         }
         st.end_group(wbefore);

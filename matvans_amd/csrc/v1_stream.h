@@ -381,13 +381,21 @@ struct QuadLane {
     // x = hi * f0 f1 f2 f3 + (r0 + f0 (r1 + f1 (r2 + f2 r3))), with the byte fields taken
     // by SDWA operand selects.  Returns the four symbols packed in column order.
     __device__ __forceinline__ uint32_t quad(const uint32_t *tab) {
+        uint32_t e[4];
+        quad_entries(tab, e);
+        return __builtin_amdgcn_perm(e[1], e[0], 0x0c0c0703u) |
+               __builtin_amdgcn_perm(e[3], e[2], 0x07030c0cu);
+    }
+
+    // The same step, returning the four table entries (symbol in byte 3 of each, in
+    // column order) instead of the packed symbols.
+    __device__ __forceinline__ void quad_entries(const uint32_t *tab, uint32_t (&e)[4]) {
         bool need = hi == 0;
         hi = need ? lo : hi;
         lo = need ? W : lo;
         uint32_t t = waddr + (need ? 256u : 0u);
         waddr = (t & (kRingWaveBytes - 1)) | (wave_base & ~(kRingWaveBytes - 1));
         W = *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(ring) + waddr);
-        uint32_t e[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) e[i] = tab[(lo >> (8 * i)) & 0xFFu];
         uint32_t f01 = mul_b0_b0(e[0], e[1]);
@@ -399,8 +407,6 @@ struct QuadLane {
         uint64_t nx = uint64_t(hi) * F + R;
         lo = uint32_t(nx);
         hi = uint32_t(nx >> 32);
-        return __builtin_amdgcn_perm(e[1], e[0], 0x0c0c0703u) |
-               __builtin_amdgcn_perm(e[3], e[2], 0x07030c0cu);
     }
 };
 
