@@ -188,22 +188,27 @@ __global__ __launch_bounds__(512) void batch_bwd(const BwdItem *__restrict__ ite
         const Tile tl = tiles[g + kShards * pos];
         const BwdItem &it = items[tl.item];
         const uint32_t path = __builtin_amdgcn_readfirstlane(it.path);
-        const uint32_t sgs = __builtin_amdgcn_readfirstlane(it.sgs);
         const uint32_t nrb = __builtin_amdgcn_readfirstlane(it.nrb);
+        // a tile's units are consecutive row blocks of one stream group (units are
+        // numbered stream-group-major): the first builds the table and synchronises, the
+        // others reuse it with no barrier
         for (uint32_t u = 0; u < tl.count; u++) {
             const uint32_t unit = tl.first + u;
-            const uint32_t rb = unit / sgs, sg = unit % sgs;
+            const uint32_t sg = unit / nrb, rb = unit % nrb;
+            const bool fresh = u == 0;
             if constexpr ((MASK & bit(kBwdV0)) != 0) {
-                if (path == kBwdV0) blk::v0_bwd_unit<NV>(it.job, rb, sg, nrb, sm.v0);
+                if (path == kBwdV0) blk::v0_bwd_unit<NV>(it.job, rb, sg, nrb, sm.v0, fresh);
             }
             if constexpr ((MASK & bit(kBwdQ32)) != 0) {
-                if (path == kBwdQ32) blk::quad_bwd_unit<NV, 32>(it.job, rb, sg, nrb, sm.q32);
+                if (path == kBwdQ32)
+                    blk::quad_bwd_unit<NV, 32>(it.job, rb, sg, nrb, sm.q32, fresh);
             }
             if constexpr ((MASK & bit(kBwdQ64)) != 0) {
-                if (path == kBwdQ64) blk::quad_bwd_unit<NV, 64>(it.job, rb, sg, nrb, sm.q64);
+                if (path == kBwdQ64)
+                    blk::quad_bwd_unit<NV, 64>(it.job, rb, sg, nrb, sm.q64, fresh);
             }
-            __syncthreads();   // the next unit rebuilds the tables and rings
         }
+        __syncthreads();   // the next tile rebuilds the table
         if (queue) {
             if (tid == 0) {
                 const uint32_t k = atomicAdd(counter, 1u);

@@ -246,13 +246,16 @@ __device__ __forceinline__ void bwd_group_mfma(const uint32_t (&P)[4], const flo
             make_float4(acc[0], acc[1], acc[2], acc[3]);
 }
 
-// .bin v0 unit: row block rb, streams 8 sg .. 8 sg + 7.
+// .bin v0 unit: row block rb, streams 8 sg .. 8 sg + 7.  fresh (block-uniform): build the
+// table and synchronise the block first; a unit that follows one of the same matrix in
+// the same block reuses the table, and its waves go on without a barrier (each wave's
+// ring and stage are its own).
 template <int NV>
 __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32_t sg,
-                                            uint32_t nrb, V0BwdSmem<NV> &sm) {
+                                            uint32_t nrb, V0BwdSmem<NV> &sm, bool fresh = true) {
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid >> 6, lane = tid & 63;
-    if (tid < 256) {   // packed table f | bias << 8 | sym << 24
+    if (fresh && tid < 256) {   // packed table f | bias << 8 | sym << 24
         uint32_t f, sym, bias;
         v0_table_entry(tid, J.min_value, J.G, J.cdf, J.ppf, f, sym, bias);
         sm.lut[tid] = f | (bias << 8) | (sym << 24);
@@ -265,7 +268,7 @@ __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32
     st.init(J.payload, R.wbeg, R.wend, R.valid, R.rcur, R.rend, stream, sm.ring, sm.dummy, wave,
             lane);
     st.prologue();
-    __syncthreads();
+    if (fresh) __syncthreads();
     st.start();
 
     const uint32_t cols = J.cols;
@@ -319,16 +322,18 @@ __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32
     }
 }
 
-// Quad-coder unit (S = 32 or 64 streams): row block rb, streams 8 sg .. 8 sg + 7.
+// Quad-coder unit (S = 32, 64 or 128 streams): row block rb, streams 8 sg .. 8 sg + 7;
+// fresh as for v0_bwd_unit.
 template <int NV, int S>
 __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint32_t sg,
-                                              uint32_t nrb, QuadBwdSmem<NV, S> &sm) {
+                                              uint32_t nrb, QuadBwdSmem<NV, S> &sm,
+                                              bool fresh = true) {
     using QL = v1::QuadLane<128 / S>;
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid >> 6, lane = tid & 63;
     // packed table T[q] = f | r << 8 | sym << 24 from the A / B tables (A = f | sym << 24,
     // B = r)
-    if (tid < 256)
+    if (fresh && tid < 256)
         sm.tab[tid] = (J.glut[tid] & 0xFF0000FFu) | ((J.glut[256 + tid] & 0xFFu) << 8);
     const uint32_t stream = sg * kBwdWaves + wave;   // 0..S-1
     BwdRows<NV> R;
@@ -338,7 +343,7 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
     st.init(J.payload, R.wbeg, R.wend, R.valid, R.rcur, R.rend, stream, sm.ring, sm.dummy, wave,
             lane);
     st.prologue();
-    __syncthreads();
+    if (fresh) __syncthreads();
     st.start();
 
     const uint32_t cols = J.cols;
