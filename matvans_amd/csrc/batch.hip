@@ -552,13 +552,13 @@ extern "C" int matvans_batch_create_backward(const matvans_batch_bwd_item *items
 
 extern "C" void matvans_batch_destroy(matvans_batch *p) {
     if (!p) return;
-    hipFree(p->d_items);
-    hipFree(p->d_bitems);
-    hipFree(p->d_tiles);
-    hipFree(p->d_rtiles);
-    hipFree(p->d_counters);
-    hipFree(p->d_tables);
-    hipFree(p->d_partial);
+    // release what the plan owns; a failure here (a device already in error) has no
+    // caller to report to
+    for (void *d : {static_cast<void *>(p->d_items), static_cast<void *>(p->d_bitems),
+                    static_cast<void *>(p->d_tiles), static_cast<void *>(p->d_rtiles),
+                    static_cast<void *>(p->d_counters), static_cast<void *>(p->d_tables),
+                    static_cast<void *>(p->d_partial)})
+        (void)hipFree(d);
     delete p;
 }
 
@@ -611,8 +611,11 @@ extern "C" int matvans_batch_autotune(matvans_batch *p, void *stream, uint32_t r
     auto s = reinterpret_cast<hipStream_t>(stream);
     hipEvent_t a, b;
     hipError_t e = hipEventCreate(&a);
-    if (e == hipSuccess) e = hipEventCreate(&b);
     if (e != hipSuccess) return hip_fail(e, "event");
+    if ((e = hipEventCreate(&b)) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return hip_fail(e, "event");
+    }
     matvans_batch_config best = p->cfg;
     float best_t = 1e30f;
     std::vector<matvans_batch_config> grid;
@@ -631,16 +634,21 @@ extern "C" int matvans_batch_autotune(matvans_batch *p, void *stream, uint32_t r
             const matvans_batch_config &cfg = grid[k];
             if ((st = matvans_batch_configure(p, &cfg))) goto done;
             if ((st = matvans_batch_launch(p, stream))) goto done;   // warm-up
-            hipEventRecord(a, s);
+            if ((e = hipEventRecord(a, s)) != hipSuccess) {
+                st = hip_fail(e, "event record");
+                goto done;
+            }
             for (uint32_t r = 0; r < reps && !st; r++) st = matvans_batch_launch(p, stream);
-            hipEventRecord(b, s);
             if (st) goto done;
-            if ((e = hipEventSynchronize(b)) != hipSuccess) {
-                st = hip_fail(e, "sync");
+            if ((e = hipEventRecord(b, s)) != hipSuccess || (e = hipEventSynchronize(b)) != hipSuccess) {
+                st = hip_fail(e, "event sync");
                 goto done;
             }
             float ms = 0;
-            hipEventElapsedTime(&ms, a, b);
+            if ((e = hipEventElapsedTime(&ms, a, b)) != hipSuccess) {
+                st = hip_fail(e, "event time");
+                goto done;
+            }
             t[k] = std::min(t[k], ms / float(reps));
         }
     }
@@ -650,8 +658,8 @@ extern "C" int matvans_batch_autotune(matvans_batch *p, void *stream, uint32_t r
             best = grid[k];
         }
 done:
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
     if (st) return st;
     if ((st = matvans_batch_configure(p, &best))) return st;
     if (best_out) *best_out = best;
