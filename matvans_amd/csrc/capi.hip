@@ -246,7 +246,7 @@ extern "C" int matvans_int8_matvec(int32_t *dst, const int8_t *mat, const int8_t
     if (!aligned(mat, 16) || !aligned(vecs, 16) || vec_stride % 16)
         return fail(MATVANS_E_ALIGN, "matrix and vectors must be 16-byte aligned");
     return hip_status(mv::launch_int8_matvec(dst, mat, vecs, n_vec, vec_stride, rows, cols,
-                                             as_stream(stream)),
+                                             current_device_cus(), as_stream(stream)),
                       "int8_matvec launch");
 }
 

@@ -12,7 +12,8 @@
 //
 // Both matvecs are HBM-bound: each wave streams whole rows with 16-byte nontemporal
 // loads (four in flight per lane and row) while the activation vectors sit in LDS (in
-// the caches when they exceed 64 KiB); the int8 matvec streams one or two rows per wave.
+// the caches beyond 64 KiB; the int8 matvec stages up to 160 KiB in persistent 16-wave
+// blocks); the int8 matvec streams one or two rows per wave.
 #include <hip/hip_runtime.h>
 
 #include "launch.h"
@@ -24,6 +25,8 @@ constexpr int kThreads = 256;
 constexpr int kRowsPerBlock = 8;   // 2 rows per wave, 4 waves
 constexpr int kInt8Loads = 4;      // int8 matvec: 16-byte loads in flight per lane and row
 constexpr uint32_t kMaxLdsBytes = 64 * 1024;
+constexpr uint32_t kMaxBigLdsBytes = 160 * 1024;   // the LDS of one gfx950 CU
+constexpr uint32_t kBigMinLds = 16 * 1024;   // int8 matvec: persistent 16-wave blocks above
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -52,8 +55,8 @@ __device__ __forceinline__ float wave_sumf(float a) {
 // R rows per wave in lockstep: each 16-byte chunk of the vectors is read once for R rows
 // (vector traffic per matrix byte NV / R); each lane keeps kInt8Loads chunks of each row
 // in flight.
-template <int NV, int R, bool LDS>
-__global__ __launch_bounds__(kThreads) void int8_matvec_kernel(
+template <int NV, int R, bool LDS, int WPB = 4>
+__global__ __launch_bounds__(64 * WPB) void int8_matvec_kernel(
     int32_t *__restrict__ dst, const int8_t *__restrict__ mat, const int8_t *__restrict__ vecs,
     uint32_t vec_stride, uint32_t rows, uint32_t cols) {
     extern __shared__ uint4 svec[];   // [NV][cols/16] when LDS
@@ -61,7 +64,7 @@ __global__ __launch_bounds__(kThreads) void int8_matvec_kernel(
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     if (LDS) {
         for (uint32_t v = 0; v < NV; v++)
-            for (uint32_t c = tid; c < nch; c += kThreads)
+            for (uint32_t c = tid; c < nch; c += 64 * WPB)
                 svec[v * nch + c] =
                     *reinterpret_cast<const uint4 *>(vecs + size_t(v) * vec_stride + c * 16);
         __syncthreads();
@@ -70,54 +73,55 @@ __global__ __launch_bounds__(kThreads) void int8_matvec_kernel(
         if (LDS) return svec[v * nch + c];
         return *reinterpret_cast<const uint4 *>(vecs + size_t(v) * vec_stride + c * 16);
     };
-    const uint32_t row0 = (blockIdx.x * 4 + wave) * R;
-    if (row0 >= rows) return;
-    const uint4 *m[R];
-    bool ok[R];
+    for (uint32_t row0 = (blockIdx.x * WPB + wave) * R; row0 < rows;
+         row0 += gridDim.x * WPB * R) {
+        const uint4 *m[R];
+        bool ok[R];
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        ok[r] = row0 + r < rows;
-        m[r] = reinterpret_cast<const uint4 *>(mat + size_t(ok[r] ? row0 + r : row0) * cols);
-    }
-    int32_t acc[R][NV];
-#pragma unroll
-    for (int r = 0; r < R; r++)
-#pragma unroll
-        for (int v = 0; v < NV; v++) acc[r][v] = 0;
-    for (uint32_t c0 = lane; c0 < nch; c0 += 64 * kInt8Loads) {
-        uint4 a[kInt8Loads][R];
-#pragma unroll
-        for (int u = 0; u < kInt8Loads; u++) {
-            const uint32_t c = c0 + 64 * u;
-#pragma unroll
-            for (int r = 0; r < R; r++)
-                a[u][r] = c < nch && ok[r] ? ntload(m[r] + c) : make_uint4(0, 0, 0, 0);
+        for (int r = 0; r < R; r++) {
+            ok[r] = row0 + r < rows;
+            m[r] = reinterpret_cast<const uint4 *>(mat + size_t(ok[r] ? row0 + r : row0) * cols);
         }
+        int32_t acc[R][NV];
 #pragma unroll
-        for (int u = 0; u < kInt8Loads; u++) {
-            const uint32_t c = c0 + 64 * u;
-            if (c >= nch) break;
+        for (int r = 0; r < R; r++)
 #pragma unroll
-            for (int v = 0; v < NV; v++) {
-                const uint4 b = vchunk(v, c);
+            for (int v = 0; v < NV; v++) acc[r][v] = 0;
+        for (uint32_t c0 = lane; c0 < nch; c0 += 64 * kInt8Loads) {
+            uint4 a[kInt8Loads][R];
 #pragma unroll
-                for (int r = 0; r < R; r++) {
-                    int32_t t = acc[r][v];
-                    t = __builtin_amdgcn_sdot4(int(a[u][r].x), int(b.x), t, false);
-                    t = __builtin_amdgcn_sdot4(int(a[u][r].y), int(b.y), t, false);
-                    t = __builtin_amdgcn_sdot4(int(a[u][r].z), int(b.z), t, false);
-                    acc[r][v] = __builtin_amdgcn_sdot4(int(a[u][r].w), int(b.w), t, false);
+            for (int u = 0; u < kInt8Loads; u++) {
+                const uint32_t c = c0 + 64 * u;
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    a[u][r] = c < nch && ok[r] ? ntload(m[r] + c) : make_uint4(0, 0, 0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < kInt8Loads; u++) {
+                const uint32_t c = c0 + 64 * u;
+                if (c >= nch) break;
+#pragma unroll
+                for (int v = 0; v < NV; v++) {
+                    const uint4 b = vchunk(v, c);
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        int32_t t = acc[r][v];
+                        t = __builtin_amdgcn_sdot4(int(a[u][r].x), int(b.x), t, false);
+                        t = __builtin_amdgcn_sdot4(int(a[u][r].y), int(b.y), t, false);
+                        t = __builtin_amdgcn_sdot4(int(a[u][r].z), int(b.z), t, false);
+                        acc[r][v] = __builtin_amdgcn_sdot4(int(a[u][r].w), int(b.w), t, false);
+                    }
                 }
             }
         }
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                const int32_t t = wave_sum(acc[r][v]);
+                if (lane == 0 && ok[r]) dst[size_t(v) * rows + row0 + r] = t;
+            }
     }
-#pragma unroll
-    for (int r = 0; r < R; r++)
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-            const int32_t t = wave_sum(acc[r][v]);
-            if (lane == 0 && ok[r]) dst[size_t(v) * rows + row0 + r] = t;
-        }
 }
 
 template <bool LDS>
@@ -274,13 +278,26 @@ hipError_t int8_nv_r(int32_t *dst, const int8_t *mat, const int8_t *vecs, uint32
     return hipGetLastError();
 }
 
-// Two rows per wave once there are enough rows to keep the CUs busy with half the waves
-// (>= 8192) or once the vectors outgrow 32 KiB (their traffic per matrix byte halves);
-// one row per wave otherwise.  Measured at 4096 x 4096, 14336 x 4096 and 4096 x 14336
-// with 1, 4 and 8 vectors, this picks the faster of the two in every case.
+// Several vectors of 16-160 KiB: persistent 16-wave blocks, as many as the CUs hold at
+// once (two per CU while two stages fit in the LDS), stage the vectors once and loop
+// over the rows, one row per wave, so the stage is not re-read for every few rows.
+// Otherwise 4-wave blocks: two rows per wave once there are enough rows to keep the CUs
+// busy with half the waves (>= 8192) or once the vectors outgrow 32 KiB (past the LDS
+// their traffic per matrix byte halves), one row per wave otherwise.  Measured at
+// 4096 x 4096, 14336 x 4096 and 4096 x 14336 with 1, 4 and 8 vectors, this picks the
+// fastest of the variants tried in every case.
 template <int NV>
 hipError_t int8_nv(int32_t *dst, const int8_t *mat, const int8_t *vecs, uint32_t vec_stride,
-                   uint32_t rows, uint32_t cols, hipStream_t s) {
+                   uint32_t rows, uint32_t cols, uint32_t cus, hipStream_t s) {
+    const uint32_t lds = NV * cols;
+    if (NV >= 2 && lds > kBigMinLds && lds <= kMaxBigLdsBytes) {
+        const uint32_t per_cu = lds * 2 <= kMaxBigLdsBytes ? 2u : 1u;
+        uint32_t g = (rows + 15) / 16;
+        g = g < cus * per_cu ? g : cus * per_cu;
+        hipLaunchKernelGGL((int8_matvec_kernel<NV, 1, true, 16>), dim3(g), dim3(1024), lds, s,
+                           dst, mat, vecs, vec_stride, rows, cols);
+        return hipGetLastError();
+    }
     if (rows >= 8192 || uint64_t(NV) * cols > 32768u)
         return int8_nv_r<NV, 2>(dst, mat, vecs, vec_stride, rows, cols, s);
     return int8_nv_r<NV, 1>(dst, mat, vecs, vec_stride, rows, cols, s);
@@ -290,16 +307,16 @@ hipError_t int8_nv(int32_t *dst, const int8_t *mat, const int8_t *vecs, uint32_t
 
 hipError_t launch_int8_matvec(int32_t *dst, const int8_t *mat, const int8_t *vecs,
                               uint32_t n_vec, uint32_t vec_stride, uint32_t rows, uint32_t cols,
-                              hipStream_t s) {
+                              uint32_t cus, hipStream_t s) {
     switch (n_vec) {
-        case 1: return int8_nv<1>(dst, mat, vecs, vec_stride, rows, cols, s);
-        case 2: return int8_nv<2>(dst, mat, vecs, vec_stride, rows, cols, s);
-        case 3: return int8_nv<3>(dst, mat, vecs, vec_stride, rows, cols, s);
-        case 4: return int8_nv<4>(dst, mat, vecs, vec_stride, rows, cols, s);
-        case 5: return int8_nv<5>(dst, mat, vecs, vec_stride, rows, cols, s);
-        case 6: return int8_nv<6>(dst, mat, vecs, vec_stride, rows, cols, s);
-        case 7: return int8_nv<7>(dst, mat, vecs, vec_stride, rows, cols, s);
-        case 8: return int8_nv<8>(dst, mat, vecs, vec_stride, rows, cols, s);
+        case 1: return int8_nv<1>(dst, mat, vecs, vec_stride, rows, cols, cus, s);
+        case 2: return int8_nv<2>(dst, mat, vecs, vec_stride, rows, cols, cus, s);
+        case 3: return int8_nv<3>(dst, mat, vecs, vec_stride, rows, cols, cus, s);
+        case 4: return int8_nv<4>(dst, mat, vecs, vec_stride, rows, cols, cus, s);
+        case 5: return int8_nv<5>(dst, mat, vecs, vec_stride, rows, cols, cus, s);
+        case 6: return int8_nv<6>(dst, mat, vecs, vec_stride, rows, cols, cus, s);
+        case 7: return int8_nv<7>(dst, mat, vecs, vec_stride, rows, cols, cus, s);
+        case 8: return int8_nv<8>(dst, mat, vecs, vec_stride, rows, cols, cus, s);
         default: return hipErrorInvalidValue;
     }
 }

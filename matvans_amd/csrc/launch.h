@@ -94,7 +94,7 @@ hipError_t launch_quantize_vectors(const float *x, uint32_t n_vec, uint32_t x_st
 
 hipError_t launch_int8_matvec(int32_t *dst, const int8_t *mat, const int8_t *vecs,
                               uint32_t n_vec, uint32_t vec_stride, uint32_t rows, uint32_t cols,
-                              hipStream_t s);
+                              uint32_t cus, hipStream_t s);
 hipError_t launch_f32_gemv(float *y, const float *mat, const float *x, uint32_t rows,
                            uint32_t cols, float alpha, hipStream_t s);
 hipError_t launch_absmax_requant(const int32_t *acc, uint32_t n, int8_t *out8, float *delta,

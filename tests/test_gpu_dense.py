@@ -19,13 +19,15 @@ def note(msg):
     sys.stdout.flush()
 
 
-# one row per wave with the vectors in LDS (small cases, 4096 x 14336 x 1); two rows per
-# wave from 8192 rows (8193: a last wave with one row) or beyond 32 KiB of vectors, in LDS
-# up to 64 KiB (7 x 8192 x 5) and from the caches above (4097 x 14336 x 8, 100 x 20000 x 3)
+# 4-wave blocks, one row per wave with the vectors in LDS (small cases, 4096 x 14336 x 1)
+# or two from 8192 rows (8193: a last wave with one row); persistent 16-wave blocks for
+# 16-160 KiB of vectors (7 x 8192 x 5, 100 x 20000 x 3, 14336 x 4096 x 8, 4097 x 14336 x
+# 8: more rows than the grid's waves); two rows per wave from the caches beyond 160 KiB
+# (5 x 24576 x 7)
 @pytest.mark.parametrize("rows,cols,n_vec", [(1, 16, 1), (5, 48, 2), (37, 4096, 1),
                                              (4096, 14336, 1), (14336, 4096, 8),
                                              (100, 20000, 3), (8193, 512, 2), (7, 8192, 5),
-                                             (4097, 14336, 8)])
+                                             (4097, 14336, 8), (5, 24576, 7)])
 def test_int8_matvec(cuda, rows, cols, n_vec):
     import torch
     note(f"int8 {rows}x{cols} nvec={n_vec}")
