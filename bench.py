@@ -214,12 +214,14 @@ def _v0_copy(c):
                                c.ppf_data, c.payload)
 
 
-def bench_block(dev, steps, warmup, scale=1.0, nvecs=(1, 4, 8), fmts=("v0", "q32", "q64")):
+def bench_block(dev, steps, warmup, scale=1.0, nvecs=(1, 4, 8), fmts=("v0", "q32", "q64"),
+                replays=1):
     """One decoder block (q, k, v, o, gate, up, down; sigma=6 int8) with 1, 4 and 8 vectors
     in one persistent launch per step, per encoding; the tile size, tile order and grid
     are autotuned per (encoding, n_vec), and the fastest encoding (the streams-per-row
     choice) is reported as best.  Also times the same matrices as one launch per matrix,
-    the uncompressed int8 matvecs, and the backward dx = W^T (s_w * g) of every matrix."""
+    the uncompressed int8 matvecs, and the backward dx = W^T (s_w * g) of every matrix.
+    Each figure is the median of `replays` timed replays of a graph of `steps` launches."""
     import torch
     import matvans_amd as mv
     from matvans_amd.batch import Batch, BatchBackward
@@ -258,10 +260,12 @@ def bench_block(dev, steps, warmup, scale=1.0, nvecs=(1, 4, 8), fmts=("v0", "q32
                     if not np.array_equal(o.cpu().numpy(), w):
                         print(f"block {fmt} n_vec={nv}: batched result differs", file=sys.stderr)
                         sys.exit(1)
-            ms = time_launches(lambda i: batches[i].launch(), n_copies, steps, warmup)
+            ms = time_launches(lambda i: batches[i].launch(), n_copies, steps, warmup,
+                               replays=replays)
             outs = batches[0].outputs
             ms1 = time_launches(lambda i: [m.multiply(v, out=o) for m, v, o in
-                                           zip(copies[i], dv, outs)], n_copies, steps, warmup)
+                                           zip(copies[i], dv, outs)], n_copies, steps, warmup,
+                                replays=replays)
             alg = stored + sum(nv * c + 4 * nv * r for _, r, c in shapes)
             per[f"n_vec={nv}"] = {"us": round(ms * 1e3, 2), "separate_launches_us":
                                   round(ms1 * 1e3, 2), "tile_units_order_bpc": list(cfg),
@@ -281,7 +285,7 @@ def bench_block(dev, steps, warmup, scale=1.0, nvecs=(1, 4, 8), fmts=("v0", "q32
             msb = time_launches(lambda i: [m.backward(gg, sw, alpha, out=d, workspace=w)
                                            for m, gg, sw, d, w in
                                            zip(copies[i], g, s_w, dx, ws)],
-                                n_copies, steps, warmup)
+                                n_copies, steps, warmup, replays=replays)
             per[f"backward_separate_n_vec={nv}_us"] = round(msb * 1e3, 2)
             # the same backward as one batched plan per copy (two launches per step)
             bb = [BatchBackward([(m, gg, sw, alpha) for m, gg, sw in zip(cp, g, s_w)])
@@ -297,7 +301,8 @@ def bench_block(dev, steps, warmup, scale=1.0, nvecs=(1, 4, 8), fmts=("v0", "q32
                 if o.cpu().numpy().tobytes() != d.cpu().numpy().tobytes():
                     print(f"block {fmt}: batched backward differs", file=sys.stderr)
                     sys.exit(1)
-            msbb = time_launches(lambda i: bb[i].launch(), n_copies, steps, warmup)
+            msbb = time_launches(lambda i: bb[i].launch(), n_copies, steps, warmup,
+                                 replays=replays)
             per[f"backward_n_vec={nv}_us"] = round(msbb * 1e3, 2)
             per[f"backward_batch_cfg_n_vec={nv}"] = list(bcfg)
             fwd = per[f"n_vec={nv}"]["us"]
@@ -317,7 +322,7 @@ def bench_block(dev, steps, warmup, scale=1.0, nvecs=(1, 4, 8), fmts=("v0", "q32
                 print("block int8 comparator differs", file=sys.stderr)
                 sys.exit(1)
         ms8 = time_launches(lambda i: [mv.int8_matvec(t, v, out=o) for t, v, o in
-                                       zip(w8c[i], dv, o8)], n8, steps, warmup)
+                                       zip(w8c[i], dv, o8)], n8, steps, warmup, replays=replays)
         res[f"int8_n_vec={nv}_us"] = round(ms8 * 1e3, 2)
     del w8c, w8
     best = {}
@@ -775,7 +780,7 @@ def main():
     batch = [b for b in args.batch.split(",") if b and b != "none"]
     bsteps, bwarm = max(1, min(args.steps, 50)), max(1, min(args.warmup, 5))
     if "block" in batch:
-        line["block"] = bench_block(dev, bsteps, bwarm, args.batch_scale)
+        line["block"] = bench_block(dev, bsteps, bwarm, args.batch_scale, replays=args.replays)
         torch.cuda.empty_cache()
     if "uneven" in batch:
         line["uneven"] = bench_uneven(dev, max(1, bsteps // 5), bwarm, args.batch_scale)
