@@ -77,9 +77,10 @@ def test_edge_alphabets(cuda, S, P, mode, coder):
 LLM = [(4096, 14336, "l", 4.0), (14336, 4096, "l", 4.0), (4096, 4096, "g", 6.0)]
 
 
-@pytest.mark.parametrize("S,P,mode,coder", [(128, 8, 0, 0), (64, 8, 0, 0), (64, 12, 0, 0),
-                                            (64, 8, 1, 0), (32, 12, 0, 0), (32, 8, 0, 1),
-                                            (64, 8, 0, 1), (128, 8, 0, 1)])
+@pytest.mark.parametrize("S,P,mode,coder", [(128, 8, 0, 0), (64, 8, 0, 0), (32, 8, 0, 0),
+                                            (64, 12, 0, 0), (64, 8, 1, 0), (32, 12, 0, 0),
+                                            (32, 8, 0, 1), (64, 8, 0, 1), (128, 8, 0, 1),
+                                            (64, 8, 1, 1)])
 @pytest.mark.parametrize("rows,cols,kind,p", LLM)
 def test_llm_shapes(cuda, S, P, mode, coder, rows, cols, kind, p):
     note(f"v1 LLM S={S} P={P} mode={mode} coder={coder} {rows}x{cols} {kind}{p}")
