@@ -23,7 +23,8 @@ def _p(t):
 class Batch:
     """entries: list of (matrix, vecs), (matrix, vecs, scale) or (matrix, vecs, scale,
     int32) where matrix is a CompressedMatrix (.bin v0), a CompressedMatrixV1 with
-    coder=1 and 32, 64 or 128 streams, or a (BufferedMatrices, k) pair; vecs is an int8 device
+    coder=1, one table per matrix and 32, 64 or 128 streams, or a (BufferedMatrices, k)
+    pair (a ValueError names any other entry); vecs is an int8 device
     tensor [n_vec, cols] (or [cols]).  All matrices must be on the device.
     ``outputs[i]`` is the int32 [n_vec, rows] result of entry i (None when int32 is
     False); ``out8[i]`` the next layer's int8 input clamp(rint(float(acc) * scale)),
@@ -78,10 +79,11 @@ class Batch:
                     it.format = V0
                     it.min_value, it.G = m.min_value, m.G
                     it.cdf, it.ppf = _p(m.d_cdf_data), _p(m.d_ppf_data)
-                elif isinstance(m, CompressedMatrixV1) and m.coder == 1:
+                elif isinstance(m, CompressedMatrixV1) and m.coder == 1 and m.table_mode == 0:
                     it.format, it.streams, it.lut = QUAD, m.streams, _p(m.d_lut)
                 else:
-                    raise ValueError("batch entries: .bin v0, MVX1 quad or family-B matrices")
+                    raise ValueError("batch entries: .bin v0, MVX1 quad (one table per "
+                                     "matrix) or family-B matrices")
             it.rows, it.cols = rows, cols
             self.outputs.append(acc)
             self.out8.append(o8)
@@ -140,7 +142,8 @@ class BatchBackward(Batch):
     (persistent partial sums from the work queue, then one fixed-order reduce); bitwise
     equal to each matrix's own ``backward``.  entries: list of (matrix, g, s_w, alpha)
     with matrix a CompressedMatrix (.bin v0) or a quad CompressedMatrixV1 (32 or 64
-    streams) on the device, g fp32 [n_vec, rows] (or [rows]), s_w fp32 [rows], alpha
+    streams, one table per matrix) on the device, g fp32 [n_vec, rows] (or [rows]), s_w
+    fp32 [rows], alpha
     fp32 [n_vec] or None (ones).  ``outputs[i]`` is dx of entry i, fp32 [n_vec, cols]."""
 
     def __init__(self, entries):  # noqa: D107 (the base initialiser is not used)
@@ -163,10 +166,11 @@ class BatchBackward(Batch):
                 it.format = V0
                 it.min_value, it.G = m.min_value, m.G
                 it.cdf, it.ppf = _p(m.d_cdf_data), _p(m.d_ppf_data)
-            elif isinstance(m, CompressedMatrixV1) and m.coder == 1:
+            elif isinstance(m, CompressedMatrixV1) and m.coder == 1 and m.table_mode == 0:
                 it.format, it.streams, it.lut = QUAD, m.streams, _p(m.d_lut)
             else:
-                raise ValueError("backward batch entries: .bin v0 or MVX1 quad matrices")
+                raise ValueError("backward batch entries: .bin v0 or MVX1 quad matrices "
+                                 "(one table per matrix)")
             self._keep += [m, gg, s_w, alpha]
             self.outputs.append(dx)
             self.out8.append(None)

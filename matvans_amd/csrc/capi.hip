@@ -296,13 +296,14 @@ extern "C" int matvans_v1_decompress_backward(const matvans_v1_dev_matrix *m, fl
                                               const float *s_w, const float *alpha,
                                               float *workspace, size_t workspace_bytes,
                                               void *stream) {
-    if (!m || !dx || !g || !s_w || !alpha || !workspace || !m->cursors || !m->payload || !m->lut)
-        return fail(MATVANS_E_INVALID, "null argument or device pointer");
-    if (n_vec < 1 || n_vec > 8) return fail(MATVANS_E_INVALID, "n_vec %u outside 1..8", n_vec);
+    if (!m || !dx || !g || !s_w || !alpha || !workspace)
+        return fail(MATVANS_E_INVALID, "null argument");
     if (m->coder != 1 || m->precision != 8 || m->table_mode != 0 ||
         (m->streams != 32 && m->streams != 64 && m->streams != 128))
         return fail(MATVANS_E_UNSUPPORTED,
                     "backward needs the quad coder with 32, 64 or 128 streams and one 8-bit table");
+    if (!m->cursors || !m->payload || !m->lut) return fail(MATVANS_E_INVALID, "null device pointer");
+    if (n_vec < 1 || n_vec > 8) return fail(MATVANS_E_INVALID, "n_vec %u outside 1..8", n_vec);
     if (m->rows == 0 || m->cols == 0 || m->cols % (16 * m->streams))
         return fail(MATVANS_E_SHAPE, "shape %u x %u (cols must be a multiple of %u)", m->rows,
                     m->cols, 16 * m->streams);

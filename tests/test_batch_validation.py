@@ -81,3 +81,20 @@ def test_empty_and_null_plans():
     with pytest.raises(MatvansError) as e:
         call("matvans_batch_launch", None, None)
     assert e.value.status == 1
+
+
+def test_per_row_table_matrices_are_refused_by_name():
+    """A quad matrix with one table per row has no matrix-wide decode table: both batch
+    constructors name the accepted encodings instead of failing on a null table."""
+    import numpy as np
+    import torch
+    from matvans_amd.batch import Batch, BatchBackward
+    from matvans_amd.mvx import CompressedMatrixV1
+    m = np.random.default_rng(1).integers(-20, 21, (64, 1024)).astype(np.int8)
+    rowcdf = CompressedMatrixV1.encode(m, 64, 8, 1, coder=1)
+    single = CompressedMatrixV1.encode(m, 64, 8, 0, coder=0)
+    for c in (rowcdf, single):
+        with pytest.raises(ValueError, match="one table per matrix"):
+            Batch([(c, torch.zeros(1024, dtype=torch.int8))])
+        with pytest.raises(ValueError, match="one table per matrix"):
+            BatchBackward([(c, torch.zeros(64), torch.ones(64), None)])

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import matvans_amd as mv
+from matvans_amd.capi import MatvansError
 from matvans_amd.mvx import CompressedMatrixV1
 from oracle import oracle as O
 
@@ -143,3 +144,21 @@ def test_autograd_function(cuda, fmt, batch):
     gref = dense_ref(w, s_w.cpu().numpy(), gy.cpu().numpy(), np.ones(batch))
     got = x.grad.cpu().numpy().astype(np.float64)
     assert np.linalg.norm(got - gref) <= 1e-5 * np.linalg.norm(gref)
+
+
+@pytest.mark.parametrize("S,P,mode,coder", [(64, 8, 1, 1), (64, 8, 0, 0), (32, 12, 0, 0)])
+def test_unsupported_encodings_are_refused(cuda, S, P, mode, coder):
+    """The backward exists for .bin v0 and the quad coder with one table per matrix; the
+    per-row-table quad coder and the single-symbol coders are refused with
+    MATVANS_E_UNSUPPORTED before any launch (the output keeps its contents)."""
+    import torch
+    note(f"backward refuses S={S} P={P} mode={mode} coder={coder}")
+    w, _ = gaussian_int8(64, 2048, 6.0, seed=9)
+    cm = CompressedMatrixV1.encode(w, S, P, mode, coder=coder).to_device()
+    g = torch.ones((1, 64), device="cuda")
+    s_w = torch.ones(64, device="cuda")
+    out = torch.full((1, 2048), float("nan"), device="cuda")
+    with pytest.raises(MatvansError) as e:
+        cm.backward(g, s_w, out=out)
+    assert e.value.status == 8
+    assert bool(torch.isnan(out).all())
