@@ -92,7 +92,7 @@ FORMAT_TEXT = {
     "b-rows": "family B (buffered) ans-warp-rows: quad rANS, 7-bit CDF, 32 coders/row",
     "b-stripes": "family B (buffered) ans-warp-stripes: quad rANS, 7-bit CDF, 32 rows/warp",
 }
-BACKWARD_FORMATS = ("v0", "q32", "q64")
+BACKWARD_FORMATS = ("v0", "q32", "q64", "q64-rowcdf")
 DEFAULT_FORMATS = ["v0", "q32", "q64", "q128", "q64-rowcdf", "s128-p8", "s64-p8", "s64-p12",
                    "s64-p8-rowcdf",
                    "s32-p12", "b-rows"]

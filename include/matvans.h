@@ -316,8 +316,9 @@ int matvans_v1_decompress_and_multiply(const matvans_v1_dev_matrix *m, int32_t *
                                        const int8_t *vecs, uint32_t n_vec,
                                        uint32_t vec_stride, void *stream);
 
-/* N3 on the quad coder: the same backward from an MVX1 matrix with coder 1, 32, 64 or
- * 128 streams and one 8-bit table:
+/* N3 on the quad coder: the same backward from an MVX1 matrix with coder 1 and 8-bit
+ * tables, either one table per matrix (32, 64 or 128 streams) or one per row (64 streams;
+ * other encodings: MATVANS_E_UNSUPPORTED):
  *     dx[v*cols + c] = alpha[v] * sum_r W[r,c] * s_w[r] * g[v*g_stride + r]
  * Workspace: matvans_backward_workspace_size(rows, cols, n_vec) bytes, 16-byte aligned.
  * Bitwise reproducible.  No reference counterpart. */

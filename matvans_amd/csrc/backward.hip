@@ -27,10 +27,10 @@ __global__ __launch_bounds__(512) void v0_bwd_partial(BwdJob job) {
     blk::v0_bwd_unit<NV>(job, blockIdx.x, blockIdx.y, gridDim.x, sm);
 }
 
-template <int NV, int S>
+template <int NV, int S, int MODE>
 __global__ __launch_bounds__(512) void quad_bwd_partial(BwdJob job) {
-    __shared__ blk::QuadBwdSmem<NV, S> sm;
-    blk::quad_bwd_unit<NV, S>(job, blockIdx.x, blockIdx.y, gridDim.x, sm);
+    __shared__ blk::QuadBwdSmem<NV, S, MODE> sm;
+    blk::quad_bwd_unit<NV, S, MODE>(job, blockIdx.x, blockIdx.y, gridDim.x, sm);
 }
 
 __global__ __launch_bounds__(1024) void bwd_reduce(BwdJob job, uint32_t nrb) {
@@ -53,25 +53,25 @@ hipError_t bwd_nv(const BwdJob &j, hipStream_t s) {
     return e != hipSuccess ? e : launch_reduce(j, s);
 }
 
-template <int NV, int S>
+template <int NV, int S, int MODE>
 hipError_t quad_bwd_nv(const BwdJob &j, hipStream_t s) {
     dim3 grid((j.rows + 63) / 64, S / kBwdWaves), block(512);
-    hipLaunchKernelGGL((quad_bwd_partial<NV, S>), grid, block, 0, s, j);
+    hipLaunchKernelGGL((quad_bwd_partial<NV, S, MODE>), grid, block, 0, s, j);
     hipError_t e = hipGetLastError();
     return e != hipSuccess ? e : launch_reduce(j, s);
 }
 
-template <int S>
+template <int S, int MODE = 0>
 hipError_t quad_bwd_s(const BwdJob &j, hipStream_t s) {
     switch (j.n_vec) {
-        case 1: return quad_bwd_nv<1, S>(j, s);
-        case 2: return quad_bwd_nv<2, S>(j, s);
-        case 3: return quad_bwd_nv<3, S>(j, s);
-        case 4: return quad_bwd_nv<4, S>(j, s);
-        case 5: return quad_bwd_nv<5, S>(j, s);
-        case 6: return quad_bwd_nv<6, S>(j, s);
-        case 7: return quad_bwd_nv<7, S>(j, s);
-        case 8: return quad_bwd_nv<8, S>(j, s);
+        case 1: return quad_bwd_nv<1, S, MODE>(j, s);
+        case 2: return quad_bwd_nv<2, S, MODE>(j, s);
+        case 3: return quad_bwd_nv<3, S, MODE>(j, s);
+        case 4: return quad_bwd_nv<4, S, MODE>(j, s);
+        case 5: return quad_bwd_nv<5, S, MODE>(j, s);
+        case 6: return quad_bwd_nv<6, S, MODE>(j, s);
+        case 7: return quad_bwd_nv<7, S, MODE>(j, s);
+        case 8: return quad_bwd_nv<8, S, MODE>(j, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -101,7 +101,9 @@ hipError_t launch_v0_backward(const V0BwdArgs &a, hipStream_t s) {
 
 hipError_t launch_quad_backward(const QuadBwdArgs &a, hipStream_t s) {
     BwdJob j{a.partial, a.dx, a.g, a.s_w, a.alpha, a.g_stride, a.n_vec, a.rows, a.cols,
-             a.cursors, a.payload, a.payload_size, 0, 0, nullptr, nullptr, a.lut};
+             a.cursors, a.payload, a.payload_size, a.min_value, a.G, nullptr, nullptr, a.lut,
+             a.freq};
+    if (a.table_mode == 1) return a.streams == 64 ? quad_bwd_s<64, 1>(j, s) : hipErrorInvalidValue;
     if (a.streams == 32) return quad_bwd_s<32>(j, s);
     if (a.streams == 64) return quad_bwd_s<64>(j, s);
     if (a.streams == 128) return quad_bwd_s<128>(j, s);

@@ -27,6 +27,7 @@
 #include <cstdint>
 
 #include "alu.h"
+#include "blk_v1.h"
 #include "v0_stream.h"
 #include "v1_stream.h"
 
@@ -57,6 +58,7 @@ struct BwdJob {
     uint32_t G;
     const uint8_t *cdf, *ppf;
     const uint32_t *glut;        // quad: A / B decode tables (2 x 256 dwords)
+    const uint8_t *freq;         // quad, one table per row: 8-bit frequencies [rows][G]
 };
 
 // From this many gradient vectors on, a group's sums are computed on the matrix cores
@@ -71,10 +73,12 @@ struct V0BwdSmem {
     uint32_t wstage[NV >= kMfmaNV ? kBwdWaves * 256 : 1];   // per wave: 64 x 16 int8
 };
 
-template <int NV, int S>
+template <int NV, int S, int MODE = 0>
 struct QuadBwdSmem {
     uint32_t ring[kBwdWaves * v1::QuadLane<128 / S>::R4 * 64];
-    uint32_t tab[256];
+    uint32_t tab[MODE == 0 ? 256 : 1];
+    uint32_t rowtab[MODE == 1 ? 64 * 256 : 1];            // one packed table per row
+    uint32_t rowscratch[MODE == 1 ? kBwdWaves * 192 : 1];  // 768 bytes per wave
     uint32_t dummy[v1::QuadLane<128 / S>::M * 64];
     uint32_t wstage[NV >= kMfmaNV ? kBwdWaves * 256 : 1];
 };
@@ -324,17 +328,35 @@ __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32
 
 // Quad-coder unit (S = 32, 64 or 128 streams): row block rb, streams 8 sg .. 8 sg + 7;
 // fresh as for v0_bwd_unit.
-template <int NV, int S>
+template <int NV, int S, int MODE = 0>
 __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint32_t sg,
-                                              uint32_t nrb, QuadBwdSmem<NV, S> &sm,
+                                              uint32_t nrb, QuadBwdSmem<NV, S, MODE> &sm,
                                               bool fresh = true) {
+    static_assert(MODE == 0 || S == 64, "one table per row: 64 streams");
     using QL = v1::QuadLane<128 / S>;
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid >> 6, lane = tid & 63;
-    // packed table T[q] = f | r << 8 | sym << 24 from the A / B tables (A = f | sym << 24,
-    // B = r)
-    if (fresh && tid < 256)
-        sm.tab[tid] = (J.glut[tid] & 0xFF0000FFu) | ((J.glut[256 + tid] & 0xFFu) << 8);
+    // packed table T[q] = f | r << 8 | sym << 24: from the A / B tables (A = f | sym << 24,
+    // B = r), or one per row of the unit, lane l reading row l's
+    const uint32_t *tab;
+    if constexpr (MODE == 0) {
+        if (fresh && tid < 256)
+            sm.tab[tid] = (J.glut[tid] & 0xFF0000FFu) | ((J.glut[256 + tid] & 0xFFu) << 8);
+        tab = sm.tab;
+    } else {
+        __syncthreads();   // every wave of the previous unit is done with the tables
+        uint8_t *scratch = reinterpret_cast<uint8_t *>(sm.rowscratch + wave * 192);
+        for (uint32_t k = 0; k < 64u / kBwdWaves; k++) {   // the same count on every wave
+            const uint32_t r = wave * (64u / kBwdWaves) + k;
+            uint32_t *t = sm.rowtab + r * 256u;
+            build_row_table(J.freq, rb * 64u + r, J.G, rb * 64u + r < J.rows, J.min_value,
+                            lane, scratch,
+                            [&](uint32_t q, uint32_t f, uint32_t b, uint32_t sy) {
+                                t[q] = f | (b << 8) | (sy << 24);
+                            });
+        }
+        tab = sm.rowtab + lane * 256u;
+    }
     const uint32_t stream = sg * kBwdWaves + wave;   // 0..S-1
     BwdRows<NV> R;
     R.init(J, rb, lane);
@@ -343,7 +365,7 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
     st.init(J.payload, R.wbeg, R.wend, R.valid, R.rcur, R.rend, stream, sm.ring, sm.dummy, wave,
             lane);
     st.prologue();
-    if (fresh) __syncthreads();
+    if (fresh || MODE == 1) __syncthreads();
     st.start();
 
     const uint32_t cols = J.cols;
@@ -359,7 +381,7 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
         if constexpr (NV >= kMfmaNV) {
             uint32_t P[4];
 #pragma unroll
-            for (int d = 0; d < 4; d++) P[d] = st.quad(sm.tab);
+            for (int d = 0; d < 4; d++) P[d] = st.quad(tab);
             bwd_group_mfma(P, M.b, lane, sm.wstage + wave * 256, base + t * (16u * S), plane,
                            J.n_vec);
         } else {
@@ -367,7 +389,7 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
             L.group(
                 [&](uint32_t &v1, uint32_t &v2) {
                     uint32_t e[4];
-                    st.quad_entries(sm.tab, e);
+                    st.quad_entries(tab, e);
                     quad_cols(e, v1, v2);
                 },
                 lane, tot);

@@ -298,11 +298,16 @@ extern "C" int matvans_v1_decompress_backward(const matvans_v1_dev_matrix *m, fl
                                               void *stream) {
     if (!m || !dx || !g || !s_w || !alpha || !workspace)
         return fail(MATVANS_E_INVALID, "null argument");
-    if (m->coder != 1 || m->precision != 8 || m->table_mode != 0 ||
-        (m->streams != 32 && m->streams != 64 && m->streams != 128))
+    if (m->coder != 1 || m->precision != 8 || m->table_mode > 1 ||
+        (m->streams != 32 && m->streams != 64 && m->streams != 128) ||
+        (m->table_mode == 1 && m->streams != 64))
         return fail(MATVANS_E_UNSUPPORTED,
-                    "backward needs the quad coder with 32, 64 or 128 streams and one 8-bit table");
-    if (!m->cursors || !m->payload || !m->lut) return fail(MATVANS_E_INVALID, "null device pointer");
+                    "backward needs the quad coder with 8-bit tables: one per matrix with 32, "
+                    "64 or 128 streams, or one per row with 64");
+    if (!m->cursors || !m->payload || (m->table_mode == 0 && !m->lut))
+        return fail(MATVANS_E_INVALID, "null device pointer");
+    if (m->table_mode == 1 && (!m->freq || m->G < 2 || m->G > 256))
+        return fail(MATVANS_E_INVALID, "per-row tables need freq and 2 <= G <= 256");
     if (n_vec < 1 || n_vec > 8) return fail(MATVANS_E_INVALID, "n_vec %u outside 1..8", n_vec);
     if (m->rows == 0 || m->cols == 0 || m->cols % (16 * m->streams))
         return fail(MATVANS_E_SHAPE, "shape %u x %u (cols must be a multiple of %u)", m->rows,
@@ -316,7 +321,8 @@ extern "C" int matvans_v1_decompress_backward(const matvans_v1_dev_matrix *m, fl
         return fail(MATVANS_E_CAPACITY, "workspace %zu bytes < %zu", workspace_bytes, need);
     mv::QuadBwdArgs a{dx, workspace, g, n_vec, n_vec > 1 ? g_stride : m->rows, s_w, alpha,
                       m->rows, m->cols, m->streams, m->cursors, m->lut, m->payload,
-                      m->payload_size};
+                      m->payload_size, m->table_mode, m->freq, int32_t(m->min_value),
+                      uint32_t(m->G)};
     return hip_status(mv::launch_quad_backward(a, as_stream(stream)), "quad backward launch");
 }
 

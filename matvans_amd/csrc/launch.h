@@ -82,9 +82,13 @@ struct QuadBwdArgs {
     const float *alpha;      // [n_vec]
     uint32_t rows, cols, streams;
     const uint32_t *cursors;
-    const uint32_t *lut;     // A / B decode tables, 2 x 256 dwords
+    const uint32_t *lut;     // A / B decode tables, 2 x 256 dwords (table mode 0)
     const uint16_t *payload;
     uint32_t payload_size;
+    uint32_t table_mode;     // 1: one 8-bit table per row, from freq (64 streams)
+    const uint8_t *freq;     // [rows][G] (table mode 1)
+    int32_t min_value;
+    uint32_t G;
 };
 
 hipError_t launch_quad_backward(const QuadBwdArgs &a, hipStream_t s);

@@ -21,17 +21,18 @@ def note(msg):
     sys.stdout.flush()
 
 
-FORMATS = ["v0", "q32", "q64", "q128"]
+FORMATS = ["v0", "q32", "q64", "q128", "q64r"]   # q64r: 64 streams, one table per row
 
 
 def encode(fmt, w):
     """The matrix on the device in one of the encodings with a backward: .bin v0, or the
-    MVX1 quad coder with 32 / 64 / 128 streams."""
+    MVX1 quad coder with 32 / 64 / 128 streams and one table per matrix, or 64 streams and
+    one table per row."""
     if fmt == "v0":
         return mv.CompressedMatrix.encode(w, n_threads=16).to_device()
-    S = int(fmt[1:])
     assert _fits(fmt, w.shape[1])
-    return CompressedMatrixV1.encode(w, S, 8, 0, coder=1).to_device()
+    return CompressedMatrixV1.encode(w, _streams(fmt), 8, int(fmt.endswith("r")),
+                                     coder=1).to_device()
 
 
 def dense_ref(w, s_w, g, alpha):
@@ -40,9 +41,13 @@ def dense_ref(w, s_w, g, alpha):
                              w.astype(np.float64))
 
 
+def _streams(fmt):
+    return int(fmt[1:].rstrip("r"))
+
+
 def _fits(fmt, cols):
     """MVX1 layouts need the column count to be a multiple of 16 * streams."""
-    return fmt == "v0" or cols % (16 * int(fmt[1:])) == 0
+    return fmt == "v0" or cols % (16 * _streams(fmt)) == 0
 
 
 SHAPES = [(256, 512, 1), (100, 1024, 3), (4096, 14336, 1), (14336, 4096, 2), (1000, 2048, 8),
@@ -146,11 +151,11 @@ def test_autograd_function(cuda, fmt, batch):
     assert np.linalg.norm(got - gref) <= 1e-5 * np.linalg.norm(gref)
 
 
-@pytest.mark.parametrize("S,P,mode,coder", [(64, 8, 1, 1), (64, 8, 0, 0), (32, 12, 0, 0)])
+@pytest.mark.parametrize("S,P,mode,coder", [(64, 8, 0, 0), (32, 12, 0, 0), (64, 8, 1, 0)])
 def test_unsupported_encodings_are_refused(cuda, S, P, mode, coder):
-    """The backward exists for .bin v0 and the quad coder with one table per matrix; the
-    per-row-table quad coder and the single-symbol coders are refused with
-    MATVANS_E_UNSUPPORTED before any launch (the output keeps its contents)."""
+    """The backward exists for .bin v0 and the quad coder; the single-symbol coders are
+    refused with MATVANS_E_UNSUPPORTED before any launch (the output keeps its
+    contents)."""
     import torch
     note(f"backward refuses S={S} P={P} mode={mode} coder={coder}")
     w, _ = gaussian_int8(64, 2048, 6.0, seed=9)
