@@ -20,6 +20,7 @@ import ctypes
 import numpy as np
 
 from . import capi
+from . import _check
 from .capi import DevMatrix, HostMatrix, MatvansError, call, lib
 
 __all__ = ["CompressedMatrix", "MatvansError", "decmpressAndMultiply", "decompress_and_multiply",
@@ -154,13 +155,13 @@ class CompressedMatrix:
         torch = _torch()
         if self.d_payload is None:
             raise RuntimeError("matrix is not on the device; call to_device() first")
+        dev = self.d_payload.device
         squeeze = vec.dim() == 1
-        v = vec.unsqueeze(0) if squeeze else vec
-        if v.dtype != torch.int8 or v.shape[-1] != self.cols:
-            raise ValueError(f"expected int8 vectors of length {self.cols}")
+        v = _check.vectors("vec", vec, cols=self.cols, device=dev, dtype=torch.int8)
         n = v.shape[0]
         if out is None:
-            out = torch.empty((n, self.rows), dtype=torch.int32, device=v.device)
+            out = torch.empty((n, self.rows), dtype=torch.int32, device=dev)
+        _check.output("out", out, shape=(n, self.rows), dtype=torch.int32, device=dev)
         call("matvans_decompress_and_multiply_nvec", _ptr(out), _ptr(v), n, v.stride(0),
              self.rows, self.cols, _ptr(self.d_cursors), self.min_value, self.G,
              _ptr(self.d_cdf_data), _ptr(self.d_ppf_data), _ptr(self.d_payload),
@@ -173,19 +174,25 @@ class CompressedMatrix:
         torch = _torch()
         if self.d_payload is None:
             raise RuntimeError("matrix is not on the device; call to_device() first")
+        dev = self.d_payload.device
         squeeze = g.dim() == 1
-        gg = (g.unsqueeze(0) if squeeze else g).contiguous()
-        if gg.dtype != torch.float32 or gg.shape[-1] != self.rows:
-            raise ValueError(f"expected fp32 gradients of length {self.rows}")
+        gg = _check.vectors("g", g, cols=self.rows, device=dev, dtype=torch.float32)
         n = gg.shape[0]
-        dev = gg.device
+        _check.tensor("s_w", s_w, dtype=torch.float32, device=dev, shape=(self.rows,))
         if alpha is None:
             alpha = torch.ones(n, dtype=torch.float32, device=dev)
+        _check.tensor("alpha", alpha, dtype=torch.float32, device=dev, shape=(n,))
         need = int(lib().matvans_backward_workspace_size(self.rows, self.cols, n))
+        if workspace is not None:
+            _check.tensor("workspace", workspace, dtype=None, device=dev,
+                          dtypes=(torch.float32, torch.int32, torch.uint8, torch.int8))
+            if not workspace.is_contiguous():
+                raise ValueError("workspace: must be contiguous")
         if workspace is None or workspace.numel() * workspace.element_size() < need:
             workspace = torch.empty(max(1, need // 4), dtype=torch.float32, device=dev)
         if out is None:
             out = torch.empty((n, self.cols), dtype=torch.float32, device=dev)
+        _check.output("out", out, shape=(n, self.cols), dtype=torch.float32, device=dev)
         call("matvans_decompress_backward", _ptr(out), _ptr(gg), n, gg.stride(0), _ptr(s_w),
              _ptr(alpha), self.rows, self.cols, _ptr(self.d_cursors), self.min_value, self.G,
              _ptr(self.d_cdf_data), _ptr(self.d_ppf_data), _ptr(self.d_payload),
@@ -196,7 +203,14 @@ class CompressedMatrix:
     def decompress_and_mult(self, d_result8, d_result32, vector, d_delta, stream=None):
         """Stream-ordered composite: decode-matvec -> absmax -> requant; delta stays on
         the device (matvansWarpPerRow.cu:150-176)."""
+        torch = _torch()
         s = self.dev_struct()
+        dev = self.d_payload.device
+        _check.tensor("d_result8", d_result8, dtype=torch.int8, device=dev, min_numel=self.rows)
+        _check.tensor("d_result32", d_result32, dtype=torch.int32, device=dev,
+                      min_numel=self.rows)
+        _check.tensor("vector", vector, dtype=torch.int8, device=dev, min_numel=self.cols)
+        _check.tensor("d_delta", d_delta, dtype=torch.float32, device=dev, min_numel=1)
         call("matvans_decompress_and_mult", ctypes.byref(s), _ptr(d_result8), _ptr(d_result32),
              _ptr(vector), _ptr(d_delta), _stream(stream))
 
@@ -270,6 +284,16 @@ def decmpressAndMultiply(dst, vec, rows, cols, grid_spacing, cursors, min_value,
                          ppf_data, payload, payload_size, stream=None):
     """The reference kernel's argument list (matvansWarpPerRow-kernel.cu:46-50) on torch
     device tensors; launches the fused HIP kernel on ``stream``."""
+    torch = _torch()
+    dev = payload.device
+    _check.tensor("dst", dst, dtype=torch.int32, device=dev, min_numel=int(rows))
+    _check.tensor("vec", vec, dtype=torch.int8, device=dev, min_numel=int(cols))
+    _check.tensor("cursors", cursors, dtype=None, dtypes=(torch.int32, torch.uint32),
+                  device=dev, min_numel=int(rows))
+    _check.tensor("cdf_data", cdf_data, dtype=torch.uint8, device=dev, min_numel=int(G) + 1)
+    _check.tensor("ppf_data", ppf_data, dtype=torch.uint8, device=dev, min_numel=256)
+    _check.tensor("payload", payload, dtype=None, dtypes=(torch.int16, torch.uint16),
+                  device=dev, min_numel=int(payload_size))
     call("matvans_decompress_and_multiply", _ptr(dst), _ptr(vec), int(rows), int(cols),
          float(grid_spacing), _ptr(cursors), int(min_value), int(G), _ptr(cdf_data),
          _ptr(ppf_data), _ptr(payload), int(payload_size), _stream(stream))
@@ -282,11 +306,15 @@ def decompress_and_multiply(matrix: CompressedMatrix, vec, out=None, stream=None
 def int8_matvec(mat, vecs, out=None, stream=None):
     """Uncompressed int8 comparator: int32 mat @ vecs^T, vecs [n_vec, cols] or [cols]."""
     torch = _torch()
-    squeeze = vecs.dim() == 1
-    v = vecs.unsqueeze(0) if squeeze else vecs
+    _check.tensor("mat", mat, dtype=torch.int8)
+    if mat.dim() != 2 or not mat.is_contiguous():
+        raise ValueError("mat: expected a contiguous [rows, cols] int8 matrix")
     rows, cols = mat.shape
+    squeeze = vecs.dim() == 1
+    v = _check.vectors("vecs", vecs, cols=cols, device=mat.device, dtype=torch.int8)
     if out is None:
         out = torch.empty((v.shape[0], rows), dtype=torch.int32, device=mat.device)
+    _check.output("out", out, shape=(v.shape[0], rows), dtype=torch.int32, device=mat.device)
     call("matvans_int8_matvec", _ptr(out), _ptr(mat), _ptr(v), v.shape[0], v.stride(0), rows,
          cols, _stream(stream))
     return out[0] if squeeze else out
@@ -295,9 +323,14 @@ def int8_matvec(mat, vecs, out=None, stream=None):
 def f32_gemv(mat, x, alpha=1.0, out=None, stream=None):
     """fp32 comparator: alpha * mat @ x."""
     torch = _torch()
+    _check.tensor("mat", mat, dtype=torch.float32)
+    if mat.dim() != 2 or not mat.is_contiguous():
+        raise ValueError("mat: expected a contiguous [rows, cols] fp32 matrix")
     rows, cols = mat.shape
+    _check.tensor("x", x, dtype=torch.float32, device=mat.device, shape=(cols,))
     if out is None:
         out = torch.empty(rows, dtype=torch.float32, device=mat.device)
+    _check.tensor("out", out, dtype=torch.float32, device=mat.device, shape=(rows,))
     call("matvans_f32_gemv", _ptr(out), _ptr(mat), _ptr(x), rows, cols, float(alpha),
          _stream(stream))
     return out
@@ -306,11 +339,16 @@ def f32_gemv(mat, x, alpha=1.0, out=None, stream=None):
 def absmax_requant(acc, clamp=False, out8=None, delta=None, stream=None):
     """(int8 requantised, fp32 delta tensor[1]) of an int32 accumulator vector."""
     torch = _torch()
+    _check.tensor("acc", acc, dtype=torch.int32)
+    if not acc.is_contiguous():
+        raise ValueError("acc: must be contiguous")
     n = acc.numel()
     if out8 is None:
         out8 = torch.empty(n, dtype=torch.int8, device=acc.device)
     if delta is None:
         delta = torch.empty(1, dtype=torch.float32, device=acc.device)
+    _check.tensor("out8", out8, dtype=torch.int8, device=acc.device, min_numel=n)
+    _check.tensor("delta", delta, dtype=torch.float32, device=acc.device, min_numel=1)
     call("matvans_absmax_requant", _ptr(acc), n, _ptr(out8), _ptr(delta), int(bool(clamp)),
          _stream(stream))
     return out8, delta
@@ -319,10 +357,16 @@ def absmax_requant(acc, clamp=False, out8=None, delta=None, stream=None):
 def scale_output(acc, s_w, s_x, out=None, stream=None):
     """y[v, r] = s_x[v] * s_w[r] * float(acc[v, r])."""
     torch = _torch()
+    _check.tensor("acc", acc, dtype=torch.int32)
     a = acc if acc.dim() == 2 else acc.unsqueeze(0)
+    if not a.is_contiguous():
+        raise ValueError("acc: must be contiguous")
     n_vec, rows = a.shape
+    _check.tensor("s_w", s_w, dtype=torch.float32, device=acc.device, shape=(rows,))
+    _check.tensor("s_x", s_x, dtype=torch.float32, device=acc.device, shape=(n_vec,))
     if out is None:
         out = torch.empty((n_vec, rows), dtype=torch.float32, device=acc.device)
+    _check.output("out", out, shape=(n_vec, rows), dtype=torch.float32, device=acc.device)
     call("matvans_scale_output", _ptr(a), rows, n_vec, _ptr(s_w), _ptr(s_x), _ptr(out),
          _stream(stream))
     return out if acc.dim() == 2 else out[0]
@@ -332,6 +376,7 @@ def quantize_vectors(x, stream=None):
     """Per-vector activation quantiser: (int8 xq [n, cols], fp32 s_x [n]) with
     s_x = max|x| / 127 and xq = clamp(rint(x / s_x), -127, 127)."""
     torch = _torch()
+    _check.tensor("x", x, dtype=torch.float32)
     xx = (x.unsqueeze(0) if x.dim() == 1 else x).contiguous()
     n, cols = xx.shape
     xq = torch.empty((n, cols), dtype=torch.int8, device=xx.device)

@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import ctypes
 
+from . import _check
 from .buffered import BufferedMatrices
 from .capi import BatchBwdItem, BatchConfig, BatchItem, call, lib
 from .mvx import CompressedMatrixV1
@@ -43,7 +44,22 @@ class Batch:
             want32 = bool(rest[1]) if len(rest) > 1 else True
             if not want32 and scale is None:
                 raise ValueError("batch entry %d: no output (int32 off and no scale)" % i)
-            v = v.unsqueeze(0) if v.dim() == 1 else v
+            if isinstance(m, tuple):
+                bm, k = m
+                if not 0 <= k < bm.num:
+                    raise ValueError(f"batch entry {i}: matrix index {k} outside 0..{bm.num - 1}")
+                if bm.d is None:
+                    raise ValueError(f"batch entry {i}: matrices are not on the device")
+                mdev, mcols = bm.d["compressed"].device, bm.shapes[k][1]
+            else:
+                if not (isinstance(m, CompressedMatrix) or
+                        (isinstance(m, CompressedMatrixV1) and m.coder == 1 and m.table_mode == 0)):
+                    raise ValueError("batch entries: .bin v0, MVX1 quad (one table per "
+                                     "matrix) or family-B matrices")
+                if getattr(m, "d_payload", None) is None:
+                    raise ValueError(f"batch entry {i}: matrix is not on the device")
+                mdev, mcols = m.d_payload.device, m.cols
+            v = _check.vectors(f"entry {i} vecs", v, cols=mcols, device=mdev, dtype=torch.int8)
             self._keep.append(v)
             it = arr[i]
             it.n_vec, it.vec_stride, it.vecs = v.shape[0], v.stride(0), v.data_ptr()
@@ -153,11 +169,20 @@ class BatchBackward(Batch):
         self.outputs, self.out8 = [], []
         arr = (BatchBwdItem * len(entries))()
         for i, (m, g, s_w, alpha) in enumerate(entries):
-            gg = (g.unsqueeze(0) if g.dim() == 1 else g).contiguous()
+            if not (isinstance(m, CompressedMatrix) or
+                    (isinstance(m, CompressedMatrixV1) and m.coder == 1 and m.table_mode == 0)):
+                raise ValueError("backward batch entries: .bin v0 or MVX1 quad matrices "
+                                 "(one table per matrix)")
+            if getattr(m, "d_payload", None) is None:
+                raise ValueError(f"backward batch entry {i}: matrix is not on the device")
+            dev = m.d_payload.device
+            gg = _check.vectors(f"entry {i} g", g, cols=m.rows, device=dev, dtype=torch.float32)
             n = gg.shape[0]
+            _check.tensor(f"entry {i} s_w", s_w, dtype=torch.float32, device=dev, shape=(m.rows,))
             if alpha is None:
-                alpha = torch.ones(n, dtype=torch.float32, device=gg.device)
-            dx = torch.empty((n, m.cols), dtype=torch.float32, device=gg.device)
+                alpha = torch.ones(n, dtype=torch.float32, device=dev)
+            _check.tensor(f"entry {i} alpha", alpha, dtype=torch.float32, device=dev, shape=(n,))
+            dx = torch.empty((n, m.cols), dtype=torch.float32, device=dev)
             it = arr[i]
             it.rows, it.cols, it.n_vec, it.g_stride = m.rows, m.cols, n, gg.stride(0)
             it.g, it.s_w, it.alpha, it.dx = gg.data_ptr(), s_w.data_ptr(), alpha.data_ptr(), dx.data_ptr()

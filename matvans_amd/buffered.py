@@ -12,6 +12,7 @@ import ctypes
 
 import numpy as np
 
+from . import _check
 from .capi import call, lib
 
 __all__ = ["BufferedMatrices", "read_safetensors"]
@@ -145,11 +146,18 @@ class BufferedMatrices:
         import torch
         if self.d is None:
             raise RuntimeError("call to_device() first")
+        if not 0 <= k < self.num:
+            raise ValueError(f"k: matrix index {k} outside 0..{self.num - 1}")
         rows, cols = self.shapes[k]
-        v = vecs.unsqueeze(0) if vecs.dim() == 1 else vecs
+        dev = self.d["compressed"].device
+        v = _check.vectors("vecs", vecs, cols=cols, device=dev, dtype=torch.int8)
         n = v.shape[0]
         if out8 is None and acc32 is None:
-            acc32 = torch.empty((n, rows), dtype=torch.int32, device=v.device)
+            acc32 = torch.empty((n, rows), dtype=torch.int32, device=dev)
+        if acc32 is not None:
+            _check.output("acc32", acc32, shape=(n, rows), dtype=torch.int32, device=dev)
+        if out8 is not None:
+            _check.output("out8", out8, shape=(n, rows), dtype=torch.int8, device=dev)
         fn = "matvans_b_rows_matvec" if self.layout == "rows" else "matvans_b_stripes_matvec"
         rowp = lambda t: ctypes.c_void_p(t.data_ptr() + 128 * k)
         call(fn, _ptr(self.d["compressed"]), self.compressed.size, _ptr(self.d["offsets"][k]),

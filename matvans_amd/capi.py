@@ -172,6 +172,37 @@ def _check_stamp():
                            "python -m matvans_amd.build")
 
 
+def _bind_hip_runtime():
+    """Map the HIP runtime that the library's calls must share with torch before the
+    library itself.  libmatvans.so needs libamdhip64.so.7; torch ships its own copy under
+    torch/lib.  If the library is loaded first, the dynamic loader maps ROCm's copy (and
+    its libhsa-runtime64), and a later ``import torch`` maps torch's copy beside it: two
+    HIP and two HSA runtimes in one process, each with its own device state, which has
+    crashed GPU calls.  Importing torch first maps torch's runtime, and the library's
+    dependency then resolves to it (same soname), so one runtime serves both.  Without
+    torch the library uses ROCm's runtime alone, which is also a single runtime."""
+    try:
+        import torch  # noqa: F401  (imported for its side effect on the loader)
+    except ImportError:
+        pass
+
+
+def hip_runtimes_mapped() -> list[str]:
+    """Paths of every libamdhip64 mapped into this process (Linux /proc/self/maps)."""
+    seen = []
+    try:
+        with open("/proc/self/maps") as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) >= 6 and "libamdhip64" in parts[-1]:
+                    path = os.path.realpath(parts[-1])
+                    if path not in seen:
+                        seen.append(path)
+    except OSError:
+        pass
+    return seen
+
+
 def lib():
     """The loaded library; raises if it is not built or was built from other sources."""
     global _LIB
@@ -180,7 +211,13 @@ def lib():
             raise RuntimeError(f"{LIB_PATH} is not built; run python -m matvans_amd.build")
         if LIB_PATH == _DEFAULT_LIB_PATH:
             _check_stamp()
+        _bind_hip_runtime()
         L = ctypes.CDLL(LIB_PATH)
+        hips = hip_runtimes_mapped()
+        if len(hips) > 1:
+            raise RuntimeError("two HIP runtimes are mapped in this process (" +
+                               ", ".join(hips) + "); load torch before any other HIP "
+                               "library so that libmatvans.so shares torch's runtime")
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)
             fn.restype = res

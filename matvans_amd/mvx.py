@@ -14,6 +14,7 @@ import ctypes
 
 import numpy as np
 
+from . import _check
 from .capi import V1DevMatrix, V1Host, call, lib
 
 __all__ = ["CompressedMatrixV1", "read_mvx1", "write_mvx1"]
@@ -111,13 +112,13 @@ class CompressedMatrixV1:
         """int32 W @ vec for one vector [cols] or a batch [n_vec, cols] (n_vec <= 8)."""
         import torch
         s = self.dev_struct()
+        dev = self.d_payload.device
         squeeze = vec.dim() == 1
-        v = vec.unsqueeze(0) if squeeze else vec
-        if v.dtype != torch.int8 or v.shape[-1] != self.cols:
-            raise ValueError(f"expected int8 vectors of length {self.cols}")
+        v = _check.vectors("vec", vec, cols=self.cols, device=dev, dtype=torch.int8)
         n = v.shape[0]
         if out is None:
-            out = torch.empty((n, self.rows), dtype=torch.int32, device=v.device)
+            out = torch.empty((n, self.rows), dtype=torch.int32, device=dev)
+        _check.output("out", out, shape=(n, self.rows), dtype=torch.int32, device=dev)
         call("matvans_v1_decompress_and_multiply", ctypes.byref(s), _ptr(out), _ptr(v), n,
              v.stride(0), _stream(stream))
         return out[0] if squeeze else out
@@ -128,19 +129,25 @@ class CompressedMatrixV1:
         reproducible."""
         import torch
         s = self.dev_struct()
+        dev = self.d_payload.device
         squeeze = g.dim() == 1
-        gg = (g.unsqueeze(0) if squeeze else g).contiguous()
-        if gg.dtype != torch.float32 or gg.shape[-1] != self.rows:
-            raise ValueError(f"expected fp32 gradients of length {self.rows}")
+        gg = _check.vectors("g", g, cols=self.rows, device=dev, dtype=torch.float32)
         n = gg.shape[0]
-        dev = gg.device
+        _check.tensor("s_w", s_w, dtype=torch.float32, device=dev, shape=(self.rows,))
         if alpha is None:
             alpha = torch.ones(n, dtype=torch.float32, device=dev)
+        _check.tensor("alpha", alpha, dtype=torch.float32, device=dev, shape=(n,))
         need = int(lib().matvans_backward_workspace_size(self.rows, self.cols, n))
+        if workspace is not None:
+            _check.tensor("workspace", workspace, dtype=None, device=dev,
+                          dtypes=(torch.float32, torch.int32, torch.uint8, torch.int8))
+            if not workspace.is_contiguous():
+                raise ValueError("workspace: must be contiguous")
         if workspace is None or workspace.numel() * workspace.element_size() < need:
             workspace = torch.empty(max(1, need // 4), dtype=torch.float32, device=dev)
         if out is None:
             out = torch.empty((n, self.cols), dtype=torch.float32, device=dev)
+        _check.output("out", out, shape=(n, self.cols), dtype=torch.float32, device=dev)
         call("matvans_v1_decompress_backward", ctypes.byref(s), _ptr(out), _ptr(gg), n,
              gg.stride(0), _ptr(s_w), _ptr(alpha), _ptr(workspace),
              workspace.numel() * workspace.element_size(), _stream(stream))

@@ -44,9 +44,10 @@ def last_json(out):
 
 @pytest.fixture
 def gpu_ready(cuda):
-    """The test process initialises its own GPU runtimes (torch's and the library's, with
-    one small device call) before any child process uses the GPU: when children ran
-    first, this process's later GPU calls failed ("no ROCm-capable device") or crashed."""
+    """The test process initialises its GPU runtime (one small device call through the
+    library, which shares torch's HIP runtime) before any child process uses the GPU.
+    The failures once seen when children ran first came from this process mapping two
+    HIP runtimes (library loaded before torch); capi.lib() now maps torch's first."""
     import torch
     import matvans_amd as mv
     w = torch.ones((4, 16), dtype=torch.int8, device=cuda)
