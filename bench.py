@@ -341,47 +341,51 @@ def bench_block(dev, steps, warmup, scale=1.0, nvecs=(1, 4, 8), fmts=("v0", "q32
 
 def bench_uneven(dev, steps, warmup, scale=1.0, n_mat=32):
     """32 matrices, rows ~ U[256, 16384] (multiple of 32), cols ~ U{512, 768, ..., 8192},
-    entry sigma from {2, 4, 8, 16} (clipped to [-64, 63]), family-B row and stripe layouts mixed, 1..8 vectors
-    each, all in one persistent launch per step (autotuned), against one launch per
-    matrix."""
+    entry sigma from {2, 4, 8, 16}, row and stripe layouts mixed (rows: the quad coder with
+    32 streams per row, columns zero-padded to a multiple of 512 where needed; stripes:
+    family B's stripe layout; sigma = 16 matrices take the row layout, whose model holds
+    their full range), 1..8 vectors each, all in one persistent launch per step
+    (autotuned), against one launch per matrix.  No entry is clipped."""
     import torch
     from matvans_amd.batch import Batch
     from matvans_amd.buffered import BufferedMatrices
-    rng = np.random.default_rng(20250312)
-    specs = []
-    for i in range(n_mat):
-        rows = max(32, int(rng.integers(256 // 32, 16384 // 32 + 1) * 32 * scale) // 32 * 32)
-        cols = max(256, int(rng.integers(2, 33) * 256 * scale) // 256 * 256)
-        sigma = float(rng.choice([2.0, 4.0, 8.0, 16.0]))
-        layout = "rows" if rng.random() < 0.5 else "stripes"
-        specs.append((rows, cols, sigma, layout, int(rng.integers(1, 9))))
-    # family B's 7-bit model holds at most 128 distinct values (the reference encoder
-    # divides by zero beyond that): sigma=16 entries are clipped to [-64, 63], which moves
-    # about 6e-5 of them
-    mats = [np.clip(sample_int8(rng, (r, c), s), -64, 63) for r, c, s, _, _ in specs]
+    from matvans_amd.mvx import CompressedMatrixV1
+    from tests.helpers import pad_cols, uneven_specs
+    specs = uneven_specs(n_mat, scale=scale)
+    rng = np.random.default_rng(20250313)
+    mats = [sample_int8(rng, (r, c), s) for r, c, s, _, _ in specs]
     xs = [act_vectors(rng, nv, c) for (_, c, _, _, nv) in specs]
-    groups = {}
-    for i, (_, _, _, layout, _) in enumerate(specs):
-        groups.setdefault(layout, []).append(i)
-    bms, where = {}, {}
-    for layout, idx in groups.items():
-        bms[layout] = BufferedMatrices.encode([mats[i] for i in idx], layout)
-        for k, i in enumerate(idx):
-            where[i] = (layout, k)
-    stored = sum(b.compressed.nbytes + sum(o.nbytes for o in b.offsets) + 384 * b.num
-                 for b in bms.values())
+    stripes = [i for i, s in enumerate(specs) if s[3] == "stripes"]
+    bm = BufferedMatrices.encode([mats[i] for i in stripes], "stripes") if stripes else None
+    where = {i: k for k, i in enumerate(stripes)}
+    quads, xpad = {}, {}
+    for i, (rows, cols, _, layout, nv) in enumerate(specs):
+        if layout == "rows":
+            mp = pad_cols(mats[i], 512)
+            quads[i] = CompressedMatrixV1.encode(mp, 32, 8, 0, coder=1)
+            xp = np.zeros((nv, mp.shape[1]), np.int8)
+            xp[:, :cols] = xs[i]
+            xpad[i] = xp
+    stored = sum(c.stored_bytes for c in quads.values())
+    if bm is not None:
+        stored += bm.compressed.nbytes + sum(o.nbytes for o in bm.offsets) + 384 * bm.num
     entries = sum(m.size for m in mats)
+    padded = sum(c.rows * c.cols for c in quads.values()) + sum(mats[i].size for i in stripes)
     n_copies = min(8, max(1, -(-2 * MALL_BYTES // stored)))
     copies = []
     for _ in range(n_copies):
-        d = {lay: BufferedMatrices(b.layout, b.compressed, b.offsets, b.ppfs, b.cums, b.probs,
-                                   b.scales, shapes=b.shapes).to_device(dev)
-             for lay, b in bms.items()}
-        copies.append(d)
-    dv = [torch.from_numpy(x).to(dev) for x in xs]
-    batches = []
-    for d in copies:
-        batches.append(Batch([((d[where[i][0]], where[i][1]), dv[i]) for i in range(n_mat)]))
+        d_b = (BufferedMatrices(bm.layout, bm.compressed, bm.offsets, bm.ppfs, bm.cums,
+                                bm.probs, bm.scales, shapes=bm.shapes).to_device(dev)
+               if bm is not None else None)
+        d_q = {i: _v1_copy(c).to_device(dev) for i, c in quads.items()}
+        copies.append((d_b, d_q))
+    dv = {i: torch.from_numpy(xpad[i] if i in xpad else xs[i]).to(dev) for i in range(n_mat)}
+
+    def entries_of(cp):
+        d_b, d_q = cp
+        return [((d_b, where[i]), dv[i]) if i in where else (d_q[i], dv[i])
+                for i in range(n_mat)]
+    batches = [Batch(entries_of(cp)) for cp in copies]
     cfg, _ = batches[0].autotune(reps=2)
     for b in batches[1:]:
         b.configure(*cfg)
@@ -393,18 +397,22 @@ def bench_uneven(dev, steps, warmup, scale=1.0, n_mat=32):
             sys.exit(1)
     ms = time_launches(lambda i: batches[i].launch(), n_copies, steps, warmup)
     outs = batches[0].outputs
-    ms1 = time_launches(lambda i: [copies[i][where[j][0]].mat_vec(where[j][1], dv[j],
-                                                                  acc32=outs[j])
-                                   for j in range(n_mat)], n_copies, steps, warmup)
-    rows_only = [i for i in range(n_mat) if specs[i][3] == "rows"]
+
+    def separate(i):
+        d_b, d_q = copies[i]
+        for j in range(n_mat):
+            if j in where:
+                d_b.mat_vec(where[j], dv[j], acc32=outs[j])
+            else:
+                d_q[j].multiply(dv[j], out=outs[j])
+    ms1 = time_launches(separate, n_copies, steps, warmup)
     alg = stored + sum(nv * c + 4 * nv * r for r, c, _, _, nv in specs)
-    return {"matrices": n_mat, "entries": int(entries),
+    return {"matrices": n_mat, "entries": int(entries), "entries_with_padding": int(padded),
             "bits_per_entry": round(8 * stored / entries, 4),
-            "layouts": {k: len(v) for k, v in groups.items()},
+            "layouts": {"rows (q32)": len(quads), "stripes (family B)": len(stripes)},
             "vectors": int(sum(s[4] for s in specs)), "us": round(ms * 1e3, 2),
             "separate_launches_us": round(ms1 * 1e3, 2), "tile_units_order_bpc": list(cfg),
-            "compressed_GBps": round(alg / (ms * 1e-3) / 1e9, 1), "copies": n_copies,
-            "rows_layout_matrices": len(rows_only)}
+            "compressed_GBps": round(alg / (ms * 1e-3) / 1e9, 1), "copies": n_copies}
 
 
 def bench_shapes(dev, steps, warmup, nvecs=(1, 4, 8)):

@@ -197,6 +197,24 @@ int matvans_bin_matrix(const matvans_bin *bin, uint32_t k, matvans_host_matrix *
 int matvans_bin_write(const char *path, uint32_t result_hash, const int8_t *vector,
                       uint32_t len_v, uint32_t num_matrices, const matvans_host_matrix *mats);
 
+/* ------------------------------------------------------------------------------
+ * A11 harness. The uncompressed baseline container raw-matrices_*.bin (writer
+ * create_matrices_WarpPerRow.py:94-129; reader Matrix::deserialize and main,
+ * baselines/baseline_int8/baseline_matVec_custom.cu:371-393,480-508).  Host memory.
+ * The vector's pad to 4 bytes is skipped (the reference reader assumes len_v % 4 == 0);
+ * the pad byte after an odd-sized matrix is skipped as the reference does (:388-391).
+ * ----------------------------------------------------------------------------*/
+typedef struct matvans_raw matvans_raw;   /* parsed file, owned by the library */
+
+int matvans_raw_read(const char *path, matvans_raw **out);
+void matvans_raw_free(matvans_raw *raw);
+/* Header: num_matrices, result_hash (:482-484); start vector (:487-490). */
+int matvans_raw_header(const matvans_raw *raw, uint32_t *num_matrices, uint32_t *result_hash,
+                       const int8_t **vector, uint32_t *len_v);
+/* Matrix k: shape, grid_spacing and its row-major int8 entries (valid until raw_free). */
+int matvans_raw_matrix(const matvans_raw *raw, uint32_t k, uint32_t *rows, uint32_t *cols,
+                       float *grid_spacing, const int8_t **data);
+
 /* Entropy model (create_matrices_WarpPerRow.py:131-187) and ppf (:189-208).
  * cdf16 receives G+1 <= 257 uint16 entries (cdf16[G] == 256), ppf 256 entries.
  * Counts with equal frequency are ordered by descending value (the reference inherits

@@ -264,6 +264,33 @@ def read_bin(path):
         lib().matvans_bin_free(h)
 
 
+def read_raw(path):
+    """Parse an uncompressed baseline file (raw-matrices_*.bin) with the library reader
+    (baseline_matVec_custom.cu:371-393,480-508).  Returns (header dict, vector int8,
+    [int8 matrices [rows, cols]])."""
+    h = ctypes.c_void_p()
+    call("matvans_raw_read", str(path).encode(), ctypes.byref(h))
+    try:
+        num, rh, lv = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        vp = ctypes.c_void_p()
+        call("matvans_raw_header", h, ctypes.byref(num), ctypes.byref(rh), ctypes.byref(vp),
+             ctypes.byref(lv))
+        vec = (np.ctypeslib.as_array(ctypes.cast(vp, ctypes.POINTER(ctypes.c_int8)),
+                                     (lv.value,)).copy() if lv.value else np.zeros(0, np.int8))
+        mats = []
+        for k in range(num.value):
+            r, c, gs, dp = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_float(), ctypes.c_void_p()
+            call("matvans_raw_matrix", h, k, ctypes.byref(r), ctypes.byref(c), ctypes.byref(gs),
+                 ctypes.byref(dp))
+            n = r.value * c.value
+            a = (np.ctypeslib.as_array(ctypes.cast(dp, ctypes.POINTER(ctypes.c_int8)), (n,)).copy()
+                 if n else np.zeros(0, np.int8))
+            mats.append(a.reshape(r.value, c.value))
+        return {"num_matrices": num.value, "result_hash": rh.value}, vec, mats
+    finally:
+        lib().matvans_raw_free(h)
+
+
 def write_bin(path, result_hash, vector, matrices):
     """Write a .bin v0 file with the library writer."""
     vec = np.ascontiguousarray(vector, np.int8)

@@ -98,6 +98,12 @@ SIGNATURES = {
     "matvans_fxhash": (u32, [P, u64]),
     "matvans_bin_read": (i32, [ctypes.c_char_p, ctypes.POINTER(P)]),
     "matvans_bin_free": (None, [P]),
+    "matvans_raw_read": (i32, [ctypes.c_char_p, ctypes.POINTER(P)]),
+    "matvans_raw_free": (None, [P]),
+    "matvans_raw_header": (i32, [P, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(P),
+                                 ctypes.POINTER(u32)]),
+    "matvans_raw_matrix": (i32, [P, u32, ctypes.POINTER(u32), ctypes.POINTER(u32),
+                                 ctypes.POINTER(f32), ctypes.POINTER(P)]),
     "matvans_bin_header": (i32, [P, ctypes.POINTER(u32), ctypes.POINTER(u32),
                                  ctypes.POINTER(u32), ctypes.POINTER(P), ctypes.POINTER(u32)]),
     "matvans_bin_matrix": (i32, [P, u32, ctypes.POINTER(HostMatrix)]),

@@ -174,6 +174,102 @@ extern "C" int matvans_bin_read(const char *path, matvans_bin **out) {
 
 extern "C" void matvans_bin_free(matvans_bin *bin) { delete bin; }
 
+// ------------------------------------------------------------------ raw container
+// The uncompressed baseline file: header '<LL' (num_matrices, result_hash), '<L' len_v,
+// the vector padded to 4 bytes, then per matrix '<LLf' (rows, cols, grid_spacing), the
+// row-major int8 entries and one pad byte when rows * cols is odd
+// (create_matrices_WarpPerRow.py:94-129; reader baseline_matVec_custom.cu:371-393,
+// 480-508).  The reference reader does not skip the vector's pad either (:487-490), so it
+// reads only files with len_v % 4 == 0; this reader skips it, as the writer emits it.
+struct matvans_raw {
+    uint32_t num_matrices = 0, result_hash = 0;
+    std::vector<int8_t> vector;
+    std::vector<std::vector<int8_t>> data;   // one owned row-major copy per matrix
+    std::vector<uint32_t> rows, cols;
+    std::vector<float> grid_spacing;
+};
+
+extern "C" int matvans_raw_read(const char *path, matvans_raw **out) {
+    if (!path || !out) return fail(MATVANS_E_INVALID, "null argument");
+    *out = nullptr;
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return fail(MATVANS_E_IO, "cannot open %s", path);
+    std::vector<uint8_t> file;
+    std::fseek(f, 0, SEEK_END);
+    long sz = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    if (sz < 0) { std::fclose(f); return fail(MATVANS_E_IO, "cannot size %s", path); }
+    file.resize(static_cast<size_t>(sz));
+    size_t got = sz ? std::fread(file.data(), 1, file.size(), f) : 0;
+    std::fclose(f);
+    if (got != file.size()) return fail(MATVANS_E_IO, "short read on %s", path);
+
+    auto *raw = new matvans_raw();
+    Reader rd{file.data(), file.data() + file.size()};
+    uint32_t len_v = 0;
+    if (!rd.take(&raw->num_matrices, 4) || !rd.take(&raw->result_hash, 4) ||
+        !rd.take(&len_v, 4)) {
+        delete raw;
+        return fail(MATVANS_E_FORMAT, "truncated header");
+    }
+    raw->vector.resize(len_v);
+    if (!rd.take(raw->vector.data(), len_v) || !rd.take(nullptr, 3 - (len_v + 3) % 4)) {
+        delete raw;
+        return fail(MATVANS_E_FORMAT, "truncated vector");
+    }
+    for (uint32_t k = 0; k < raw->num_matrices; k++) {
+        uint32_t rows = 0, cols = 0;
+        float gs = 0.f;
+        if (!rd.take(&rows, 4) || !rd.take(&cols, 4) || !rd.take(&gs, 4)) {
+            delete raw;
+            return fail(MATVANS_E_FORMAT, "matrix %u: truncated header", k);
+        }
+        const uint64_t n = uint64_t(rows) * cols;
+        if (n > uint64_t(rd.end - rd.p)) {
+            delete raw;
+            return fail(MATVANS_E_FORMAT, "matrix %u: %ux%u entries past the end of the file",
+                        k, rows, cols);
+        }
+        std::vector<int8_t> d(static_cast<size_t>(n));
+        if (!rd.take(d.data(), static_cast<size_t>(n)) || (n % 2 == 1 && !rd.take(nullptr, 1))) {
+            delete raw;
+            return fail(MATVANS_E_FORMAT, "matrix %u: truncated entries", k);
+        }
+        raw->data.push_back(std::move(d));
+        raw->rows.push_back(rows);
+        raw->cols.push_back(cols);
+        raw->grid_spacing.push_back(gs);
+    }
+    *out = raw;
+    mv::clear_error();
+    return MATVANS_OK;
+}
+
+extern "C" void matvans_raw_free(matvans_raw *raw) { delete raw; }
+
+extern "C" int matvans_raw_header(const matvans_raw *raw, uint32_t *num_matrices,
+                                  uint32_t *result_hash, const int8_t **vector,
+                                  uint32_t *len_v) {
+    if (!raw) return fail(MATVANS_E_INVALID, "null file");
+    if (num_matrices) *num_matrices = raw->num_matrices;
+    if (result_hash) *result_hash = raw->result_hash;
+    if (vector) *vector = raw->vector.data();
+    if (len_v) *len_v = static_cast<uint32_t>(raw->vector.size());
+    return MATVANS_OK;
+}
+
+extern "C" int matvans_raw_matrix(const matvans_raw *raw, uint32_t k, uint32_t *rows,
+                                  uint32_t *cols, float *grid_spacing, const int8_t **data) {
+    if (!raw) return fail(MATVANS_E_INVALID, "null file");
+    if (k >= raw->num_matrices)
+        return fail(MATVANS_E_INVALID, "matrix %u of %u", k, raw->num_matrices);
+    if (rows) *rows = raw->rows[k];
+    if (cols) *cols = raw->cols[k];
+    if (grid_spacing) *grid_spacing = raw->grid_spacing[k];
+    if (data) *data = raw->data[k].data();
+    return MATVANS_OK;
+}
+
 extern "C" int matvans_bin_header(const matvans_bin *bin, uint32_t *num_matrices,
                                   uint32_t *result_hash, uint32_t *max_word_count,
                                   const int8_t **vector, uint32_t *len_v) {

@@ -2,6 +2,7 @@
 // libmatvans (matvansWarpPerRow/matvansWarpPerRow.cu:225-393).
 //
 //     matvans_run FILE [--iters N] [--quiet]
+//     matvans_run --raw FILE [--iters N] [--quiet]
 //
 // FILE is a .bin v0 file (create_matrices_WarpPerRow.py) or an MVX1 file.  The program
 // reads it with the library reader, uploads every matrix once, and runs the reference's
@@ -11,6 +12,12 @@
 // lines (hash check, time with the payload upload, time of the chain alone) and at the
 // end one JSON line with the best times.  Exit status: 0 when every iteration's hash
 // matched, 1 otherwise, 2 on an error.
+//
+// --raw runs the uncompressed int8 comparator's harness instead
+// (baselines/baseline_int8/baseline_matVec_custom.cu:434-620): FILE is a raw-matrices_*.bin
+// file (create_matrices_WarpPerRow.py:94-129) read with the library's raw reader, and each
+// step is the int8 matvec followed by the clamped absmax requant (:75-86,399-430), with
+// the same hash check against the header's result_hash (:574-586).
 //
 // Differences from the reference harness, on purpose: device buffers are allocated once
 // with the library's upload handles (matvans_matrix_upload; the reference allocates and
@@ -93,9 +100,108 @@ bool is_mvx1(const char *path) {
 void usage() {
     std::printf(
         "usage: matvans_run FILE [--iters N] [--quiet]\n"
+        "       matvans_run --raw FILE [--iters N] [--quiet]\n"
         "  FILE: .bin v0 (create_matrices_WarpPerRow.py) or MVX1 compressed matrices\n"
         "  runs the chain v_k = requant(W_k v_(k-1)) N times (default 10) and checks the\n"
-        "  FxHash of the last vector against the file's result_hash\n");
+        "  FxHash of the last vector against the file's result_hash\n"
+        "  --raw: FILE is an uncompressed raw-matrices_*.bin file; each step is the int8\n"
+        "  comparator matvec and the clamped requant (the int8 baseline's harness)\n");
+}
+
+// The int8 comparator's chain over a raw file (baseline_matVec_custom.cu:434-620).
+int run_raw(const char *path, int iters, bool quiet) {
+    matvans_raw *raw = nullptr;
+    mv_check(matvans_raw_read(path, &raw), "read raw file");
+    uint32_t num = 0, result_hash = 0, len_v = 0;
+    const int8_t *h_vec = nullptr;
+    mv_check(matvans_raw_header(raw, &num, &result_hash, &h_vec, &len_v), "raw header");
+    if (num == 0) {
+        std::fprintf(stderr, "matvans_run: %s holds no matrices\n", path);
+        return 2;
+    }
+    Arena arena;
+    struct RawStep {
+        uint32_t rows, cols;
+        const int8_t *host;
+        int8_t *dev;
+    };
+    std::vector<RawStep> steps;
+    uint32_t max_len = len_v, len = len_v;
+    double macs = 0;
+    for (uint32_t k = 0; k < num; k++) {
+        RawStep st{};
+        float gs = 0.f;
+        mv_check(matvans_raw_matrix(raw, k, &st.rows, &st.cols, &gs, &st.host), "raw matrix");
+        if (st.cols != len) {
+            std::fprintf(stderr, "matvans_run: matrix %u has %u cols, the vector %u entries\n",
+                         k, st.cols, len);
+            return 2;
+        }
+        len = st.rows;
+        st.dev = upload(arena, st.host, size_t(st.rows) * st.cols);
+        max_len = std::max(max_len, std::max(st.rows, st.cols));
+        macs += double(st.rows) * st.cols;
+        steps.push_back(st);
+    }
+    int8_t *d_v0 = upload(arena, h_vec, len_v);
+    int8_t *d_a = arena.alloc<int8_t>(max_len);
+    int8_t *d_b = arena.alloc<int8_t>(max_len);
+    int32_t *d_acc = arena.alloc<int32_t>(max_len);
+    float *d_delta = arena.alloc<float>(1);
+    hipStream_t stream;
+    hip_check(hipStreamCreate(&stream), "stream");
+    hipEvent_t start1, start2, stop1, stop2;
+    for (hipEvent_t *e : {&start1, &start2, &stop1, &stop2}) hip_check(hipEventCreate(e), "event");
+    std::vector<int8_t> h_out(max_len);
+    float best1 = 1e30f, best2 = 1e30f;
+    int matched = 0;
+    for (int it = 0; it < iters; it++) {
+        hip_check(hipEventRecord(start1, stream), "record");
+        for (const RawStep &st : steps)
+            hip_check(hipMemcpyAsync(st.dev, st.host, size_t(st.rows) * st.cols,
+                                     hipMemcpyHostToDevice, stream),
+                      "matrix upload");
+        hip_check(hipMemcpyAsync(d_a, d_v0, len_v, hipMemcpyDeviceToDevice, stream), "vector");
+        hip_check(hipEventRecord(start2, stream), "record");
+        int8_t *vec = d_a, *res = d_b;
+        for (const RawStep &st : steps) {
+            mv_check(matvans_int8_matvec(d_acc, st.dev, vec, 1, st.cols, st.rows, st.cols, stream),
+                     "int8 matvec");
+            mv_check(matvans_absmax_requant(d_acc, st.rows, res, d_delta, 1, stream), "requant");
+            std::swap(vec, res);
+        }
+        hip_check(hipEventRecord(stop2, stream), "record");
+        const uint32_t out_len = steps.back().rows;
+        hip_check(hipMemcpyAsync(h_out.data(), vec, out_len, hipMemcpyDeviceToHost, stream),
+                  "result");
+        hip_check(hipEventRecord(stop1, stream), "record");
+        hip_check(hipEventSynchronize(stop1), "sync");
+        float ms1 = 0, ms2 = 0;
+        hip_check(hipEventElapsedTime(&ms1, start1, stop1), "elapsed");
+        hip_check(hipEventElapsedTime(&ms2, start2, stop2), "elapsed");
+        best1 = std::min(best1, ms1);
+        best2 = std::min(best2, ms2);
+        const uint32_t h = matvans_fxhash(h_out.data(), out_len);
+        if (h == result_hash) {
+            matched++;
+            if (!quiet) std::printf("Hashes match!(%u)\n", h);
+        } else if (!quiet) {
+            std::printf("Hashes *don't* match! (%u, header %u)\n", h, result_hash);
+        }
+        if (!quiet) {
+            std::printf("Time with memcpy:    %f ms\n", ms1);
+            std::printf("Time without memcpy: %f ms\n", ms2);
+        }
+    }
+    std::printf(
+        "{\"file\": \"%s\", \"container\": \"raw\", \"matrices\": %u, \"iters\": %d, "
+        "\"hash_matches\": %d, \"result_hash\": %u, \"best_ms_with_upload\": %.6f, "
+        "\"best_ms_chain\": %.6f, \"macs_per_chain\": %.0f}\n",
+        path, num, iters, matched, result_hash, best1, best2, macs);
+    for (hipEvent_t e : {start1, start2, stop1, stop2}) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(stream);
+    matvans_raw_free(raw);
+    return matched == iters ? 0 : 1;
 }
 
 }  // namespace
@@ -103,7 +209,7 @@ void usage() {
 int main(int argc, char **argv) {
     const char *path = nullptr;
     int iters = 10;
-    bool quiet = false;
+    bool quiet = false, raw = false;
     for (int i = 1; i < argc; i++) {
         std::string a = argv[i];
         if (a == "--help" || a == "-h") {
@@ -113,6 +219,8 @@ int main(int argc, char **argv) {
             iters = std::max(1, std::atoi(argv[++i]));
         } else if (a == "--quiet") {
             quiet = true;
+        } else if (a == "--raw") {
+            raw = true;
         } else if (!path && a[0] != '-') {
             path = argv[i];
         } else {
@@ -124,6 +232,7 @@ int main(int argc, char **argv) {
         usage();
         return 2;
     }
+    if (raw) return run_raw(path, iters, quiet);
 
     // ---- read the file (library reader) and upload every matrix once
     Arena arena;

@@ -85,19 +85,24 @@ def reference_chain(mats, w_deltas, vector):
     return out, fxhash(out)
 
 
-def requant_f32(acc):
+def requant_f32(acc, clamp=False):
     """The reference device epilogue (matvansWarpPerRow.cu:51-72): a = float(max|acc|)
     (thrust transform_reduce; the max is taken on integer-valued floats), delta = a/127
     in fp32, y = (int8) rint(float(acc) / delta) in fp32 with round-half-even, no clamp.
-    Returns (int8 y, float32 delta)."""
+    clamp=True is the int8 baseline's variant (baselines/baseline_int8/
+    baseline_matVec_custom.cu:64-86: the max over fabs as float, __float2int_rn, then
+    max(-128, min(127, .))).  Returns (int8 y, float32 delta)."""
     acc = np.asarray(acc, dtype=np.int32)
     a = np.float32(np.abs(acc.astype(np.int64)).max()) if acc.size else np.float32(0)
     delta = np.float32(a / np.float32(127))
     with np.errstate(divide="ignore", invalid="ignore"):
         q = np.rint(acc.astype(np.float32) / delta)
-    y = q.astype(np.int64).astype(np.int8) if np.all(np.isfinite(q)) else \
-        np.zeros_like(acc, dtype=np.int8)
-    return y, delta
+    if not np.all(np.isfinite(q)):
+        return np.zeros_like(acc, dtype=np.int8), delta
+    q = q.astype(np.int64)
+    if clamp:
+        q = np.clip(q, -128, 127)
+    return q.astype(np.int8), delta
 
 
 # ------------------------------------------------------------------ entropy model

@@ -54,6 +54,25 @@ int try_read(const std::string &path, const std::string &kind) {
         matvans_bin_free(b);
         return 0;
     }
+    if (kind == "raw") {
+        matvans_raw *r = nullptr;
+        int st = matvans_raw_read(path.c_str(), &r);
+        if (st) return st;
+        uint32_t num = 0, rh = 0, lv = 0;
+        const int8_t *v = nullptr;
+        matvans_raw_header(r, &num, &rh, &v, &lv);
+        volatile uint32_t sink = matvans_fxhash(v, lv);
+        for (uint32_t k = 0; k < num; k++) {
+            uint32_t rows = 0, cols = 0;
+            float gs = 0.f;
+            const int8_t *d = nullptr;
+            if (matvans_raw_matrix(r, k, &rows, &cols, &gs, &d)) continue;
+            if (size_t(rows) * cols) sink += matvans_fxhash(d, uint64_t(rows) * cols);
+        }
+        (void)sink;
+        matvans_raw_free(r);
+        return 0;
+    }
     if (kind == "mvx") {
         matvans_v1_file *f = nullptr;
         int st = matvans_v1_read(path.c_str(), &f);
@@ -95,6 +114,7 @@ int try_read(const std::string &path, const std::string &kind) {
 std::string kind_of(const std::string &f) {
     if (f.size() > 4 && f.compare(f.size() - 4, 4, ".bin") == 0) return "bin";
     if (f.size() > 4 && f.compare(f.size() - 4, 4, ".mvx") == 0) return "mvx";
+    if (f.size() > 4 && f.compare(f.size() - 4, 4, ".raw") == 0) return "raw";
     return "st";
 }
 

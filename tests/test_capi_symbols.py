@@ -28,7 +28,7 @@ def test_header_declares_entry_points():
 
 
 def test_library_exports_every_declared_symbol():
-    lib = ctypes.CDLL(capi.LIB_PATH)
+    lib = capi.lib()
     missing = [n for n in sorted(declared_functions()) if not hasattr(lib, n)]
     assert not missing, missing
 

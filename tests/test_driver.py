@@ -81,3 +81,44 @@ def test_driver_chain_on_mvx1(gpu_ready, tmp_path, streams, precision, coder):
     assert r.returncode == 0, r.stdout + r.stderr
     j = last_json(r.stdout)
     assert j["hash_matches"] == 2 and j["container"] == "MVX1"
+
+
+def test_driver_raw_usage_and_missing_file(tmp_path):
+    r = run("--raw", str(tmp_path / "nope.bin"))
+    assert r.returncode == 2 and "read raw file" in r.stderr
+    r = run("--help")
+    assert "--raw" in r.stdout
+
+
+@pytest.mark.gpu
+def test_driver_raw_int8_chain_reproduces_hash(gpu_ready, tmp_path):
+    """The int8 comparator's harness (baseline_matVec_custom.cu:434-620) on the golden raw
+    file: int8 matvec + clamped requant per matrix, FxHash of the last vector equals the
+    header's result_hash."""
+    import gzip
+    p = tmp_path / "raw-matrices_512.bin"
+    p.write_bytes(gzip.decompress(open(os.path.join(GOLDEN, "raw-matrices_512.bin.gz"),
+                                       "rb").read()))
+    r = run("--raw", str(p), "--iters", "3")
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("Hashes match!") == 3
+    j = last_json(r.stdout)
+    assert j["hash_matches"] == 3 and j["container"] == "raw"
+
+
+@pytest.mark.gpu
+def test_python_raw_int8_chain_reproduces_hash(gpu_ready, tmp_path):
+    """The same chain through the Python bindings: library raw reader, int8 comparator,
+    clamped device requant."""
+    import gzip
+    import torch
+    import matvans_amd as mv
+    p = tmp_path / "raw-matrices_512.bin"
+    p.write_bytes(gzip.decompress(open(os.path.join(GOLDEN, "raw-matrices_512.bin.gz"),
+                                       "rb").read()))
+    hdr, vec, mats = mv.read_raw(p)
+    v = torch.from_numpy(vec).cuda()
+    for m in mats:
+        acc = mv.int8_matvec(torch.from_numpy(m).cuda(), v)
+        v, _ = mv.absmax_requant(acc, clamp=True)
+    assert mv.hash_int8_array(v.cpu().numpy()) == hdr["result_hash"]

@@ -4,11 +4,13 @@ by the library, 1..8 vectors, ragged tails and the int8 epilogue."""
 import os
 import sys
 
+import ctypes
+
 import numpy as np
 import pytest
 
 from matvans_amd.buffered import BufferedMatrices, read_safetensors
-from matvans_amd.capi import MatvansError
+from matvans_amd.capi import MatvansError, call
 from oracle import family_b as B
 
 from .helpers import gaussian_int8, laplace_int8, uniform_vec
@@ -76,8 +78,15 @@ def test_argument_errors(cuda):
     note("family-B argument validation")
     m = gaussian_int8(32, 256, 3.0)[0]
     bm = BufferedMatrices.encode([m], "rows").to_device()
-    with pytest.raises(MatvansError) as e:         # 9 vectors
-        bm.mat_vec(0, torch.zeros((9, 256), dtype=torch.int8, device="cuda"))
+    v9 = torch.zeros((9, 256), dtype=torch.int8, device="cuda")
+    with pytest.raises(ValueError, match="9 vectors"):   # the binding's check
+        bm.mat_vec(0, v9)
+    acc = torch.empty((9, 32), dtype=torch.int32, device="cuda")
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    with pytest.raises(MatvansError) as e:         # the C ABI's own check: 9 vectors
+        call("matvans_b_rows_matvec", p(bm.d["compressed"]), bm.compressed.size,
+             p(bm.d["offsets"][0]), bm.end_offset(0), p(bm.d["ppf"]), p(bm.d["cum"]),
+             p(bm.d["prob"]), p(v9), 9, 256, 256, 32, 1.0, None, p(acc), None)
     assert e.value.status == 1
     bm.shapes[0] = (32, 200)                       # rows layout needs cols % 128 == 0
     with pytest.raises(MatvansError) as e:

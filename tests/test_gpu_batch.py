@@ -208,8 +208,22 @@ def test_argument_errors(cuda):
     note("batch argument validation")
     m = gaussian_int8(8, 2048, 6.0)[0]
     c = make("q32", m)
-    with pytest.raises(MatvansError) as e:       # 9 vectors
-        Batch([(c, torch.zeros((9, 2048), dtype=torch.int8, device="cuda"))])
+    v9 = torch.zeros((9, 2048), dtype=torch.int8, device="cuda")
+    with pytest.raises(ValueError, match="9 vectors"):   # the binding's check
+        Batch([(c, v9)])
+    # the C ABI's own check on the same item
+    import ctypes
+    from matvans_amd.capi import BatchItem, call
+    dst = torch.empty((9, 8), dtype=torch.int32, device="cuda")
+    it = (BatchItem * 1)()
+    it[0].format, it[0].rows, it[0].cols, it[0].n_vec = 1, 8, 2048, 9
+    it[0].vec_stride, it[0].vecs, it[0].dst = 2048, v9.data_ptr(), dst.data_ptr()
+    it[0].cursors, it[0].payload, it[0].payload_size = (c.d_cursors.data_ptr(),
+                                                        c.d_payload.data_ptr(), c.payload_size)
+    it[0].streams, it[0].lut = 32, c.d_lut.data_ptr()
+    h = ctypes.c_void_p()
+    with pytest.raises(MatvansError) as e:
+        call("matvans_batch_create", it, 1, ctypes.byref(h))
     assert e.value.status == 1
 
 

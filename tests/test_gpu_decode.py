@@ -189,8 +189,17 @@ def test_argument_errors_do_not_launch(cuda):
         mv.decmpressAndMultiply(dst, dv, 8, 500, 1.0, cm.d_cursors, cm.min_value, cm.G,
                                 cm.d_cdf_data, cm.d_ppf_data, cm.d_payload, cm.payload_size)
     assert e.value.status == 2
-    with pytest.raises(MatvansError) as e:   # 9 vectors
-        cm.multiply(torch.zeros((9, 512), dtype=torch.int8, device="cuda"))
+    v9 = torch.zeros((9, 512), dtype=torch.int8, device="cuda")
+    with pytest.raises(ValueError, match="9 vectors"):   # the binding's check
+        cm.multiply(v9)
+    out9 = torch.empty((9, 8), dtype=torch.int32, device="cuda")
+    with pytest.raises(MatvansError) as e:   # the C ABI's own check: 9 vectors
+        mv.capi.call("matvans_decompress_and_multiply_nvec", ctypes.c_void_p(out9.data_ptr()),
+                     ctypes.c_void_p(v9.data_ptr()), 9, 512, 8, 512,
+                     ctypes.c_void_p(cm.d_cursors.data_ptr()), cm.min_value, cm.G,
+                     ctypes.c_void_p(cm.d_cdf_data.data_ptr()),
+                     ctypes.c_void_p(cm.d_ppf_data.data_ptr()),
+                     ctypes.c_void_p(cm.d_payload.data_ptr()), cm.payload_size, None)
     assert e.value.status == 1
 
 

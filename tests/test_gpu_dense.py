@@ -75,9 +75,8 @@ def test_absmax_requant(cuda, clamp):
                 np.zeros(7, np.int32)] + edge_max_cases(rng):
         out8, d = mv.absmax_requant(torch.from_numpy(acc).cuda(), clamp=clamp)
         torch.cuda.synchronize()
-        ref8, refd = O.requant_f32(acc)
-        if not clamp:
-            assert np.array_equal(out8.cpu().numpy(), ref8)
+        ref8, refd = O.requant_f32(acc, clamp=clamp)
+        assert np.array_equal(out8.cpu().numpy(), ref8)
         if acc.any():
             assert d.cpu().numpy()[0] == refd
         else:

@@ -97,8 +97,12 @@ def test_argument_errors(cuda):
     m, _ = gaussian_int8(8, 2048, 6.0)
     c = CompressedMatrixV1.encode(m, 64, 8, 0).to_device()
     v = torch.zeros((9, 2048), dtype=torch.int8, device="cuda")
-    with pytest.raises(MatvansError) as e:      # 9 vectors
+    with pytest.raises(ValueError, match="9 vectors"):   # the binding's check
         c.multiply(v)
+    out9 = torch.empty((9, 8), dtype=torch.int32, device="cuda")
+    with pytest.raises(MatvansError) as e:      # the C ABI's own check: 9 vectors
+        call("matvans_v1_decompress_and_multiply", ctypes.byref(c.dev_struct()),
+             ctypes.c_void_p(out9.data_ptr()), ctypes.c_void_p(v.data_ptr()), 9, 2048, None)
     assert e.value.status == 1
     s = c.dev_struct()
     s.table_mode = 1

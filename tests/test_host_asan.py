@@ -36,6 +36,11 @@ def fuzz_binary(tmp_path_factory):
 def test_readers_and_encoders_under_sanitizers(fuzz_binary, tmp_path):
     files = sorted(GOLDEN.glob("*.bin")) + sorted(GOLDEN.glob("*.safetensors"))
     assert files, "golden files missing"
+    # the raw baseline file, unpacked (the reader takes plain files)
+    import gzip
+    raw = tmp_path / "raw-matrices_512.raw"
+    raw.write_bytes(gzip.decompress((GOLDEN / "raw-matrices_512.bin.gz").read_bytes()))
+    files.append(raw)
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0",
                UBSAN_OPTIONS="print_stacktrace=1")
     r = subprocess.run([str(fuzz_binary), str(tmp_path), *map(str, files)],

@@ -122,3 +122,45 @@ def test_encoder_error_paths():
     with pytest.raises(MatvansError) as e:
         mv.encode_rows(m, 0, np.array([0, 255, 256], np.uint16))
     assert e.value.status == 7       # MATVANS_E_RANGE
+
+
+def test_raw_reader_matches_golden_file(golden_dir, tmp_path):
+    """The library's raw-container reader (the int8 baseline's input) returns the header,
+    vector and matrices the oracle reads from the golden raw file the reference generator
+    wrote; every truncation is rejected with a status code."""
+    import gzip
+    from matvans_amd.capi import MatvansError
+    data = gzip.decompress(open(os.path.join(golden_dir, "raw-matrices_512.bin.gz"), "rb").read())
+    p = tmp_path / "raw-matrices_512.bin"
+    p.write_bytes(data)
+    hdr, vec, mats = mv.read_raw(p)
+    num, rh, ovec, omats = O.read_raw_bin(os.path.join(golden_dir, "raw-matrices_512.bin.gz"))
+    assert hdr == {"num_matrices": num, "result_hash": rh}
+    assert np.array_equal(vec, ovec)
+    assert len(mats) == len(omats)
+    for a, b in zip(mats, omats):
+        assert np.array_equal(a, b)
+    for n in (0, 7, 11, 12 + len(vec) - 1, len(data) - 1):
+        q = tmp_path / f"cut{n}.bin"
+        q.write_bytes(data[:n])
+        with pytest.raises(MatvansError):
+            mv.read_raw(q)
+
+
+def test_raw_reader_odd_matrix_and_unaligned_vector(tmp_path):
+    """A raw file with a vector length that is not a multiple of 4 (padded by the writer)
+    and an odd-sized matrix (one pad byte), written as create_matrices_WarpPerRow.py:94-129
+    writes it."""
+    import struct
+    rng = np.random.default_rng(3)
+    vec = rng.integers(-127, 128, 6).astype(np.int8)
+    m1 = rng.integers(-127, 128, (3, 6)).astype(np.int8)
+    m2 = rng.integers(-127, 128, (5, 3)).astype(np.int8)
+    b = struct.pack("<LL", 2, 1234) + struct.pack("<L", 6) + vec.tobytes() + b"\0\0"
+    for m in (m1, m2):
+        b += struct.pack("<LLf", *m.shape, 1.0) + m.tobytes() + (b"\0" if m.size % 2 else b"")
+    p = tmp_path / "odd.bin"
+    p.write_bytes(b)
+    hdr, v, mats = mv.read_raw(p)
+    assert hdr["result_hash"] == 1234 and np.array_equal(v, vec)
+    assert np.array_equal(mats[0], m1) and np.array_equal(mats[1], m2)

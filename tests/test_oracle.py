@@ -234,3 +234,20 @@ def test_python_coder_round_trip_small():
             x = (x << 16) | stream[pos]
             pos += 1
     assert out == list(syms) and pos == len(stream) and x == 1 << 16
+
+
+def test_clamped_requant_chain_reproduces_raw_file_hash(golden_dir):
+    """The int8 baseline's chain (int32 matvec, absmax/127, clamped rint; baseline_matVec_
+    custom.cu:64-86,399-430) over the golden raw file reproduces its header result_hash
+    (:574-586), and the clamp never changes a value: |float(acc) / (max/127)| <= 127."""
+    num, rh, vec, mats = O.read_raw_bin(os.path.join(golden_dir, "raw-matrices_512.bin.gz"))
+    v = vec
+    for m in mats:
+        acc = O.matvec_i32(m, v)
+        v, _ = O.requant_f32(acc, clamp=True)
+        assert np.array_equal(v, O.requant_f32(acc)[0])
+    assert O.fxhash(v) == rh
+    rng = np.random.default_rng(9)
+    for acc in (rng.integers(-2**31 + 1, 2**31 - 1, 5000).astype(np.int32),
+                np.array([2**31 - 1, -2**31 + 1, 7], np.int32)):
+        assert np.array_equal(O.requant_f32(acc, clamp=True)[0], O.requant_f32(acc)[0])
