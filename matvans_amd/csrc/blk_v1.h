@@ -5,6 +5,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "v1_stream.h"
 
 namespace mv {
@@ -83,13 +85,13 @@ struct V1Smem {
     int32_t part[WPR > 1 ? kWaves * NV : 1];
 };
 
-template <int NV, int S, int MODE = 0>
+template <int NV, int S, int MODE = 0, int SETW = 4>
 struct QuadSmem {
     static constexpr int WPR = S > 64 ? S / 64 : 1;
-    uint32_t ring[kWaves * 16 * 64];
+    uint32_t ring[kWaves * 4 * SETW * 64];
     uint32_t tab[MODE == 0 ? 256 : 1];
     uint32_t rowtab[MODE == 1 ? kWaves * 448 : 1];   // per wave: packed table + scratch
-    uint32_t dummy[4 * 64];
+    uint32_t dummy[SETW * 64];
     int32_t part[WPR > 1 ? kWaves * NV : 1];
 };
 
@@ -316,15 +318,17 @@ __device__ __forceinline__ void v1_block(const V1Job &J, uint32_t bid, V1Smem<NV
 }
 
 // Quad coder (MVX1 coder 1, P = 8, one table per matrix): same mapping, per-lane 64-bit
-// state; a 16-symbol group is four quads, each one v_dot4 per vector.
-template <int NV, int S, int MODE = 0>
+// state; a 16-symbol group is four quads, each one v_dot4 per vector.  SETW words per
+// lane per ring set: 4 (a set per group) or 8 (a set per two groups, twice the ring).
+template <int NV, int S, int MODE = 0, int SETW = 4>
 __device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid,
-                                           QuadSmem<NV, S, MODE> &sm) {
+                                           QuadSmem<NV, S, MODE, SETW> &sm) {
     static_assert(MODE == 0 || S == 64, "one table per row needs one row per wave");
     constexpr int RPW = S < 64 ? 64 / S : 1;
     constexpr int WPR = S > 64 ? S / 64 : 1;
     constexpr int G4 = 128 / S;
-    using QL = v1::QuadLane<G4>;
+    constexpr int GPS = SETW / 4;      // groups per set period
+    using QL = v1::QuadLane<G4, SETW>;
     uint32_t *ring = sm.ring;
     uint32_t *dummy = sm.dummy;
     int32_t *part = sm.part;
@@ -374,15 +378,24 @@ __device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid,
     st.prologue();
 
     const uint32_t groups = cols / (16u * S);
-    auto vload = [&](uint32_t t, uint4 *c) {
-        uint32_t tt = t < groups ? t : groups - 1;
+    // the vectors through a buffer resource: the lane's byte offset is fixed, the group's
+    // offset is wave-uniform (scalar), so a load costs no vector address arithmetic
+    const uint32_t nv_eff = J.n_vec < uint32_t(NV) ? J.n_vec : uint32_t(NV);
+    const __amdgpu_buffer_rsrc_t vrs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<int8_t *>(vecs), 0,
+        static_cast<int>((nv_eff - 1) * vec_stride + cols), 0x00020000);
+    uint32_t voff[NV];
 #pragma unroll
-        for (int v = 0; v < NV; v++)
-            c[v] = *reinterpret_cast<const uint4 *>(vecs +
-                                                    size_t(v < int(J.n_vec) ? v : 0) * vec_stride +
-                                                    tt * (16u * S) + sl * 16u);
+    for (int v = 0; v < NV; v++) voff[v] = (v < int(nv_eff) ? uint32_t(v) : 0u) * vec_stride + sl * 16u;
+    auto vload = [&](uint32_t t, uint32_t (&c)[NV][4]) {
+        const uint32_t tt = t < groups ? t : groups - 1;
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            auto x = __builtin_amdgcn_raw_buffer_load_b128(vrs, voff[v], tt * (16u * S), 0);
+            c[v][0] = x[0]; c[v][1] = x[1]; c[v][2] = x[2]; c[v][3] = x[3];
+        }
     };
-    uint4 vc0[NV], vc1[NV];
+    uint32_t vc0[NV][4], vc1[NV][4];
     // load order vc0, granules A, granules B, vc1 as in the loop body (A, vc0, B, vc1), so
     // that the loop's wait before group A leaves the two newest loads (B, vc1) in flight
     vload(0, vc0);
@@ -393,31 +406,54 @@ __device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid,
     int32_t acc[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) acc[v] = 0;
-    auto group = [&](const uint4 *vc) {
-        uint32_t wbefore = st.waddr;
+    auto group = [&](const uint32_t (&vc)[NV][4]) {
 #pragma unroll
         for (int d = 0; d < 4; d++) {
             uint32_t packed = st.quad(tab);
 #pragma unroll
             for (int v = 0; v < NV; v++)
                 acc[v] = __builtin_amdgcn_sdot4(static_cast<int>(packed),
-                                                static_cast<int>(dword_of(vc[v], d)), acc[v],
-                                                false);
+                                                static_cast<int>(vc[v][d]), acc[v], false);
+        }
+    };
+    // a set period: GPS groups after one refill; the ring position advance is folded into
+    // the consumed-word count once per period (at most 4 * GPS words < R4)
+    // (par: the parity of t0, which picks the vector buffer of each group statically)
+    auto one = [&](auto par, uint32_t tg) {
+        if constexpr (decltype(par)::value == 0) {
+            group(vc0);
+            vload(tg + 2, vc0);
+        } else {
+            group(vc1);
+            vload(tg + 2, vc1);
+        }
+    };
+    static_assert(GPS == 1 || GPS == 2, "set period of one or two groups");
+    auto period = [&](auto par, uint32_t t0, uint32_t ng) {
+        constexpr int P = decltype(par)::value;
+        uint32_t wbefore = st.waddr;
+        one(std::integral_constant<int, P % 2>{}, t0);
+        if constexpr (GPS == 2) {
+            if (ng > 1) one(std::integral_constant<int, (P + 1) % 2>{}, t0 + 1);
         }
         st.end_group(wbefore);
     };
+    using Even = std::integral_constant<int, 0>;
+    using Second = std::integral_constant<int, GPS % 2>;
     uint32_t t = 0;
-    for (; t + 1 < groups; t += 2) {
+    for (; t + 2 * GPS <= groups; t += 2 * GPS) {
         st.refill_a();
-        group(vc0);
-        vload(t + 2, vc0);
+        period(Even{}, t, GPS);
         st.refill_b();
-        group(vc1);
-        vload(t + 3, vc1);
+        period(Second{}, t + GPS, GPS);
     }
     if (t < groups) {
         st.refill_a();
-        group(vc0);
+        period(Even{}, t, groups - t < uint32_t(GPS) ? groups - t : uint32_t(GPS));
+        if (t + GPS < groups) {
+            st.refill_b();
+            period(Second{}, t + GPS, groups - t - GPS);
+        }
     }
 
 #pragma unroll

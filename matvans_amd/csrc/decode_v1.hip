@@ -36,10 +36,17 @@ __global__ __launch_bounds__(256) void v1_fwd(V1Job job) {
     blk::v1_block<NV, S, P, MODE>(job, blockIdx.x, sm);
 }
 
+// ring words per lane per set of the one-matrix quad kernels: two groups per set for 32
+// streams per row (halves the ring bookkeeping; 2 waves per SIMD at 4096 rows leave LDS
+// to spare), one group per set for 64 and 128 streams (more resident waves per CU)
+template <int S>
+constexpr int quad_set_words() { return S == 32 ? 8 : 4; }
+
 template <int NV, int S, int MODE>
 __global__ __launch_bounds__(256) void v1q_fwd(V1Job job) {
-    __shared__ blk::QuadSmem<NV, S, MODE> sm;
-    blk::quad_block<NV, S, MODE>(job, blockIdx.x, sm);
+    constexpr int W = quad_set_words<S>();
+    __shared__ blk::QuadSmem<NV, S, MODE, W> sm;
+    blk::quad_block<NV, S, MODE, W>(job, blockIdx.x, sm);
 }
 
 V1Job job_of(const V1Args &a) {

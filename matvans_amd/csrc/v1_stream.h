@@ -263,22 +263,25 @@ struct Lane {
 };
 
 // Quad coder (coder 1): per-lane 64-bit state {lo, hi}, 32-bit words in a ring of
-// R4 = 16 u32 words per lane (lane-interleaved: word slot w of lane i at (w*64 + i)*4
-// bytes of the wave's 4 KiB ring).  A 16-symbol group is four quads and takes at most
-// M = 4 words.  The ring is filled in sets of M words per lane: NG = M / G4 consecutive
-// granules (G4 = 128/S words per stream per granule), loaded two groups ahead under one
-// fit check, one ring position and one validity marker per set whatever S is.  The fit
-// rule and the no-unwritten-slot bound are those of Lane with a set as the unit
-// (R4 = 16 >= 3M + M).  A set may run past the row's last granule: those loads read the
+// R4 = 4 * SETW u32 words per lane (lane-interleaved: word slot w of lane i at
+// (w*64 + i)*4 bytes of the wave's ring).  A 16-symbol group is four quads and takes at
+// most 4 words.  The ring is filled in sets of M = SETW words per lane (SETW = 4: one set
+// per group; SETW = 8: one set per two groups, which halves the per-set bookkeeping at
+// twice the ring): NG = M / G4 consecutive granules (G4 = 128/S words per stream per
+// granule), loaded two set periods ahead under one fit check, one ring position and one
+// validity marker per set whatever S is.  A set period consumes at most M words, so the
+// fit rule and the no-unwritten-slot bound are those of Lane with a set as the unit
+// (R4 = 4M >= 3M + M).  A set may run past the row's last granule: those loads read the
 // next row's words inside the window or return zeros past it (buffer range check), into
 // ring slots the fit check has shown to be free, and the lane never consumes them.
-template <int G4>
+template <int G4, int SETW = 4>
 struct QuadLane {
-    static constexpr int R4 = 16;
+    static constexpr int M = SETW;
+    static constexpr int R4 = 4 * SETW;
     static constexpr uint32_t kRingWaveBytes = R4 * 64 * 4;
-    static constexpr int M = 4;
     static constexpr int NG = M / G4;
     static_assert(R4 >= 3 * M + M, "ring too small");
+    static_assert(NG >= 1 && (R4 & (R4 - 1)) == 0, "set shape");
     static constexpr int kPreSets = (R4 * 3 / 4) / M;
 
     __amdgpu_buffer_rsrc_t rsrc;
@@ -393,8 +396,10 @@ struct QuadLane {
         bool need = hi == 0;
         hi = need ? lo : hi;
         lo = need ? W : lo;
+        // the add may carry into the wave's base bits; the high bits are taken from the
+        // old address (one bit-field insert)
         uint32_t t = waddr + (need ? 256u : 0u);
-        waddr = (t & (kRingWaveBytes - 1)) | (wave_base & ~(kRingWaveBytes - 1));
+        waddr = (t & (kRingWaveBytes - 1)) | (waddr & ~(kRingWaveBytes - 1));
         W = *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(ring) + waddr);
 #pragma unroll
         for (int i = 0; i < 4; i++) e[i] = tab[(lo >> (8 * i)) & 0xFFu];
