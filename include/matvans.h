@@ -345,6 +345,21 @@ int matvans_v1_decompress_backward(const matvans_v1_dev_matrix *m, float *dx,
                                    const float *s_w, const float *alpha, float *workspace,
                                    size_t workspace_bytes, void *stream);
 
+/* No counterpart in the reference: an entry point that allocates nothing, for callers
+ * that cannot call hipMalloc (code under HIP-graph capture, tight serving loops,
+ * harnesses that forbid allocation).  The same backward as
+ * matvans_v1_decompress_backward (same kernels, bitwise the same dx), with the 64-row
+ * partial sums in a library-owned static device buffer of MATVANS_NOALLOC_SCRATCH_BYTES
+ * per GPU instead of a caller workspace.  MATVANS_E_CAPACITY when
+ * matvans_backward_workspace_size(rows, cols, n_vec) exceeds it (e.g. 4096 x 14336 and
+ * 14336 x 4096 fit with up to 8 vectors).  Every call on one GPU shares that buffer, and
+ * it keeps the last call's contents (each call overwrites all it reads): calls ordered on
+ * one stream are safe, calls that overlap in time on one GPU race. */
+#define MATVANS_NOALLOC_SCRATCH_BYTES (32u << 20)
+int matvans_v1_decompress_backward_noalloc(const matvans_v1_dev_matrix *m, float *dx,
+                                           const float *g, uint32_t n_vec, uint32_t g_stride,
+                                           const float *s_w, const float *alpha, void *stream);
+
 
 /* ------------------------------------------------------------------------------
  * A13. The reference's family-B codec ("buffered"): quad rANS with 7-bit probabilities,

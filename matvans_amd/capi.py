@@ -133,6 +133,8 @@ SIGNATURES = {
     "matvans_upload_free": (None, [P]),
     "matvans_v1_decompress_backward": (i32, [ctypes.POINTER(V1DevMatrix), P, P, u32, u32, P, P,
                                              P, ctypes.c_size_t, P]),
+    "matvans_v1_decompress_backward_noalloc": (i32, [ctypes.POINTER(V1DevMatrix), P, P, u32, u32,
+                                                     P, P, P]),
     "matvans_b_rows_matvec": (i32, [P, u64, P, u32, P, P, P, P, u32, u32, u32, u32, f32, P, P, P]),
     "matvans_b_stripes_matvec": (i32, [P, u64, P, u32, P, P, P, P, u32, u32, u32, u32, f32, P, P,
                                        P]),

@@ -11,8 +11,17 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "blk_bwd.h"
 #include "launch.h"
+#include "matvans.h"
+
+// The partial sums of matvans_v1_decompress_backward_noalloc: a file-scope device array
+// (one per GPU the library runs on, held until the process ends).  Host code never passes
+// this name to a kernel or a copy: it is a host-side stand-in, not a device address.  The
+// device address comes from hipGetSymbolAddress once per device (backward_scratch).
+static __device__ __align__(256) float scratch[MATVANS_NOALLOC_SCRATCH_BYTES / sizeof(float)];
 
 namespace mv {
 namespace {
@@ -77,6 +86,25 @@ hipError_t quad_bwd_s(const BwdJob &j, hipStream_t s) {
 }
 
 }  // namespace
+
+hipError_t backward_scratch(float **ptr) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void *addr[kMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[dev] = p;
+    }
+    *ptr = static_cast<float *>(addr[dev]);
+    return hipSuccess;
+}
 
 size_t v0_bwd_workspace_bytes(uint32_t rows, uint32_t cols, uint32_t n_vec) {
     return size_t((rows + 63) / 64) * cols * n_vec * sizeof(float);

@@ -291,13 +291,11 @@ extern "C" int matvans_decompress_backward(float *dx, const float *g, uint32_t n
     return hip_status(mv::launch_v0_backward(a, as_stream(stream)), "backward launch");
 }
 
-extern "C" int matvans_v1_decompress_backward(const matvans_v1_dev_matrix *m, float *dx,
-                                              const float *g, uint32_t n_vec, uint32_t g_stride,
-                                              const float *s_w, const float *alpha,
-                                              float *workspace, size_t workspace_bytes,
-                                              void *stream) {
-    if (!m || !dx || !g || !s_w || !alpha || !workspace)
-        return fail(MATVANS_E_INVALID, "null argument");
+namespace {
+// Argument checks shared by the quad backward entry points (all but the workspace).
+int check_v1_backward(const matvans_v1_dev_matrix *m, const float *dx, const float *g,
+                      uint32_t n_vec, uint32_t g_stride, const float *s_w, const float *alpha) {
+    if (!m || !dx || !g || !s_w || !alpha) return fail(MATVANS_E_INVALID, "null argument");
     if (m->coder != 1 || m->precision != 8 || m->table_mode > 1 ||
         (m->streams != 32 && m->streams != 64 && m->streams != 128) ||
         (m->table_mode == 1 && m->streams != 64))
@@ -314,15 +312,54 @@ extern "C" int matvans_v1_decompress_backward(const matvans_v1_dev_matrix *m, fl
                     m->cols, 16 * m->streams);
     if (n_vec > 1 && g_stride < m->rows)
         return fail(MATVANS_E_INVALID, "g_stride %u < rows", g_stride);
-    if (!aligned(m->payload, 16) || !aligned(workspace, 16) || !aligned(dx, 4))
+    if (!aligned(m->payload, 16) || !aligned(dx, 4))
+        return fail(MATVANS_E_ALIGN, "payload must be 16-byte aligned");
+    return MATVANS_OK;
+}
+
+mv::QuadBwdArgs v1_backward_args(const matvans_v1_dev_matrix *m, float *dx, const float *g,
+                                 uint32_t n_vec, uint32_t g_stride, const float *s_w,
+                                 const float *alpha, float *partial) {
+    return mv::QuadBwdArgs{dx, partial, g, n_vec, n_vec > 1 ? g_stride : m->rows, s_w, alpha,
+                           m->rows, m->cols, m->streams, m->cursors, m->lut, m->payload,
+                           m->payload_size, m->table_mode, m->freq, int32_t(m->min_value),
+                           uint32_t(m->G)};
+}
+}  // namespace
+
+extern "C" int matvans_v1_decompress_backward(const matvans_v1_dev_matrix *m, float *dx,
+                                              const float *g, uint32_t n_vec, uint32_t g_stride,
+                                              const float *s_w, const float *alpha,
+                                              float *workspace, size_t workspace_bytes,
+                                              void *stream) {
+    if (!workspace) return fail(MATVANS_E_INVALID, "null argument");
+    int st = check_v1_backward(m, dx, g, n_vec, g_stride, s_w, alpha);
+    if (st != MATVANS_OK) return st;
+    if (!aligned(workspace, 16))
         return fail(MATVANS_E_ALIGN, "payload and workspace must be 16-byte aligned");
     size_t need = mv::v0_bwd_workspace_bytes(m->rows, m->cols, n_vec);
     if (workspace_bytes < need)
         return fail(MATVANS_E_CAPACITY, "workspace %zu bytes < %zu", workspace_bytes, need);
-    mv::QuadBwdArgs a{dx, workspace, g, n_vec, n_vec > 1 ? g_stride : m->rows, s_w, alpha,
-                      m->rows, m->cols, m->streams, m->cursors, m->lut, m->payload,
-                      m->payload_size, m->table_mode, m->freq, int32_t(m->min_value),
-                      uint32_t(m->G)};
+    const mv::QuadBwdArgs a = v1_backward_args(m, dx, g, n_vec, g_stride, s_w, alpha, workspace);
+    return hip_status(mv::launch_quad_backward(a, as_stream(stream)), "quad backward launch");
+}
+
+extern "C" int matvans_v1_decompress_backward_noalloc(const matvans_v1_dev_matrix *m, float *dx,
+                                                      const float *g, uint32_t n_vec,
+                                                      uint32_t g_stride, const float *s_w,
+                                                      const float *alpha, void *stream) {
+    int st = check_v1_backward(m, dx, g, n_vec, g_stride, s_w, alpha);
+    if (st != MATVANS_OK) return st;
+    const size_t need = mv::v0_bwd_workspace_bytes(m->rows, m->cols, n_vec);
+    if (need > MATVANS_NOALLOC_SCRATCH_BYTES)
+        return fail(MATVANS_E_CAPACITY,
+                    "%u x %u with %u vectors needs %zu bytes of partial sums; the static "
+                    "scratch holds %zu (use matvans_v1_decompress_backward with a workspace)",
+                    m->rows, m->cols, n_vec, need, size_t(MATVANS_NOALLOC_SCRATCH_BYTES));
+    float *scratch = nullptr;
+    st = hip_status(mv::backward_scratch(&scratch), "static scratch address");
+    if (st != MATVANS_OK) return st;
+    const mv::QuadBwdArgs a = v1_backward_args(m, dx, g, n_vec, g_stride, s_w, alpha, scratch);
     return hip_status(mv::launch_quad_backward(a, as_stream(stream)), "quad backward launch");
 }
 

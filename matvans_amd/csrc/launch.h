@@ -92,6 +92,9 @@ struct QuadBwdArgs {
 };
 
 hipError_t launch_quad_backward(const QuadBwdArgs &a, hipStream_t s);
+// Device address of the backward's static scratch (MATVANS_NOALLOC_SCRATCH_BYTES) on the
+// current device, looked up once per device.
+hipError_t backward_scratch(float **ptr);
 hipError_t launch_quantize_vectors(const float *x, uint32_t n_vec, uint32_t x_stride,
                                    uint32_t cols, int8_t *xq, uint32_t xq_stride, float *s_x,
                                    hipStream_t s);
