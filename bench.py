@@ -75,6 +75,8 @@ FORMATS = {
     "q64": (64, 8, 0, 1),
     "q128": (128, 8, 0, 1),
     "q64-rowcdf": (64, 8, 1, 1),
+    "q32x": (32, 8, 0, 1, "auto"),
+    "q64x": (64, 8, 0, 1, "auto"),
     "b-rows": "rows",
     "b-stripes": "stripes",
 }
@@ -89,11 +91,15 @@ FORMAT_TEXT = {
     "q64": "MVX1 quad coder: 64 streams/row, 64-bit state, global 8-bit CDF",
     "q128": "MVX1 quad coder: 128 streams/row, 64-bit state, global 8-bit CDF",
     "q64-rowcdf": "MVX1 quad coder: 64 streams/row, 64-bit state, per-row 8-bit CDF",
+    "q32x": "MVX1 quad coder: 32 streams/row, global 8-bit CDF over a band, entries beyond "
+            "it coded as 0 and listed per row (outlier split)",
+    "q64x": "MVX1 quad coder: 64 streams/row, global 8-bit CDF over a band, outlier split",
     "b-rows": "family B (buffered) ans-warp-rows: quad rANS, 7-bit CDF, 32 coders/row",
     "b-stripes": "family B (buffered) ans-warp-stripes: quad rANS, 7-bit CDF, 32 rows/warp",
 }
-BACKWARD_FORMATS = ("v0", "q32", "q64", "q64-rowcdf")
-DEFAULT_FORMATS = ["v0", "q32", "q64", "q128", "q64-rowcdf", "s128-p8", "s64-p8", "s64-p12",
+BACKWARD_FORMATS = ("v0", "q32", "q64", "q64-rowcdf", "q32x", "q64x")
+DEFAULT_FORMATS = ["v0", "q32", "q64", "q128", "q32x", "q64x", "q64-rowcdf", "s128-p8", "s64-p8",
+                   "s64-p12",
                    "s64-p8-rowcdf",
                    "s32-p12", "b-rows"]
 
@@ -205,7 +211,8 @@ def exact_product(m, v):
 def _v1_copy(c):
     from matvans_amd.mvx import CompressedMatrixV1
     return CompressedMatrixV1(c.rows, c.cols, c.streams, c.precision, c.table_mode, c.min_value,
-                              c.G, c.cursors, c.payload, c.cdf, c.freq, coder=c.coder)
+                              c.G, c.cursors, c.payload, c.cdf, c.freq, coder=c.coder,
+                              out_start=c.out_start, out_rec=c.out_rec)
 
 
 def _v0_copy(c):
@@ -650,11 +657,12 @@ def main():
             return cm, cm.payload_size * 2 + rows * 4 + (cm.G + 1) + 256, lambda: mv.CompressedMatrix(
                 cm.rows, cm.cols, 1.0, cm.cursors, cm.min_value, cm.G, cm.cdf_data, cm.ppf_data,
                 cm.payload).to_device(dev)
-        S, P, mode, coder = (FORMATS[fmt] + (0,))[:4]
-        c = CompressedMatrixV1.encode(w, S, P, mode, coder=coder)
-        return c, c.stored_bytes, lambda: CompressedMatrixV1(
-            c.rows, c.cols, S, P, mode, c.min_value, c.G, c.cursors, c.payload, c.cdf,
-            c.freq, coder=coder).to_device(dev)
+        spec = FORMATS[fmt]
+        S, P, mode = spec[:3]
+        coder = spec[3] if len(spec) > 3 else 0
+        outl = spec[4] if len(spec) > 4 else None
+        c = CompressedMatrixV1.encode(w, S, P, mode, coder=coder, outliers=outl)
+        return c, c.stored_bytes, lambda: _v1_copy(c).to_device(dev)
 
     def time_format(fmt):
         try:

@@ -268,6 +268,11 @@ typedef struct {
     const uint16_t *cdf;        /* table_mode 0: [G+1] */
     const uint8_t *freq;        /* table_mode 1: [rows][G] */
     const uint16_t *payload;    /* [payload_size] */
+    /* outlier split (quad coder, table_mode 0; n_outliers 0: none): entries outside the
+     * coded band are 0 in the streams and listed here, by row */
+    uint32_t n_outliers;
+    const uint32_t *out_start;  /* [rows + 1] offsets into out_rec */
+    const uint32_t *out_rec;    /* [n_outliers] col | (uint8)value << 24, rows in order */
 } matvans_v1_host;
 
 /* Model for the given precision and table mode.  table_mode 0 writes cdf[G+1] (room for
@@ -284,13 +289,25 @@ int matvans_v1_encode(const int8_t *mat, uint32_t rows, uint32_t cols, uint32_t 
                       int32_t min_value, uint32_t G, const uint16_t *cdf, const uint8_t *freq,
                       uint32_t n_threads, uint32_t *cursors, uint16_t *payload,
                       uint64_t payload_cap, uint64_t *payload_size);
+/* Outlier split (no reference counterpart; the reference's 8-bit model gives each of a
+ * long tail's rare values at least 1/256, create_matrices_WarpPerRow.py:154).  band: the
+ * T in 1..127 minimising the estimated stored bits of the 8-bit quad encoding of the
+ * entries in [-T, T] with every entry beyond coded as 0 plus a 32-bit record (127: none).
+ * split: coded = mat with |w| > band set to 0; out_start[rows + 1] and the records
+ * out_rec[] = col | (uint8)w << 24 in row-major order (cap records at most). */
+int matvans_v1_outlier_band(const int8_t *mat, uint64_t n, int32_t *band);
+int matvans_v1_split_outliers(const int8_t *mat, uint32_t rows, uint32_t cols, int32_t band,
+                              int8_t *coded, uint32_t *out_start, uint32_t *out_rec,
+                              uint64_t cap, uint64_t *n_out);
 /* Decode table of a table_mode 0 matrix, 2 * 2^P entries: A[q] = f | (uint8)(min+s) << 24,
  * then B[q] = q - cdf[s]. */
 int matvans_v1_lut(const uint16_t *cdf, uint32_t G, int32_t min_value, uint32_t precision,
                    uint32_t *lut);
 
 typedef struct matvans_v1_file matvans_v1_file;
-/* MVX1 container: "MVX1", u32 version 1, u32 num_matrices, u32 result_hash, u32 len_v,
+/* MVX1 container: "MVX1", u32 version (1, or 2 when a matrix has outliers: each record
+ * then ends with u32 n_outliers and, when nonzero, out_start[rows + 1], out_rec[n]),
+ * u32 num_matrices, u32 result_hash, u32 len_v,
  * i8 vector[len_v] (zero pad to 4 bytes), then per matrix the fields of matvans_v1_host
  * as 4-byte words, cursors, the table (zero pad to 4), the payload (zero
  * pad to 4); the record header is rows, cols, grid_spacing, streams, precision,
@@ -314,6 +331,12 @@ typedef struct {
     const uint32_t *lut;        /* table_mode 0: [2 * 2^P] from matvans_v1_lut */
     const uint8_t *freq;        /* table_mode 1: [rows][G] */
     const uint16_t *payload;    /* 16-byte aligned */
+    /* outlier split (zeros: none): the host lists by row, and by column for the backward */
+    uint32_t n_outliers;
+    const uint32_t *out_start;      /* [rows + 1] */
+    const uint32_t *out_rec;        /* col | (uint8)value << 24, by row */
+    const uint32_t *out_col_start;  /* [cols + 1] */
+    const uint32_t *out_col_rec;    /* row | (uint8)value << 24, by column, rows ascending */
 } matvans_v1_dev_matrix;
 
 /* Upload (matvansWarpPerRow.cu:287-313, the per-matrix cudaMalloc + cudaMemcpy of
@@ -473,6 +496,9 @@ typedef struct {
     float scale;
     int32_t *acc32;              /* family B: int32 sums, or NULL */
     int8_t *out8;
+    /* quad: outlier split of the matrix (matvans_v1_dev_matrix out_start / out_rec), or
+       NULL */
+    const uint32_t *out_start, *out_rec;
 } matvans_batch_item;
 
 typedef struct {
@@ -508,6 +534,9 @@ typedef struct {
     const uint8_t *cdf, *ppf;    /* v0 */
     uint32_t streams;            /* quad: 32 or 64 */
     const uint32_t *lut;         /* quad: the 2 * 256 dwords of matvans_v1_lut */
+    /* quad: outliers by column (matvans_v1_dev_matrix out_col_start / out_col_rec), or
+       NULL */
+    const uint32_t *out_col_start, *out_col_rec;
 } matvans_batch_bwd_item;
 
 int matvans_batch_create_backward(const matvans_batch_bwd_item *items, uint32_t n,

@@ -97,6 +97,8 @@ class Batch:
                     it.cdf, it.ppf = _p(m.d_cdf_data), _p(m.d_ppf_data)
                 elif isinstance(m, CompressedMatrixV1) and m.coder == 1 and m.table_mode == 0:
                     it.format, it.streams, it.lut = QUAD, m.streams, _p(m.d_lut)
+                    if m.d_out is not None:
+                        it.out_start, it.out_rec = _p(m.d_out[0]), _p(m.d_out[1])
                 else:
                     raise ValueError("batch entries: .bin v0, MVX1 quad (one table per "
                                      "matrix) or family-B matrices")
@@ -193,6 +195,8 @@ class BatchBackward(Batch):
                 it.cdf, it.ppf = _p(m.d_cdf_data), _p(m.d_ppf_data)
             elif isinstance(m, CompressedMatrixV1) and m.coder == 1 and m.table_mode == 0:
                 it.format, it.streams, it.lut = QUAD, m.streams, _p(m.d_lut)
+                if m.d_out is not None:
+                    it.out_col_start, it.out_col_rec = _p(m.d_out[2]), _p(m.d_out[3])
             else:
                 raise ValueError("backward batch entries: .bin v0 or MVX1 quad matrices "
                                  "(one table per matrix)")

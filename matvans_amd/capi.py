@@ -43,14 +43,16 @@ class V1Host(ctypes.Structure):
     _fields_ = [("rows", u32), ("cols", u32), ("grid_spacing", f32), ("streams", u32),
                 ("precision", u32), ("table_mode", u32), ("coder", u32), ("min_value", i32),
                 ("G", u32), ("payload_size", u32), ("cursors", P), ("cdf", P), ("freq", P),
-                ("payload", P)]
+                ("payload", P), ("n_outliers", u32), ("out_start", P), ("out_rec", P)]
 
 
 class V1DevMatrix(ctypes.Structure):
     """matvans_v1_dev_matrix."""
     _fields_ = [("rows", u32), ("cols", u32), ("streams", u32), ("precision", u32),
                 ("table_mode", u32), ("coder", u32), ("min_value", i32), ("G", u32),
-                ("payload_size", u32), ("cursors", P), ("lut", P), ("freq", P), ("payload", P)]
+                ("payload_size", u32), ("cursors", P), ("lut", P), ("freq", P), ("payload", P),
+                ("n_outliers", u32), ("out_start", P), ("out_rec", P), ("out_col_start", P),
+                ("out_col_rec", P)]
 
 
 class BatchItem(ctypes.Structure):
@@ -60,7 +62,8 @@ class BatchItem(ctypes.Structure):
                 ("payload_size", u32), ("min_value", i32), ("G", u32), ("cdf", P), ("ppf", P),
                 ("streams", u32), ("lut", P), ("compressed", P), ("compressed_words", u64),
                 ("warp_offsets", P), ("end_offset", u32), ("b_ppf", P), ("b_cum", P),
-                ("b_prob", P), ("scale", f32), ("acc32", P), ("out8", P)]
+                ("b_prob", P), ("scale", f32), ("acc32", P), ("out8", P), ("out_start", P),
+                ("out_rec", P)]
 
 
 class BatchBwdItem(ctypes.Structure):
@@ -68,7 +71,8 @@ class BatchBwdItem(ctypes.Structure):
     _fields_ = [("format", u32), ("rows", u32), ("cols", u32), ("n_vec", u32),
                 ("g_stride", u32), ("g", P), ("s_w", P), ("alpha", P), ("dx", P),
                 ("cursors", P), ("payload", P), ("payload_size", u32), ("min_value", i32),
-                ("G", u32), ("cdf", P), ("ppf", P), ("streams", u32), ("lut", P)]
+                ("G", u32), ("cdf", P), ("ppf", P), ("streams", u32), ("lut", P),
+                ("out_col_start", P), ("out_col_rec", P)]
 
 
 class BatchConfig(ctypes.Structure):
@@ -119,6 +123,8 @@ SIGNATURES = {
     "matvans_v1_encode": (i32, [P, u32, u32, u32, u32, u32, u32, i32, u32, P, P, u32, P, P,
                                 u64, ctypes.POINTER(u64)]),
     "matvans_v1_lut": (i32, [P, u32, i32, u32, P]),
+    "matvans_v1_outlier_band": (i32, [P, u64, ctypes.POINTER(i32)]),
+    "matvans_v1_split_outliers": (i32, [P, u32, u32, i32, P, P, P, u64, ctypes.POINTER(u64)]),
     "matvans_v1_read": (i32, [ctypes.c_char_p, ctypes.POINTER(P)]),
     "matvans_v1_free": (None, [P]),
     "matvans_v1_header": (i32, [P, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(P),

@@ -301,6 +301,9 @@ int check_item(const matvans_batch_item &it, uint32_t k) {
             if (it.cols % (16 * it.streams)) return fail(MATVANS_E_SHAPE, "item %u: cols", k);
             if (!it.cursors || !it.lut || !it.payload)
                 return fail(MATVANS_E_INVALID, "item %u: null quad arrays", k);
+            if (!it.out_start != !it.out_rec)
+                return fail(MATVANS_E_INVALID, "item %u: outlier offsets and records go together",
+                            k);
             break;
         case MATVANS_BATCH_B_ROWS:
         case MATVANS_BATCH_B_STRIPES:
@@ -415,7 +418,9 @@ extern "C" int matvans_batch_create(const matvans_batch_item *items, uint32_t n,
             }
             m.v1 = mv::blk::V1Job{it.dst, it.vecs, stride, it.n_vec, it.rows, it.cols, it.cursors,
                                   it.min_value, it.G, lut, nullptr, it.payload, it.payload_size,
-                                  it.out8, it.scale};
+                                  it.out8, it.scale,
+                                  it.format == MATVANS_BATCH_QUAD ? it.out_start : nullptr,
+                                  it.format == MATVANS_BATCH_QUAD ? it.out_rec : nullptr};
         } else {
             const bool rows = it.format == MATVANS_BATCH_B_ROWS;
             m.path = rows ? mv::kPathBRows : mv::kPathBStripes;
@@ -512,7 +517,8 @@ extern "C" int matvans_batch_create_backward(const matvans_batch_bwd_item *items
         b.job = mv::blk::BwdJob{p->d_partial + off, it.dx, it.g, it.s_w, it.alpha,
                                 it.n_vec > 1 ? it.g_stride : it.rows, it.n_vec, it.rows, it.cols,
                                 it.cursors, it.payload, it.payload_size, it.min_value, it.G,
-                                it.cdf, it.ppf, it.lut};
+                                it.cdf, it.ppf, it.lut, nullptr,
+                                v0 ? nullptr : it.out_col_start, v0 ? nullptr : it.out_col_rec};
         off += (size_t(b.nrb) * it.cols * it.n_vec + 63) / 64 * 64;
         for (uint32_t v = 0; v < it.n_vec; v++)
             for (uint32_t cb = 0; cb < (it.cols + 63) / 64; cb++) rt.push_back({k, v, cb, 0});

@@ -59,6 +59,10 @@ struct BwdJob {
     const uint8_t *cdf, *ppf;
     const uint32_t *glut;        // quad: A / B decode tables (2 x 256 dwords)
     const uint8_t *freq;         // quad, one table per row: 8-bit frequencies [rows][G]
+    // quad outlier split (null: none): entries coded as 0 in the streams, by column as
+    // row | (uint8)value << 24 at out_col_rec[out_col_start[c] ..], rows ascending
+    const uint32_t *out_col_start;
+    const uint32_t *out_col_rec;
 };
 
 // From this many gradient vectors on, a group's sums are computed on the matrix cores
@@ -412,8 +416,9 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
 
 // dx[v][c] = alpha[v] * sum over row blocks of partial[v][b][c] for the 64 columns of
 // column block cb, with 1024 threads (16 per column): thread part p sums the row blocks
-// [p*chunk, (p+1)*chunk) in ascending order and the 16 parts are added in ascending
-// order, so the result does not depend on scheduling.
+// [p*chunk, (p+1)*chunk) in ascending order, then the column's outliers p, p + 16, ...
+// (value * s_w[r] * g[v][r]), and the 16 parts are added in ascending order, so the
+// result does not depend on scheduling.
 __device__ __forceinline__ void reduce_tile(const BwdJob &J, uint32_t nrb, uint32_t v,
                                             uint32_t cb, float (&part)[kRedParts][64]) {
     const uint32_t cl = threadIdx.x & 63, p = threadIdx.x >> 6;
@@ -433,6 +438,15 @@ __device__ __forceinline__ void reduce_tile(const BwdJob &J, uint32_t nrb, uint3
 #pragma unroll
             for (uint32_t k = 0; k < 8; k++)
                 if (b + k < b1) s += x[k];
+        }
+        if (J.out_col_start) {
+            const uint32_t o1 = J.out_col_start[c + 1];
+            for (uint32_t o = J.out_col_start[c] + p; o < o1; o += kRedParts) {
+                const uint32_t r = J.out_col_rec[o];
+                const uint32_t row = r & 0xFFFFFFu;
+                const float val = float(int8_t(r >> 24));
+                s = fmaf(val, J.s_w[row] * J.g[size_t(v) * J.g_stride + row], s);
+            }
         }
     }
     part[p][cl] = s;

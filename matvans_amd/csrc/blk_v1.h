@@ -57,6 +57,64 @@ struct V1Job {
     uint32_t payload_size;
     int8_t *out8;            // [n_vec][rows] clamp(rint(float(sum) * scale), -128, 127)
     float scale;
+    // outlier split (quad coder; null: none): entries coded as 0 in the streams, listed
+    // by row as col | (uint8)value << 24 at out_rec[out_start[row] ..]
+    const uint32_t *out_start;
+    const uint32_t *out_rec;
+};
+
+// Outlier records of one row, S lanes per row: lane sl takes records sl, sl + S, ...
+// kSlots of them are loaded ahead of the decode (their offsets with the row cursors, the
+// records with the first ring sets, the vector bytes just before the loop, so that no
+// wait of the loop covers them); the rest, rare with the band the encoder picks, are read
+// after the loop.  The corrections value * x[col] enter the lane's accumulators before the
+// row reduction.
+template <int NV, int S>
+struct RowOutliers {
+    static constexpr int kSlots = 2;
+    uint32_t beg, end;
+    uint32_t rec[kSlots];
+    int32_t x[kSlots][NV];
+    __device__ __forceinline__ void offsets(const V1Job &J, uint32_t row, bool valid) {
+        beg = end = 0;
+        if (J.out_start && valid) {
+            beg = J.out_start[row];
+            end = J.out_start[row + 1];
+        }
+    }
+    __device__ __forceinline__ void records(const V1Job &J, uint32_t sl) {
+#pragma unroll
+        for (int k = 0; k < kSlots; k++) {
+            const uint32_t o = beg + sl + uint32_t(k) * S;
+            rec[k] = o < end ? J.out_rec[o] : 0u;   // 0: column 0, value 0 (adds nothing)
+        }
+    }
+    __device__ __forceinline__ void values(const V1Job &J, uint32_t nv) {
+#pragma unroll
+        for (int k = 0; k < kSlots; k++)
+#pragma unroll
+            for (int v = 0; v < NV; v++)
+                x[k][v] = J.out_start ? int32_t(J.vecs[size_t(v < int(nv) ? v : 0) * J.vec_stride +
+                                                      (rec[k] & 0xFFFFFFu)])
+                                      : 0;
+    }
+    __device__ __forceinline__ void apply(const V1Job &J, uint32_t sl, uint32_t nv,
+                                          int32_t (&acc)[NV]) const {
+#pragma unroll
+        for (int k = 0; k < kSlots; k++) {
+            const int32_t val = int32_t(int8_t(rec[k] >> 24));
+#pragma unroll
+            for (int v = 0; v < NV; v++) acc[v] += val * x[k][v];
+        }
+        for (uint32_t o = beg + sl + uint32_t(kSlots) * S; o < end; o += S) {
+            const uint32_t r = J.out_rec[o];
+            const int32_t val = int32_t(int8_t(r >> 24));
+#pragma unroll
+            for (int v = 0; v < NV; v++)
+                acc[v] += val * int32_t(J.vecs[size_t(v < int(nv) ? v : 0) * J.vec_stride +
+                                               (r & 0xFFFFFFu)]);
+        }
+    }
 };
 
 // Output of row `row`, vector v: the int32 sum and / or the next layer's int8 input with
@@ -372,9 +430,12 @@ __device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid,
     wend = __builtin_amdgcn_readfirstlane(wend);
     const uint32_t rcur = valid ? cursors[row] : 0u;
     const uint32_t rend = valid ? (row + 1 < rows ? cursors[row + 1] : payload_size) : 0u;
+    RowOutliers<NV, S> outl;
+    outl.offsets(J, row, valid);
 
     QL st;
     st.init(payload, wbeg, wend, valid, rcur, rend, sl, ring, dummy, wave, lane);
+    outl.records(J, sl);
     st.prologue();
 
     const uint32_t groups = cols / (16u * S);
@@ -402,6 +463,7 @@ __device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid,
     __syncthreads();
     st.start();
     vload(1, vc1);
+    outl.values(J, nv_eff);
 
     int32_t acc[NV];
 #pragma unroll
@@ -456,6 +518,7 @@ __device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid,
         }
     }
 
+    outl.apply(J, sl, nv_eff, acc);
 #pragma unroll
     for (int v = 0; v < NV; v++) {
         int32_t a = acc[v];

@@ -148,9 +148,14 @@ extern "C" int matvans_v1_decompress_and_multiply(const matvans_v1_dev_matrix *m
     if (!aligned(vecs, 16) || vec_stride % 16)
         return fail(MATVANS_E_ALIGN, "vectors must be 16-byte aligned (stride %u)", vec_stride);
     if (vec_stride < m->cols) return fail(MATVANS_E_INVALID, "vec_stride %u < cols %u", vec_stride, m->cols);
+    if (m->n_outliers && (m->coder != 1 || m->table_mode != 0))
+        return fail(MATVANS_E_UNSUPPORTED, "outliers need the quad coder with one table");
+    if (m->n_outliers && (!m->out_start || !m->out_rec))
+        return fail(MATVANS_E_INVALID, "outliers without device offsets or records");
     mv::V1Args a{dst, vecs, n_vec, vec_stride, m->rows, m->cols, m->streams, m->precision,
                  m->table_mode, m->coder, m->min_value, m->G, m->cursors, m->lut, m->freq, m->payload,
-                 m->payload_size};
+                 m->payload_size, m->n_outliers ? m->out_start : nullptr,
+                 m->n_outliers ? m->out_rec : nullptr};
     return hip_status(mv::launch_v1_decode_matvec(a, as_stream(stream)), "v1 decode_matvec launch");
 }
 
@@ -314,6 +319,9 @@ int check_v1_backward(const matvans_v1_dev_matrix *m, const float *dx, const flo
         return fail(MATVANS_E_INVALID, "g_stride %u < rows", g_stride);
     if (!aligned(m->payload, 16) || !aligned(dx, 4))
         return fail(MATVANS_E_ALIGN, "payload must be 16-byte aligned");
+    if (m->n_outliers && (m->table_mode != 0 || !m->out_col_start || !m->out_col_rec))
+        return fail(MATVANS_E_INVALID, "outliers need one table per matrix and their "
+                    "by-column device lists");
     return MATVANS_OK;
 }
 
@@ -323,7 +331,8 @@ mv::QuadBwdArgs v1_backward_args(const matvans_v1_dev_matrix *m, float *dx, cons
     return mv::QuadBwdArgs{dx, partial, g, n_vec, n_vec > 1 ? g_stride : m->rows, s_w, alpha,
                            m->rows, m->cols, m->streams, m->cursors, m->lut, m->payload,
                            m->payload_size, m->table_mode, m->freq, int32_t(m->min_value),
-                           uint32_t(m->G)};
+                           uint32_t(m->G), m->n_outliers ? m->out_col_start : nullptr,
+                           m->n_outliers ? m->out_col_rec : nullptr};
 }
 }  // namespace
 

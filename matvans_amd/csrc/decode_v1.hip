@@ -50,8 +50,9 @@ __global__ __launch_bounds__(256) void v1q_fwd(V1Job job) {
 }
 
 V1Job job_of(const V1Args &a) {
-    return V1Job{a.dst, a.vecs, a.vec_stride, a.n_vec, a.rows, a.cols, a.cursors, a.min_value,
-                 a.G, a.lut, a.freq, a.payload, a.payload_size};
+    return V1Job{a.dst,       a.vecs,    a.vec_stride, a.n_vec,        a.rows,    a.cols,
+                 a.cursors,   a.min_value, a.G,        a.lut,          a.freq,    a.payload,
+                 a.payload_size, nullptr, 0.0f,        a.out_start,    a.out_rec};
 }
 
 template <int NV, int S, int MODE = 0>

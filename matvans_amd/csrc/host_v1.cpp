@@ -184,6 +184,78 @@ extern "C" int matvans_v1_model(const int8_t *mat, uint32_t rows, uint32_t cols,
     return MATVANS_OK;
 }
 
+// ------------------------------------------------------------------ outlier split
+// An 8-bit model gives every present value at least 1/256 of the code space, so a long
+// tail of rare values costs every entry extra bits.  The split keeps the values within
+// [-band, band] in the streams and codes every entry beyond as 0 (the most probable
+// symbol), listing it as an outlier record col | (uint8)value << 24 per row; the decode
+// adds value * x[col] for each record.
+extern "C" int matvans_v1_outlier_band(const int8_t *mat, uint64_t n, int32_t *band) {
+    if (!mat || !band || !n) return fail(MATVANS_E_INVALID, "null argument");
+    uint64_t h[256] = {};
+    for (uint64_t i = 0; i < n; i++) h[uint8_t(mat[i])]++;
+    auto count = [&](int v) { return h[uint8_t(int8_t(v))]; };
+    double best = -1.0;
+    int best_t = 128;
+    std::vector<uint64_t> c;
+    std::vector<uint32_t> f;
+    for (int t = 1; t <= 128; t++) {
+        // band [-t, t] (t = 128: every value, no outliers)
+        const int lo = t >= 128 ? -128 : -t, hi = t >= 128 ? 127 : t;
+        c.assign(size_t(hi - lo + 1), 0);
+        uint64_t outside = 0;
+        for (int v = -128; v <= 127; v++) {
+            if (v < lo || v > hi) outside += count(v);
+            else c[size_t(v - lo)] += count(v);
+        }
+        c[size_t(-lo)] += outside;   // outliers are coded as 0
+        uint32_t present = 0;
+        for (auto x : c) present += x != 0;
+        if (present < 2) continue;   // a one-symbol alphabet: the band below this one
+        f.assign(c.size(), 0);
+        if (quantize_counts(c.data(), uint32_t(c.size()), 256, f.data())) continue;
+        double bits = 32.0 * double(outside);
+        for (size_t s = 0; s < c.size(); s++)
+            if (c[s]) bits -= double(c[s]) * std::log2(double(f[s]) / 256.0);
+        if (best < 0 || bits < best) { best = bits; best_t = outside ? t : 128; }
+    }
+    *band = best_t >= 128 ? 127 : best_t;   // 127: the best band leaves no outliers
+    mv::clear_error();
+    return MATVANS_OK;
+}
+
+extern "C" int matvans_v1_split_outliers(const int8_t *mat, uint32_t rows, uint32_t cols,
+                                         int32_t band, int8_t *coded, uint32_t *out_start,
+                                         uint32_t *out_rec, uint64_t cap, uint64_t *n_out) {
+    if (!mat || !coded || !out_start || !n_out) return fail(MATVANS_E_INVALID, "null argument");
+    if (band < 0 || band > 127) return fail(MATVANS_E_INVALID, "band %d outside 0..127", band);
+    if (cols >= (1u << 24)) return fail(MATVANS_E_SHAPE, "cols %u >= 2^24", cols);
+    uint64_t n = 0;
+    for (uint32_t r = 0; r < rows; r++) {
+        if (n > 0xFFFFFFFFull) return fail(MATVANS_E_CAPACITY, "more than 2^32 outliers");
+        out_start[r] = uint32_t(n);
+        const int8_t *src = mat + size_t(r) * cols;
+        int8_t *dst = coded + size_t(r) * cols;
+        for (uint32_t c = 0; c < cols; c++) {
+            const int v = src[c];
+            if (v < -band || v > band) {
+                if (n >= cap || !out_rec)
+                    return fail(MATVANS_E_CAPACITY, "more than %llu outliers",
+                                (unsigned long long)cap);
+                out_rec[n++] = c | (uint32_t(uint8_t(int8_t(v))) << 24);
+                dst[c] = 0;
+            } else {
+                dst[c] = int8_t(v);
+            }
+        }
+    }
+    if (n > 0xFFFFFFFFull) return fail(MATVANS_E_CAPACITY, "more than 2^32 outliers");
+    out_start[rows] = uint32_t(n);
+    *n_out = n;
+    mv::clear_error();
+    return MATVANS_OK;
+}
+
 extern "C" uint64_t matvans_v1_encode_bound(uint32_t rows, uint32_t cols, uint32_t streams,
                                             uint32_t precision) {
     if (!valid_params(streams, precision, 0) || !cols) return 0;
@@ -367,7 +439,7 @@ extern "C" int matvans_v1_read(const char *path, matvans_v1_file **out) {
         return fail(MATVANS_E_FORMAT, "matrix %u: %s", k, what);
     };
     if (!rd.take(magic, 4) || std::memcmp(magic, "MVX1", 4) != 0) return bad("not an MVX1 file", 0);
-    if (!rd.take(&version, 4) || version != 1) return bad("unsupported version", 0);
+    if (!rd.take(&version, 4) || (version != 1 && version != 2)) return bad("unsupported version", 0);
     if (!rd.take(&v1->num_matrices, 4) || !rd.take(&v1->result_hash, 4) || !rd.take(&len_v, 4))
         return bad("truncated header", 0);
     v1->vector.resize(len_v);
@@ -392,14 +464,42 @@ extern "C" int matvans_v1_read(const char *path, matvans_v1_file **out) {
         const uint8_t *pay_p = rd.p;
         size_t pay_n = size_t(m.payload_size) * 2;
         if (!rd.take(nullptr, pay_n) || !rd.take(nullptr, pad4(pay_n))) return bad("truncated payload", k);
-        // own a 16-byte aligned copy: [payload | cursors | tables]
+        // version 2: outlier count, then (when nonzero) out_start[rows + 1], out_rec[n]
+        const uint8_t *ost_p = nullptr, *orec_p = nullptr;
+        if (version == 2) {
+            if (!rd.take(&m.n_outliers, 4)) return bad("truncated outlier count", k);
+            if (m.n_outliers) {
+                if (m.coder != 1 || m.table_mode != 0) return bad("outliers need the quad coder", k);
+                ost_p = rd.p;
+                if (!rd.take(nullptr, (size_t(m.rows) + 1) * 4)) return bad("truncated outliers", k);
+                orec_p = rd.p;
+                if (size_t(rd.end - rd.p) / 4 < m.n_outliers) return bad("truncated outliers", k);
+                rd.take(nullptr, size_t(m.n_outliers) * 4);
+            }
+        }
+        const size_t ob = m.n_outliers ? ((size_t(m.rows) + 1) * 4 + 15) / 16 * 16 : 0;
+        const size_t orb = size_t(m.n_outliers) * 4;
+        // own a 16-byte aligned copy: [payload | cursors | tables | out_start | out_rec]
         size_t pb = (pay_n + 15) / 16 * 16, cb = (size_t(m.rows) * 4 + 15) / 16 * 16;
-        std::vector<uint8_t> blob(pb + cb + tab_n + 32);
+        size_t tb = (tab_n + 15) / 16 * 16;
+        std::vector<uint8_t> blob(pb + cb + tb + ob + orb + 32);
         uint8_t *a = blob.data();
         a += (16 - reinterpret_cast<uintptr_t>(a) % 16) % 16;
         std::memcpy(a, pay_p, pay_n);
         std::memcpy(a + pb, cur_p, size_t(m.rows) * 4);
         std::memcpy(a + pb + cb, tab_p, tab_n);
+        if (m.n_outliers) {
+            std::memcpy(a + pb + cb + tb, ost_p, (size_t(m.rows) + 1) * 4);
+            std::memcpy(a + pb + cb + tb + ob, orec_p, orb);
+            m.out_start = reinterpret_cast<const uint32_t *>(a + pb + cb + tb);
+            m.out_rec = reinterpret_cast<const uint32_t *>(a + pb + cb + tb + ob);
+            if (m.out_start[0] != 0 || m.out_start[m.rows] != m.n_outliers)
+                return bad("bad outlier offsets", k);
+            for (uint32_t r = 0; r < m.rows; r++)
+                if (m.out_start[r + 1] < m.out_start[r]) return bad("outlier offsets decrease", k);
+            for (uint32_t i = 0; i < m.n_outliers; i++)
+                if ((m.out_rec[i] & 0xFFFFFFu) >= m.cols) return bad("outlier column", k);
+        }
         m.payload = reinterpret_cast<const uint16_t *>(a);
         m.cursors = reinterpret_cast<const uint32_t *>(a + pb);
         if (m.table_mode == 0) {
@@ -460,12 +560,17 @@ extern "C" int matvans_v1_write(const char *path, uint32_t result_hash, const in
         if ((m.table_mode == 0 && !m.cdf) || (m.table_mode == 1 && !m.freq) || !m.payload ||
             (m.rows && !m.cursors))
             return fail(MATVANS_E_INVALID, "matrix %u: null table, cursors or payload", k);
+        if (m.n_outliers && (!m.out_start || !m.out_rec || m.coder != 1 || m.table_mode != 0))
+            return fail(MATVANS_E_INVALID, "matrix %u: outliers need offsets, records and the "
+                        "quad coder with one table", k);
     }
+    bool any_outliers = false;
+    for (uint32_t k = 0; k < num_matrices; k++) any_outliers |= mats[k].n_outliers != 0;
     FILE *f = std::fopen(path, "wb");
     if (!f) return fail(MATVANS_E_IO, "cannot create %s", path);
     static const uint8_t zeros[4] = {0, 0, 0, 0};
     auto put = [&](const void *p, size_t n) { return n == 0 || std::fwrite(p, 1, n, f) == n; };
-    const uint32_t version = 1;
+    const uint32_t version = any_outliers ? 2 : 1;
     bool ok = put("MVX1", 4) && put(&version, 4) && put(&num_matrices, 4) &&
               put(&result_hash, 4) && put(&len_v, 4) && put(vector, len_v) &&
               put(zeros, pad4(len_v));
@@ -479,6 +584,12 @@ extern "C" int matvans_v1_write(const char *path, uint32_t result_hash, const in
              put(m.table_mode == 0 ? static_cast<const void *>(m.cdf) : m.freq, tab_n) &&
              put(zeros, pad4(tab_n)) && put(m.payload, size_t(m.payload_size) * 2) &&
              put(zeros, pad4(size_t(m.payload_size) * 2));
+        if (ok && version == 2) {
+            ok = put(&m.n_outliers, 4);
+            if (ok && m.n_outliers)
+                ok = put(m.out_start, (size_t(m.rows) + 1) * 4) &&
+                     put(m.out_rec, size_t(m.n_outliers) * 4);
+        }
     }
     ok = (std::fclose(f) == 0) && ok;
     if (!ok) return fail(MATVANS_E_IO, "write failed on %s", path);

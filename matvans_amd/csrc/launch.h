@@ -34,6 +34,8 @@ struct V1Args {
     const uint8_t *freq;     // table_mode 1: [rows][G]
     const uint16_t *payload;
     uint32_t payload_size;
+    const uint32_t *out_start;   // outlier split (quad coder), null: none
+    const uint32_t *out_rec;
 };
 
 hipError_t launch_v1_decode_matvec(const V1Args &a, hipStream_t s);
@@ -89,6 +91,8 @@ struct QuadBwdArgs {
     const uint8_t *freq;     // [rows][G] (table mode 1)
     int32_t min_value;
     uint32_t G;
+    const uint32_t *out_col_start;   // outlier split by column (table mode 0), null: none
+    const uint32_t *out_col_rec;
 };
 
 hipError_t launch_quad_backward(const QuadBwdArgs &a, hipStream_t s);

@@ -97,6 +97,34 @@ extern "C" int matvans_v1_matrix_upload(const matvans_v1_host *m, matvans_upload
         st = m->table_mode == 0 ? put(u, lut.data(), lut.size() * 4, 0, &tab)
                                 : put(u, m->freq, size_t(m->rows) * m->G, 0, &tab);
     if (!st) st = put(u, m->payload, size_t(m->payload_size) * 2, kSlackBytes, &pay);
+    // outliers: the records by row as stored, and by column for the backward
+    void *ost = nullptr, *orec = nullptr, *ocst = nullptr, *ocrec = nullptr;
+    if (!st && m->n_outliers) {
+        if (!m->out_start || !m->out_rec) {
+            matvans_upload_free(u);
+            return fail(MATVANS_E_INVALID, "outliers without offsets or records");
+        }
+        std::vector<uint32_t> col_start(size_t(m->cols) + 1, 0), col_rec(m->n_outliers);
+        for (uint32_t i = 0; i < m->n_outliers; i++) {
+            const uint32_t c = m->out_rec[i] & 0xFFFFFFu;
+            if (c >= m->cols) {
+                matvans_upload_free(u);
+                return fail(MATVANS_E_INVALID, "outlier %u: column %u >= %u", i, c, m->cols);
+            }
+            col_start[c + 1]++;
+        }
+        for (uint32_t c = 0; c < m->cols; c++) col_start[c + 1] += col_start[c];
+        std::vector<uint32_t> fill(col_start.begin(), col_start.end() - 1);
+        for (uint32_t r = 0; r < m->rows; r++)   // rows ascending within each column
+            for (uint32_t i = m->out_start[r]; i < m->out_start[r + 1]; i++) {
+                const uint32_t c = m->out_rec[i] & 0xFFFFFFu;
+                col_rec[fill[c]++] = r | (m->out_rec[i] & 0xFF000000u);
+            }
+        st = put(u, m->out_start, (size_t(m->rows) + 1) * 4, 0, &ost);
+        if (!st) st = put(u, m->out_rec, size_t(m->n_outliers) * 4, 0, &orec);
+        if (!st) st = put(u, col_start.data(), col_start.size() * 4, 0, &ocst);
+        if (!st) st = put(u, col_rec.data(), col_rec.size() * 4, 0, &ocrec);
+    }
     if (st) {
         matvans_upload_free(u);
         return st;
@@ -105,7 +133,9 @@ extern "C" int matvans_v1_matrix_upload(const matvans_v1_host *m, matvans_upload
              m->G, m->payload_size, static_cast<const uint32_t *>(cur),
              m->table_mode == 0 ? static_cast<const uint32_t *>(tab) : nullptr,
              m->table_mode == 1 ? static_cast<const uint8_t *>(tab) : nullptr,
-             static_cast<const uint16_t *>(pay)};
+             static_cast<const uint16_t *>(pay), m->n_outliers,
+             static_cast<const uint32_t *>(ost), static_cast<const uint32_t *>(orec),
+             static_cast<const uint32_t *>(ocst), static_cast<const uint32_t *>(ocrec)};
     *out = u;
     mv::clear_error();
     return MATVANS_OK;

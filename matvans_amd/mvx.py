@@ -39,7 +39,8 @@ class CompressedMatrixV1:
     or freq uint8[rows, G] (table_mode 1), payload uint16[payload_size]."""
 
     def __init__(self, rows, cols, streams, precision, table_mode, min_value, G, cursors,
-                 payload, cdf=None, freq=None, grid_spacing=1.0, coder=0):
+                 payload, cdf=None, freq=None, grid_spacing=1.0, coder=0, out_start=None,
+                 out_rec=None):
         self.coder = int(coder)
         self.rows, self.cols = int(rows), int(cols)
         self.streams, self.precision, self.table_mode = int(streams), int(precision), int(table_mode)
@@ -51,18 +52,59 @@ class CompressedMatrixV1:
         self.freq = None if freq is None else np.ascontiguousarray(freq, np.uint8).reshape(
             self.rows, self.G)
         self.grid_spacing = float(grid_spacing)
+        # outlier split: entries coded as 0, listed by row (col | (uint8)value << 24)
+        self.out_start = None if out_start is None else np.ascontiguousarray(out_start, np.uint32)
+        self.out_rec = None if out_rec is None else np.ascontiguousarray(out_rec, np.uint32)
+        self.n_outliers = 0 if self.out_rec is None else int(self.out_rec.size)
         self.d_cursors = self.d_lut = self.d_freq = self.d_payload = None
+        self.d_out = None
 
     @property
     def stored_bytes(self):
-        """Payload + cursors + tables, as stored."""
+        """Payload + cursors + tables (+ outlier offsets and records), as stored."""
         tab = 2 * (self.G + 1) if self.table_mode == 0 else self.rows * self.G
-        return 2 * self.payload_size + 4 * self.rows + tab
+        out = 4 * (self.rows + 1) + 4 * self.n_outliers if self.n_outliers else 0
+        return 2 * self.payload_size + 4 * self.rows + tab + out
 
-    @classmethod
-    def encode(cls, matrix, streams=64, precision=8, table_mode=0, n_threads=0, coder=0):
+    @staticmethod
+    def outlier_band(matrix):
+        """The band T of the outlier split the library picks for this matrix (127: none)."""
+        m = np.ascontiguousarray(np.asarray(matrix), dtype=np.int8)
+        t = ctypes.c_int32()
+        call("matvans_v1_outlier_band", _np_ptr(m), m.size, ctypes.byref(t))
+        return t.value
+
+    @staticmethod
+    def split_outliers(matrix, band):
+        """(coded matrix with |w| > band set to 0, out_start uint32[rows + 1],
+        out_rec uint32[n] = col | (uint8)w << 24 in row-major order)."""
         m = np.ascontiguousarray(np.asarray(matrix), dtype=np.int8)
         rows, cols = m.shape
+        cap = int(np.count_nonzero(np.abs(m.astype(np.int16)) > band))
+        coded = np.empty_like(m)
+        start = np.empty(rows + 1, np.uint32)
+        rec = np.empty(max(1, cap), np.uint32)
+        n = ctypes.c_uint64()
+        call("matvans_v1_split_outliers", _np_ptr(m), rows, cols, int(band), _np_ptr(coded),
+             _np_ptr(start), _np_ptr(rec), cap, ctypes.byref(n))
+        return coded, start, rec[:n.value].copy()
+
+    @classmethod
+    def encode(cls, matrix, streams=64, precision=8, table_mode=0, n_threads=0, coder=0,
+               outliers=None):
+        """outliers (quad coder, one table): None or False, no split; "auto", the band the
+        library picks; an int, that band."""
+        m = np.ascontiguousarray(np.asarray(matrix), dtype=np.int8)
+        rows, cols = m.shape
+        out_start = out_rec = None
+        if outliers is not None and outliers is not False:
+            if coder != 1 or table_mode != 0 or precision != 8:
+                raise ValueError("the outlier split needs the quad coder with one 8-bit table")
+            band = cls.outlier_band(m) if outliers == "auto" else int(outliers)
+            if band < 127:
+                m, out_start, out_rec = cls.split_outliers(m, band)
+                if out_rec.size == 0:
+                    out_start = out_rec = None
         mn, G = ctypes.c_int32(), ctypes.c_uint32()
         cdf = np.zeros(257, np.uint16)
         freq = np.zeros(rows * 256 if table_mode == 1 else 1, np.uint8)
@@ -79,7 +121,19 @@ class CompressedMatrixV1:
         return cls(rows, cols, streams, precision, table_mode, mn.value, g, cur,
                    pay[:size.value].copy(),
                    cdf=cdf[:g + 1].copy() if table_mode == 0 else None,
-                   freq=freq[:rows * g].copy() if table_mode == 1 else None, coder=coder)
+                   freq=freq[:rows * g].copy() if table_mode == 1 else None, coder=coder,
+                   out_start=out_start, out_rec=out_rec)
+
+    def outliers_by_column(self):
+        """(out_col_start uint32[cols + 1], out_col_rec uint32[n] = row | (uint8)w << 24,
+        rows ascending within a column): the records as the backward reads them."""
+        rows_of = np.repeat(np.arange(self.rows, dtype=np.uint32), np.diff(self.out_start))
+        cols = self.out_rec & 0xFFFFFF
+        order = np.lexsort((rows_of, cols))
+        col_start = np.zeros(self.cols + 1, np.uint32)
+        np.cumsum(np.bincount(cols, minlength=self.cols), out=col_start[1:])
+        col_rec = rows_of[order] | (self.out_rec[order] & np.uint32(0xFF000000))
+        return col_start, col_rec.astype(np.uint32)
 
     def lut(self):
         """Decode table of a one-table matrix (2 * 2^P uint32)."""
@@ -97,16 +151,21 @@ class CompressedMatrixV1:
             self.d_lut = torch.from_numpy(self.lut().view(np.int32)).to(dev)
         else:
             self.d_freq = torch.from_numpy(self.freq.reshape(-1)).to(dev)
+        if self.n_outliers:
+            cs, cr = self.outliers_by_column()
+            t = lambda a: torch.from_numpy(np.ascontiguousarray(a).view(np.int32)).to(dev)
+            self.d_out = (t(self.out_start), t(self.out_rec), t(cs), t(cr))
         return self
 
     def dev_struct(self) -> V1DevMatrix:
         if self.d_payload is None:
             raise RuntimeError("matrix is not on the device; call to_device() first")
+        o = [t.data_ptr() for t in self.d_out] if self.d_out is not None else [0, 0, 0, 0]
         return V1DevMatrix(self.rows, self.cols, self.streams, self.precision, self.table_mode,
                            self.coder, self.min_value, self.G, self.payload_size, self.d_cursors.data_ptr(),
                            self.d_lut.data_ptr() if self.d_lut is not None else 0,
                            self.d_freq.data_ptr() if self.d_freq is not None else 0,
-                           self.d_payload.data_ptr())
+                           self.d_payload.data_ptr(), self.n_outliers, *o)
 
     def multiply(self, vec, out=None, stream=None):
         """int32 W @ vec for one vector [cols] or a batch [n_vec, cols] (n_vec <= 8)."""
@@ -154,13 +213,15 @@ class CompressedMatrixV1:
         return out[0] if squeeze else out
 
     def host_struct(self, keep) -> V1Host:
-        keep += [self.cursors, self.payload, self.cdf, self.freq]
+        keep += [self.cursors, self.payload, self.cdf, self.freq, self.out_start, self.out_rec]
         return V1Host(self.rows, self.cols, self.grid_spacing, self.streams, self.precision,
                       self.table_mode, self.coder, self.min_value, self.G, self.payload_size,
                       self.cursors.ctypes.data,
                       self.cdf.ctypes.data if self.cdf is not None else 0,
                       self.freq.ctypes.data if self.freq is not None else 0,
-                      self.payload.ctypes.data)
+                      self.payload.ctypes.data, self.n_outliers,
+                      self.out_start.ctypes.data if self.n_outliers else 0,
+                      self.out_rec.ctypes.data if self.n_outliers else 0)
 
 
 def read_mvx1(path):
@@ -188,7 +249,9 @@ def read_mvx1(path):
                 arr(m.payload, ctypes.c_uint16, m.payload_size),
                 cdf=arr(m.cdf, ctypes.c_uint16, m.G + 1) if m.table_mode == 0 else None,
                 freq=arr(m.freq, ctypes.c_uint8, m.rows * m.G) if m.table_mode == 1 else None,
-                grid_spacing=m.grid_spacing, coder=m.coder))
+                grid_spacing=m.grid_spacing, coder=m.coder,
+                out_start=arr(m.out_start, ctypes.c_uint32, m.rows + 1) if m.n_outliers else None,
+                out_rec=arr(m.out_rec, ctypes.c_uint32, m.n_outliers) if m.n_outliers else None))
         return {"num_matrices": num.value, "result_hash": rh.value}, vec, mats
     finally:
         lib().matvans_v1_free(h)

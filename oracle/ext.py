@@ -43,8 +43,16 @@ payload; word j of stream l is at u16 index cursor + 256*(j // G4) + 2*(G4*l + j
 with G4 = 128 / S words per stream per 512-byte granule; words 0 and 1 are the low and
 high words of the initial state.
 
+Outlier split (quad coder, one table): entries with |w| > band are coded as 0 in the
+streams and listed per row as records col | (uint8)w << 24 (out_start[rows + 1] offsets,
+row-major order); W = decode(streams) + scatter(records).  Pinned by the round trip on
+Laplace and Gaussian matrices, the library's split equal to ``split_outliers`` below, and
+the forward / backward GPU results against the integer product / dense FP64 of W.
+
 MVX1 container (little-endian):
-  "MVX1", u32 version = 1, u32 num_matrices, u32 result_hash, u32 len_v,
+  "MVX1", u32 version (1; 2 when a matrix has outliers: each record then ends with
+  u32 n_outliers and, when nonzero, u32 out_start[rows + 1], u32 out_rec[n_outliers]),
+  u32 num_matrices, u32 result_hash, u32 len_v,
   i8 vector[len_v], zero pad to 4 bytes; then per matrix:
   u32 rows, u32 cols, f32 grid_spacing, u32 streams, u32 precision, u32 table_mode,
   u32 coder, i32 min_value, u32 G, u32 payload_size, u32 cursors[rows],
@@ -386,11 +394,39 @@ def decode_stream_quad_textbook(words, n_quads, cdf):
     return out
 
 
+# ---------------------------------------------------------------------- outliers
+def split_outliers(matrix, band):
+    """(coded matrix: entries with |w| > band replaced by 0, out_start uint32[rows + 1],
+    out_rec uint32[n] = col | (uint8)w << 24, rows in order, columns ascending)."""
+    m = np.asarray(matrix, np.int8)
+    out = np.abs(m.astype(np.int16)) > band
+    rows_i, cols_i = np.nonzero(out)           # row-major order
+    rec = (cols_i.astype(np.uint32) |
+           (m[rows_i, cols_i].view(np.uint8).astype(np.uint32) << 24)).astype(np.uint32)
+    start = np.zeros(m.shape[0] + 1, np.uint32)
+    np.cumsum(out.sum(axis=1), out=start[1:])
+    coded = m.copy()
+    coded[out] = 0
+    return coded, start, rec
+
+
+def merge_outliers(coded, out_start, out_rec):
+    """The matrix W from the coded matrix and its outlier records."""
+    w = np.array(coded, np.int8, copy=True)
+    for r in range(w.shape[0]):
+        for o in range(int(out_start[r]), int(out_start[r + 1])):
+            w[r, int(out_rec[o]) & 0xFFFFFF] = np.int8(np.uint8(int(out_rec[o]) >> 24))
+    return w
+
+
 # --------------------------------------------------------------------- container
 class RecordV1:
     def __init__(self, rows, cols, streams, precision, table_mode, min_value, G, cursors,
-                 payload, cdf=None, freq=None, grid_spacing=1.0, coder=0):
+                 payload, cdf=None, freq=None, grid_spacing=1.0, coder=0, out_start=None,
+                 out_rec=None):
         self.coder = coder
+        self.out_start = None if out_start is None else np.asarray(out_start, np.uint32)
+        self.out_rec = None if out_rec is None else np.asarray(out_rec, np.uint32)
         self.rows, self.cols, self.streams, self.precision = rows, cols, streams, precision
         self.table_mode, self.min_value, self.G = table_mode, min_value, G
         self.cursors = np.asarray(cursors, np.uint32)
@@ -460,7 +496,7 @@ def read_mvx1(path):
     if data[:4] != MAGIC:
         raise ValueError("not an MVX1 file")
     ver, num, rhash, lv = struct.unpack_from("<LLLL", data, 4)
-    if ver != VERSION:
+    if ver not in (1, 2):
         raise ValueError("unsupported MVX1 version")
     o = 20
     vec = np.frombuffer(data, np.int8, lv, o).copy()
@@ -480,7 +516,17 @@ def read_mvx1(path):
         o += n + (4 - n % 4) % 4
         pay = np.frombuffer(data, "<u2", psize, o).copy()
         o += 2 * psize + (4 - (2 * psize) % 4) % 4
-        recs.append(RecordV1(rows, cols, S, P, mode, mn, G, cur, pay, cdf, freq, gs, coder))
+        ost = orec = None
+        if ver == 2:
+            (n_out,) = struct.unpack_from("<L", data, o)
+            o += 4
+            if n_out:
+                ost = np.frombuffer(data, "<u4", rows + 1, o).copy()
+                o += 4 * (rows + 1)
+                orec = np.frombuffer(data, "<u4", n_out, o).copy()
+                o += 4 * n_out
+        recs.append(RecordV1(rows, cols, S, P, mode, mn, G, cur, pay, cdf, freq, gs, coder,
+                             ost, orec))
     if o != len(data):
         raise ValueError("trailing bytes")
     return {"num_matrices": num, "result_hash": rhash}, vec, recs

@@ -110,3 +110,77 @@ def test_encoder_argument_errors():
     with pytest.raises(MatvansError) as e:       # quad coder with 12-bit tables
         CompressedMatrixV1.encode(m, 64, 12, 0, coder=1)
     assert e.value.status == 8
+
+
+# ------------------------------------------------------------------ outlier split
+def test_outlier_split_matches_oracle_and_round_trips():
+    """The library's split equals the oracle's, and the quad encoding of the coded matrix
+    plus the records gives back W (oracle decode + scatter)."""
+    for m in [laplace_int8(7, 4096, 4.0, seed=21)[0], gaussian_int8(5, 2048, 6.0, seed=22)[0]]:
+        band = CompressedMatrixV1.outlier_band(m)
+        assert 0 < band < 127
+        coded, start, rec = CompressedMatrixV1.split_outliers(m, band)
+        ocoded, ostart, orec = E.split_outliers(m, band)
+        assert np.array_equal(coded, ocoded) and np.array_equal(start, ostart)
+        assert np.array_equal(rec, orec)
+        assert np.abs(coded.astype(np.int16)).max() <= band
+        c = CompressedMatrixV1.encode(m, 32, 8, 0, coder=1, outliers=band)
+        assert c.n_outliers == rec.size > 0
+        r = E.encode_record(coded, 32, 8, 0, coder=1)
+        assert np.array_equal(c.payload, r.payload) and np.array_equal(c.cdf, r.cdf)
+        assert np.array_equal(E.merge_outliers(r.decode(), c.out_start, c.out_rec), m)
+
+
+def test_outlier_split_pays_on_a_long_tail_and_not_without_one():
+    """Laplace(0, 4): the split stores fewer bytes than the plain quad encoding; a
+    distribution without a tail (uniform over 17 values) gets band 127 (no split)."""
+    m = laplace_int8(64, 4096, 4.0, seed=23)[0]
+    plain = CompressedMatrixV1.encode(m, 32, 8, 0, coder=1)
+    split = CompressedMatrixV1.encode(m, 32, 8, 0, coder=1, outliers="auto")
+    assert split.n_outliers > 0
+    assert split.stored_bytes < 0.93 * plain.stored_bytes
+    u = np.random.default_rng(3).integers(-8, 9, (16, 2048)).astype(np.int8)
+    assert CompressedMatrixV1.outlier_band(u) == 127
+    assert CompressedMatrixV1.encode(u, 32, 8, 0, coder=1, outliers="auto").n_outliers == 0
+
+
+def test_outlier_split_extremes():
+    """band 0: every nonzero entry is an outlier (the streams code an all-zero matrix);
+    a split of a matrix without entries beyond the band lists nothing."""
+    m = gaussian_int8(3, 1024, 6.0, seed=24)[0]
+    coded, start, rec = CompressedMatrixV1.split_outliers(m, 0)
+    assert not coded.any() and rec.size == np.count_nonzero(m) and start[-1] == rec.size
+    assert np.array_equal(E.merge_outliers(coded, start, rec), m)
+    coded, start, rec = CompressedMatrixV1.split_outliers(m, 127)
+    assert np.array_equal(coded, m) and rec.size == 0 and not start.any()
+    with pytest.raises(MatvansError):
+        CompressedMatrixV1.split_outliers(m, 200)
+
+
+def test_outlier_container_round_trip(tmp_path):
+    """MVX1 version 2: the library writes the outlier records, reads them back, and the
+    oracle's reader sees the same records; a file without outliers stays version 1."""
+    import struct
+    m = laplace_int8(9, 2048, 4.0, seed=25)[0]
+    c = CompressedMatrixV1.encode(m, 32, 8, 0, coder=1, outliers="auto")
+    plain = CompressedMatrixV1.encode(m, 64, 8, 0, coder=1)
+    p = tmp_path / "o.mvx"
+    write_mvx1(p, [c, plain], vector=np.arange(5, dtype=np.int8), result_hash=7)
+    assert struct.unpack_from("<L", p.read_bytes(), 4)[0] == 2
+    hdr, vec, mats = read_mvx1(p)
+    assert mats[0].n_outliers == c.n_outliers and mats[1].n_outliers == 0
+    assert np.array_equal(mats[0].out_start, c.out_start)
+    assert np.array_equal(mats[0].out_rec, c.out_rec)
+    _, _, recs = E.read_mvx1(p)
+    assert np.array_equal(E.merge_outliers(recs[0].decode(), recs[0].out_start,
+                                           recs[0].out_rec), m)
+    assert np.array_equal(recs[1].decode(), m)
+    q = tmp_path / "plain.mvx"
+    write_mvx1(q, [plain])
+    assert struct.unpack_from("<L", q.read_bytes(), 4)[0] == 1
+    # truncated outlier section: refused
+    data = p.read_bytes()
+    r = tmp_path / "cut.mvx"
+    r.write_bytes(data[:-(4 * plain.stored_bytes // 8)])
+    with pytest.raises(MatvansError):
+        read_mvx1(r)
