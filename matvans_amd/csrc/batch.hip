@@ -37,10 +37,14 @@ using mv::fail;
 namespace mv {
 namespace {
 
-// decode path of an item: v0, quad with 32 / 64 / 128 streams, family-B rows / stripes
+// decode path of an item: v0, quad with 32 / 64 / 128 streams, family-B rows / stripes,
+// quad with 32 streams and the products on the matrix cores (six or more vectors)
 enum : uint32_t {
-    kPathV0 = 0, kPathQ32 = 1, kPathQ64 = 2, kPathBRows = 3, kPathBStripes = 4, kPathQ128 = 5
+    kPathV0 = 0, kPathQ32 = 1, kPathQ64 = 2, kPathBRows = 3, kPathBStripes = 4, kPathQ128 = 5,
+    kPathQ32M = 6
 };
+// from this many vectors on, quad items with 32 streams per row take the matrix-core path
+constexpr uint32_t kBatchMfmaMinVec = 6;
 
 struct Item {
     uint32_t path;
@@ -53,22 +57,30 @@ struct Tile {
     uint32_t item, first, count, pad;
 };
 
-template <int NV>
-union BatchSmem {
-    blk::V1Smem<NV, 32, 8, 0> v0;
-    blk::QuadSmem<NV, 32> q32;
-    blk::QuadSmem<NV, 64> q64;
-    blk::QuadSmem<NV, 128> q128;
-    blk::BSmem b;
-};
 
 constexpr uint32_t bit(uint32_t path) { return 1u << path; }
+
+// LDS of a kernel instance: the union of the block bodies of the paths it contains only,
+// so an instance without the matrix-core path does not pay for its stage
+struct NoSmem {};
+template <bool On, class T>
+using Opt = typename std::conditional<On, T, NoSmem>::type;
+template <int NV, uint32_t MASK>
+union BatchSmem {
+    Opt<(MASK & bit(kPathV0)) != 0, blk::V1Smem<NV, 32, 8, 0>> v0;
+    Opt<(MASK & bit(kPathQ32)) != 0, blk::QuadSmem<NV, 32>> q32;
+    Opt<(MASK & bit(kPathQ64)) != 0, blk::QuadSmem<NV, 64>> q64;
+    Opt<(MASK & bit(kPathQ128)) != 0, blk::QuadSmem<NV, 128>> q128;
+    Opt<(MASK & (bit(kPathBRows) | bit(kPathBStripes))) != 0, blk::BSmem> b;
+    Opt<(MASK & bit(kPathQ32M)) != 0, blk::QuadMfmaSmem<4>> q32m;
+};
 // kernel instances: the decode paths each one contains (a plan picks the smallest one
 // that covers its items; fewer paths leave fewer registers live)
 constexpr uint32_t kQuad = bit(kPathQ32) | bit(kPathQ64) | bit(kPathQ128);
 constexpr uint32_t kMasks[] = {bit(kPathV0), bit(kPathQ32), bit(kPathQ64), bit(kPathQ128),
                                bit(kPathQ32) | bit(kPathQ64), kQuad,
-                               bit(kPathBRows) | bit(kPathBStripes), 0x3Fu};
+                               bit(kPathBRows) | bit(kPathBStripes), bit(kPathQ32M),
+                               kQuad | bit(kPathQ32M), 0x7Fu};
 
 // work-queue shards: block b draws from shard b % kShards, whose tiles are t = g, g + 8,
 // g + 16, ...; each shard's counter sits on its own 128-byte line
@@ -79,7 +91,7 @@ template <int NV, uint32_t MASK>
 __global__ __launch_bounds__(256) void batch_fwd(const Item *__restrict__ items,
                                                  const Tile *__restrict__ tiles, uint32_t n_tiles,
                                                  uint32_t *counters, uint32_t queue) {
-    __shared__ BatchSmem<NV> sm;
+    __shared__ BatchSmem<NV, MASK> sm;
     __shared__ uint32_t s_pos;
     const uint32_t tid = threadIdx.x;
     // shard g holds nt tiles and nb blocks; a block's first tile is static (position
@@ -100,18 +112,20 @@ __global__ __launch_bounds__(256) void batch_fwd(const Item *__restrict__ items,
         const uint32_t path = __builtin_amdgcn_readfirstlane(it.path);
         for (uint32_t u = 0; u < tl.count; u++) {
             const uint32_t bid = tl.first + u;
-            if ((MASK & bit(kPathV0)) && path == kPathV0)
-                blk::v1_block<NV, 32, 8, 0>(it.v1, bid, sm.v0);
-            else if ((MASK & bit(kPathQ32)) && path == kPathQ32)
-                blk::quad_block<NV, 32>(it.v1, bid, sm.q32);
-            else if ((MASK & bit(kPathQ64)) && path == kPathQ64)
-                blk::quad_block<NV, 64>(it.v1, bid, sm.q64);
-            else if ((MASK & bit(kPathQ128)) && path == kPathQ128)
-                blk::quad_block<NV, 128>(it.v1, bid, sm.q128);
-            else if ((MASK & bit(kPathBRows)) && path == kPathBRows)
-                blk::b_block<NV, 0>(it.b, bid, sm.b);
-            else if ((MASK & bit(kPathBStripes)) && path == kPathBStripes)
-                blk::b_block<NV, 1>(it.b, bid, sm.b);
+            if constexpr ((MASK & bit(kPathV0)) != 0)
+                if (path == kPathV0) blk::v1_block<NV, 32, 8, 0>(it.v1, bid, sm.v0);
+            if constexpr ((MASK & bit(kPathQ32)) != 0)
+                if (path == kPathQ32) blk::quad_block<NV, 32>(it.v1, bid, sm.q32);
+            if constexpr ((MASK & bit(kPathQ64)) != 0)
+                if (path == kPathQ64) blk::quad_block<NV, 64>(it.v1, bid, sm.q64);
+            if constexpr ((MASK & bit(kPathQ128)) != 0)
+                if (path == kPathQ128) blk::quad_block<NV, 128>(it.v1, bid, sm.q128);
+            if constexpr ((MASK & bit(kPathQ32M)) != 0)
+                if (path == kPathQ32M) blk::quad_block_mfma<NV, 4>(it.v1, bid, sm.q32m);
+            if constexpr ((MASK & bit(kPathBRows)) != 0)
+                if (path == kPathBRows) blk::b_block<NV, 0>(it.b, bid, sm.b);
+            if constexpr ((MASK & bit(kPathBStripes)) != 0)
+                if (path == kPathBStripes) blk::b_block<NV, 1>(it.b, bid, sm.b);
             __syncthreads();   // the next unit rebuilds the tables and rings
         }
         if (queue) {
@@ -157,10 +171,6 @@ struct BwdItem {
 struct RTile {
     uint32_t item, v, cb, pad;
 };
-
-struct NoSmem {};
-template <bool On, class T>
-using Opt = typename std::conditional<On, T, NoSmem>::type;
 
 template <int NV, uint32_t MASK>
 union BwdSmem {
@@ -339,8 +349,9 @@ FwdKernel fwd_kernel(const matvans_batch *p) {
 #define MV_ROW(NV) {mv::batch_fwd<NV, mv::kMasks[0]>, mv::batch_fwd<NV, mv::kMasks[1]>, \
                     mv::batch_fwd<NV, mv::kMasks[2]>, mv::batch_fwd<NV, mv::kMasks[3]>, \
                     mv::batch_fwd<NV, mv::kMasks[4]>, mv::batch_fwd<NV, mv::kMasks[5]>, \
-                    mv::batch_fwd<NV, mv::kMasks[6]>, mv::batch_fwd<NV, mv::kMasks[7]>}
-    static const FwdKernel table[4][8] = {MV_ROW(1), MV_ROW(2), MV_ROW(4), MV_ROW(8)};
+                    mv::batch_fwd<NV, mv::kMasks[6]>, mv::batch_fwd<NV, mv::kMasks[7]>, \
+                    mv::batch_fwd<NV, mv::kMasks[8]>, mv::batch_fwd<NV, mv::kMasks[9]>}
+    static const FwdKernel table[4][10] = {MV_ROW(1), MV_ROW(2), MV_ROW(4), MV_ROW(8)};
 #undef MV_ROW
     return table[nv_row(p->nv)][p->kernel];
 }
@@ -410,7 +421,8 @@ extern "C" int matvans_batch_create(const matvans_batch_item *items, uint32_t n,
                 m.units = (it.rows + 7) / 8;
                 p->chain.push_back(it.cols / 32);
             } else {
-                m.path = it.streams == 32 ? mv::kPathQ32
+                m.path = it.streams == 32 ? (it.n_vec >= mv::kBatchMfmaMinVec ? mv::kPathQ32M
+                                                                               : mv::kPathQ32)
                          : it.streams == 64 ? mv::kPathQ64 : mv::kPathQ128;
                 const uint32_t rpb = 256u / it.streams;   // rows per 4-wave block: 8, 4, 2
                 m.units = (it.rows + rpb - 1) / rpb;

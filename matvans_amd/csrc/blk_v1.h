@@ -552,5 +552,178 @@ __device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid,
     }
 }
 
+
+// ---------------------------------------------------------------------------------------
+// Quad coder, 32 streams per row, several vectors: the products on the matrix cores.
+// A block of MW waves (4 or 8) decodes 2 MW rows (two per wave, one per half-wave, as
+// quad_block).  At each 16-symbol group the block's rows x 512 columns of decoded int8
+// are the A operand of v_mfma_i32_16x16x64_i8 (16 rows; with 4 waves rows 8..15 stay
+// zero): every lane writes its four packed quads (16 columns) to an LDS stage, and after
+// one block barrier wave w multiplies stage columns 512 w / MW .. + 512 / MW - 1, 64 at a
+// time (A: lane l holds row l % 16, columns 16 (l / 16) .. + 15 of the slice) by the
+// vectors' same columns (B: lane l holds vector l % 16; vectors past n_vec read as
+// zeros).  Each wave accumulates its slices over all groups in one 16 x 16 int32 tile
+// (lane l: rows 4 (l / 16) .. + 3 of vector l % 16); the tiles are added through LDS at
+// the end.  The stage is double-buffered, so one barrier per group suffices: a wave writes
+// stage (t + 1) % 2 only after the barrier of group t, which every wave passes after
+// reading stage (t - 1) % 2.  Integer sums: bit-exact whatever the order.  Per group this
+// replaces NV v_dot4 per quad and NV vector loads with 8 / MW MFMAs and as many LDS reads
+// and vector loads, one LDS write and the barrier.
+constexpr uint32_t kStageRow = 528;   // 512 columns + 16 bytes: rows in distinct banks
+
+template <int MW>
+struct QuadMfmaSmem {
+    static constexpr int kRows = 2 * MW;
+    uint32_t ring[MW * 16 * 64];
+    uint32_t tab[256];
+    uint32_t dummy[4 * 64];
+    uint32_t stage[2][16 * kStageRow / 4];   // 16 operand rows; also the tiles' sum at the end
+    int32_t outl[kRows][8];                  // outlier corrections per row and vector
+};
+
+typedef int mv_i32x4 __attribute__((ext_vector_type(4)));
+
+template <int NV, int MW>
+__device__ __forceinline__ void quad_block_mfma(const V1Job &J, uint32_t bid,
+                                                QuadMfmaSmem<MW> &sm) {
+    static_assert(MW == 4 || MW == 8, "4 or 8 waves");
+    constexpr int S = 32;
+    constexpr uint32_t kRows = 2 * MW;
+    constexpr int KS = 8 / MW;             // 64-column slices per wave and group
+    using QL = v1::QuadLane<4, 4>;
+    const uint32_t rows = J.rows, cols = J.cols, vec_stride = J.vec_stride;
+    const uint32_t payload_size = J.payload_size;
+    const uint32_t *__restrict__ cursors = J.cursors;
+    const uint32_t *__restrict__ glut = J.glut;
+    const uint16_t *__restrict__ payload = J.payload;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wave = tid >> 6, lane = tid & 63;
+    const uint32_t sl = lane & 31;
+    const uint32_t rloc = 2 * wave + (lane >> 5);          // row within the block
+    const uint32_t row0 = bid * kRows + 2 * wave;
+    const uint32_t row = bid * kRows + rloc;
+    const bool valid = row < rows;
+    uint32_t *tab = sm.tab;
+    if (tid < 256) tab[tid] = (glut[tid] & 0xFF0000FFu) | ((glut[256 + tid] & 0xFFu) << 8);
+    if constexpr (MW == 4) {   // operand rows 8..15 of both stages stay zero
+        uint32_t *z0 = sm.stage[0] + 8 * kStageRow / 4, *z1 = sm.stage[1] + 8 * kStageRow / 4;
+        for (uint32_t k = tid; k < 8 * kStageRow / 4; k += 64 * MW) z0[k] = z1[k] = 0;
+    }
+    uint32_t wbeg = 0, wend = 0;
+    if (row0 < rows) {
+        wbeg = cursors[row0];
+        const uint32_t last = row0 + 1 < rows ? row0 + 1 : rows - 1;
+        wend = last + 1 < rows ? cursors[last + 1] : payload_size;
+        if (wend > payload_size) wend = payload_size;
+        if (wend < wbeg) wend = wbeg;
+    }
+    wbeg = __builtin_amdgcn_readfirstlane(wbeg);
+    wend = __builtin_amdgcn_readfirstlane(wend);
+    const uint32_t rcur = valid ? cursors[row] : 0u;
+    const uint32_t rend = valid ? (row + 1 < rows ? cursors[row + 1] : payload_size) : 0u;
+    RowOutliers<NV, S> outl;
+    outl.offsets(J, row, valid);
+
+    QL st;
+    st.init(payload, wbeg, wend, valid, rcur, rend, sl, sm.ring, sm.dummy, wave, lane);
+    outl.records(J, sl);
+    st.prologue();
+
+    const uint32_t groups = cols / (16u * S);
+    const uint32_t nv_eff = J.n_vec < uint32_t(NV) ? J.n_vec : uint32_t(NV);
+    const __amdgpu_buffer_rsrc_t vrs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<int8_t *>(J.vecs), 0,
+        static_cast<int>((nv_eff - 1) * vec_stride + cols), 0x00020000);
+    const uint32_t vj = lane & 15;
+    const uint32_t col_base = (512u / MW) * wave + 16u * (lane >> 4);
+    const uint32_t boff = vj < nv_eff ? vj * vec_stride + col_base : v1::kOOB;
+    auto bload = [&](uint32_t t, mv_i32x4 (&b)[KS]) {
+        const uint32_t tt = t < groups ? t : groups - 1;
+#pragma unroll
+        for (int k = 0; k < KS; k++) {
+            auto x = __builtin_amdgcn_raw_buffer_load_b128(vrs, boff + 64u * k, tt * 512u, 0);
+            b[k][0] = int(x[0]); b[k][1] = int(x[1]); b[k][2] = int(x[2]); b[k][3] = int(x[3]);
+        }
+    };
+    uint8_t *stage0 = reinterpret_cast<uint8_t *>(sm.stage[0]);
+    uint8_t *stage1 = reinterpret_cast<uint8_t *>(sm.stage[1]);
+    const uint32_t wr_off = rloc * kStageRow + 16u * sl;
+    const uint32_t rd_off = (lane & 15) * kStageRow + col_base;
+
+    mv_i32x4 b0[KS], b1[KS];
+    bload(0, b0);
+    __syncthreads();
+    st.start();
+    bload(1, b1);
+    outl.values(J, nv_eff);
+
+    mv_i32x4 d = {0, 0, 0, 0};
+    auto group = [&](uint8_t *stg, mv_i32x4 (&b)[KS], uint32_t t) {
+        uint32_t P[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) P[q] = st.quad(tab);
+        *reinterpret_cast<uint4 *>(stg + wr_off) = make_uint4(P[0], P[1], P[2], P[3]);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KS; k++) {
+            const uint4 a4 = *reinterpret_cast<const uint4 *>(stg + rd_off + 64u * k);
+            mv_i32x4 a;
+            a[0] = int(a4.x); a[1] = int(a4.y); a[2] = int(a4.z); a[3] = int(a4.w);
+            d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[k], d, 0, 0, 0);
+        }
+        bload(t + 2, b);
+    };
+    uint32_t t = 0;
+    for (; t + 1 < groups; t += 2) {
+        uint32_t wb = st.waddr;
+        st.refill_a();
+        group(stage0, b0, t);
+        st.end_group(wb);
+        wb = st.waddr;
+        st.refill_b();
+        group(stage1, b1, t + 1);
+        st.end_group(wb);
+    }
+    if (t < groups) {
+        uint32_t wb = st.waddr;
+        st.refill_a();
+        group(stage0, b0, t);
+        st.end_group(wb);
+    }
+
+    // outlier corrections of this lane's row, summed over the row's 32 lanes
+    int32_t oacc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) oacc[v] = 0;
+    outl.apply(J, sl, nv_eff, oacc);
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        int32_t a = oacc[v];
+        a += __shfl_xor(a, 16, 64);
+        a += __shfl_xor(a, 8, 64);
+        a += __shfl_xor(a, 4, 64);
+        a += __shfl_xor(a, 2, 64);
+        a += __shfl_xor(a, 1, 64);
+        oacc[v] = a;
+    }
+    __syncthreads();   // every wave is done with the stages
+    int32_t *tiles = reinterpret_cast<int32_t *>(sm.stage);   // [wave][16 rows][16 vectors]
+#pragma unroll
+    for (int r = 0; r < 4; r++) tiles[wave * 256 + (4 * (lane >> 4) + r) * 16 + (lane & 15)] = d[r];
+    if (sl == 0) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) sm.outl[rloc][v] = oacc[v];
+    }
+    __syncthreads();
+    if (tid < kRows * uint32_t(NV)) {
+        const uint32_t i = tid / NV, v = tid % NV;
+        const uint32_t r = bid * kRows + i;
+        int32_t sum = sm.outl[i][v];
+#pragma unroll
+        for (int w = 0; w < MW; w++) sum += tiles[w * 256 + i * 16 + v];
+        if (r < rows && v < nv_eff) put_row(J, int(v), r, sum);
+    }
+}
+
 }  // namespace blk
 }  // namespace mv

@@ -24,6 +24,13 @@
 #include "launch.h"
 #include "blk_v1.h"
 
+#ifndef MV_MFMA_MIN_VEC
+#define MV_MFMA_MIN_VEC 4
+#endif
+#ifndef MV_MFMA_WAVES
+#define MV_MFMA_WAVES 8
+#endif
+
 namespace mv {
 namespace {
 
@@ -49,6 +56,12 @@ __global__ __launch_bounds__(256) void v1q_fwd(V1Job job) {
     blk::quad_block<NV, S, MODE, W>(job, blockIdx.x, sm);
 }
 
+template <int NV, int MW>
+__global__ __launch_bounds__(64 * MW) void v1q_mfma(V1Job job) {
+    __shared__ blk::QuadMfmaSmem<MW> sm;
+    blk::quad_block_mfma<NV, MW>(job, blockIdx.x, sm);
+}
+
 V1Job job_of(const V1Args &a) {
     return V1Job{a.dst,       a.vecs,    a.vec_stride, a.n_vec,        a.rows,    a.cols,
                  a.cursors,   a.min_value, a.G,        a.lut,          a.freq,    a.payload,
@@ -71,10 +84,20 @@ hipError_t launch_cfg(const V1Args &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// From this many vectors on, the quad coder with 32 streams per row and one table takes
+// its products on the matrix cores (quad_block_mfma).
+constexpr uint32_t kMfmaMinVec = MV_MFMA_MIN_VEC;
+
 template <int NV>
 hipError_t launch_v1_nv(const V1Args &a, hipStream_t s) {
     if (a.coder == 1) {
         if (a.precision != 8) return hipErrorInvalidValue;
+        if (a.table_mode == 0 && a.streams == 32 && a.n_vec >= kMfmaMinVec) {
+            constexpr int MW = MV_MFMA_WAVES;
+            dim3 grid((a.rows + 2 * MW - 1) / (2 * MW)), block(64 * MW);
+            hipLaunchKernelGGL((v1q_mfma<NV, MW>), grid, block, 0, s, job_of(a));
+            return hipGetLastError();
+        }
         if (a.table_mode == 1)
             return a.streams == 64 ? launch_quad<NV, 64, 1>(a, s) : hipErrorInvalidValue;
         if (a.streams == 128) return launch_quad<NV, 128>(a, s);
