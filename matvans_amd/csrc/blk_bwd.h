@@ -4,19 +4,13 @@
 //
 // Mapping (stream-major): a unit is 8 waves = 8 consecutive streams of 64 consecutive
 // rows; a wave decodes one stream of the 64 rows, one row per lane.  A stream covers the
-// same columns in every row, so at each decode step all 64 lanes contribute to the same
-// columns.  Every four columns (one decode quad) a fixed pair step (bwd_pair) folds the
-// 64 rows to 16 lanes per column: the quad's symbols as two column pairs (two byte
-// permutes of its table entries), one v_permlane32_swap, a multiply and an fma per pair
-// of rows, one v_permlane16_swap and an add.  After a group's four quads, bwd_group_reduce sums each quad's 16 lanes for all
-// four quads at once (a transposed reduction over DPP rows: row_mirror, row_half_mirror
-// and two quad_perm adds), in the same order on every run.  The group's 16 totals are
-// stored once, to partial[v][row_block][c]; reduce_tile sums the row blocks in a fixed
-// order and applies alpha.  Both stages are deterministic bit for bit, so one-matrix and
-// batched launches give identical results.
-//
-// With two or more gradient vectors the group sums run on the matrix cores instead
-// (bwd_group_mfma), one MFMA sequence for all vectors.
+// same columns in every row, so each 16-symbol group of the wave is a 64 x 16 int8 block
+// of W whose column sums, weighted by the rows' scaled gradients, are one
+// v_mfma_i32_16x16x64_i8 per four vectors (GroupPipe: the block and the gradient
+// digits reach the matrix core through the wave's LDS stage and the transposed LDS
+// loads).  The group's 16 sums per vector are stored once, to partial[v][row_block][c];
+// reduce_tile sums the row blocks in a fixed order and applies alpha.  Both stages are
+// deterministic bit for bit, so one-matrix and batched launches give identical results.
 //
 // .bin v0 streams (create_matrices_WarpPerRow.py:361-366): stream l covers columns
 // 512*(k/16) + 16*l + k%16.  Quad-coder streams (S = 32 or 64): stream l's group t covers
@@ -36,12 +30,6 @@ namespace blk {
 
 constexpr int kBwdWaves = 8;
 constexpr int kRedParts = 16;
-
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ float dpp_f(float v) {
-    int r = __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, true);
-    return __int_as_float(r);
-}
 
 // One matrix of a backward (device pointers).
 struct BwdJob {
@@ -65,26 +53,33 @@ struct BwdJob {
     const uint32_t *out_col_rec;
 };
 
-// From this many gradient vectors on, a group's sums are computed on the matrix cores
-// (bwd_group_mfma) instead of pair steps and row reductions per vector.
-constexpr int kMfmaNV = 2;
-
+// The decode table comes first, so that its lookups address it with the LDS instruction's
+// immediate offset (below 64 KiB) and one shift per symbol; the stage next, 16-byte
+// aligned for the transposed 8-byte loads and the 16-byte stage writes.
 template <int NV>
 struct V0BwdSmem {
+    alignas(16) uint32_t lut[256];
+    alignas(16) uint32_t wstage[kBwdWaves * 256];   // per wave: 64 rows x 16 int8
     uint16_t ring[kBwdWaves * kRingWords * 64];
-    uint32_t lut[256];
     uint16_t dummy[kDummyBytes / 2];
-    uint32_t wstage[NV >= kMfmaNV ? kBwdWaves * 256 : 1];   // per wave: 64 x 16 int8
 };
 
-template <int NV, int S, int MODE = 0>
+// Ring words per lane per set of the quad backward.  4 (one set per group): 8-word sets
+// save ~2 VALU per quad but double the ring, and the batched backward then fits two
+// blocks per CU instead of three, which costs more than the VALU saved.
+template <int S, int MODE>
+constexpr int bwd_set_words() {
+    return 4;
+}
+
+template <int NV, int S, int MODE = 0, int SETW = bwd_set_words<S, MODE>()>
 struct QuadBwdSmem {
-    uint32_t ring[kBwdWaves * v1::QuadLane<128 / S>::R4 * 64];
-    uint32_t tab[MODE == 0 ? 256 : 1];
+    alignas(16) uint32_t tab[MODE == 0 ? 256 : 4];
+    alignas(16) uint32_t wstage[kBwdWaves * 256];
+    uint32_t ring[kBwdWaves * v1::QuadLane<128 / S, SETW>::R4 * 64];
     uint32_t rowtab[MODE == 1 ? 64 * 256 : 1];            // one packed table per row
     uint32_t rowscratch[MODE == 1 ? kBwdWaves * 192 : 1];  // 768 bytes per wave
-    uint32_t dummy[v1::QuadLane<128 / S>::M * 64];
-    uint32_t wstage[NV >= kMfmaNV ? kBwdWaves * 256 : 1];
+    uint32_t dummy[v1::QuadLane<128 / S, SETW>::M * 64];
 };
 
 // Per-lane setup shared by both codecs: the 64-row window, this lane's row extent and
@@ -114,145 +109,150 @@ struct BwdRows {
     }
 };
 
-// The symbols of a decode quad's four table entries e[0..3] (symbol in byte 3, column
-// order) as the two inputs of the pair step: v1 = columns 0 and 2 in bytes 0 and 2,
-// v2 = columns 1 and 3 in bytes 0 and 2.
-__device__ __forceinline__ void quad_cols(const uint32_t (&e)[4], uint32_t &v1, uint32_t &v2) {
-    v1 = __builtin_amdgcn_perm(e[2], e[0], 0x0c070c03u);
-    v2 = __builtin_amdgcn_perm(e[3], e[1], 0x0c070c03u);
+
+// ---- group sums on the matrix cores
+//
+// A group is 16 columns of the wave's 64 rows: a 64 x 16 int8 block W_g, lane r holding
+// row r as four packed quads.  Its contribution out[v][c] = sum_r W_g[r][c] gs[r][v] is
+// computed exactly in integers by v_mfma_i32_16x16x64_i8, with the scaled gradients
+// gs = s_w * g written as four int8 digits per row (GradDigits):
+//
+//   A (16 x 64): row 4 (v % 4) + d holds digit d of vector v over the 64 rows (vectors
+//                4..7 use a second A operand)
+//   B (64 x 16): W_g
+//   D (16 x 16): lane 16 k + i, register d = sum_r digit_d[r][v] W_g[r][i], v = k (+4)
+//
+// Both operands hold, in lane 16 k + i, the 16 values of K rows 16 k .. 16 k + 15 at M / N
+// index i.  A wave stores its block row-major in its LDS stage (row r at bytes 16 r ..
+// 16 r + 15, one 16-byte write per lane) and reads it back transposed with two
+// ds_read_b64_tr_b8: within each 16 lanes, that load gives output lane 8 h + j, byte m,
+// the byte j of the 8 bytes read by lane 2 m + h, so lane 2 m + h pointing at row
+// 16 k + m (+ 8 for the second load), bytes 8 h .. 8 h + 7, leaves lane 16 k + i with byte
+// i of rows 16 k .. 16 k + 7 (+ 8).  The digit operand is staged the same way once per
+// unit.  The four digit sums are exact (|sum| <= 2^20) and are combined in a fixed order,
+// so the result does not depend on scheduling.
+
+typedef int bwd_i32x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) bwd_i32x2 lds_i32x2;
+
+// Byte offset, within a wave's 1 KiB stage, of the 8 bytes this lane reads in the first
+// transposed load (the second reads 128 bytes further: rows + 8).
+__device__ __forceinline__ uint32_t tr_offset(uint32_t lane) {
+    return 16u * (16u * (lane >> 4) + ((lane & 15u) >> 1)) + 8u * (lane & 1u);
 }
 
-// Pair step of one decode quad of this lane's row (v1, v2 from quad_cols), gsA / gsB the
-// scaled gradients of rows (lane & 31) and (lane & 31) + 32.  Swapping v1 against v2
-// across the wave halves leaves, in bytes 0 and 2, columns (0, 2) in lanes 0..31 and
-// (1, 3) in lanes 32..63, of row (lane & 31) in X and of row (lane & 31) + 32 in Y.  a, b
-// are those two columns' pair sums; v_permlane16_swap + add leaves the 16-lane rows
-// k = 0..3 of the wave with columns (0, 2, 1, 3), each lane holding the sum over four rows.
+// The stage (64 rows x 16 bytes) as an MFMA operand: lane 16 k + i gets byte i of rows
+// 16 k .. 16 k + 15.
+__device__ __forceinline__ mv_i32x4 stage_operand(uint8_t *stage, uint32_t tr_off) {
+    const bwd_i32x2 lo = __builtin_amdgcn_ds_read_tr8_b64_v2i32((lds_i32x2 *)(stage + tr_off));
+    const bwd_i32x2 hi =
+        __builtin_amdgcn_ds_read_tr8_b64_v2i32((lds_i32x2 *)(stage + tr_off + 128u));
+    mv_i32x4 r;
+    r[0] = lo[0]; r[1] = lo[1]; r[2] = hi[0]; r[3] = hi[1];
+    return r;
+}
+
+// Scaled gradients of the wave's 64 rows as int8 digits.  For vector v, with m = f 2^e
+// (f in [0.5, 1)) the largest |gs| over the 64 rows, q = rint(gs 2^-e * 127 * 2^24 / f) is
+// an integer with |q| <= 127 * 2^24 + 256 < 2^31, written in balanced base 256,
+// q = 2^24 d0 + 2^16 d1 + 2^8 d2 + d3, every digit in [-128, 127] (d3, d2, d1 are
+// sign-extended low bytes and the remainders exact multiples of 256).  gs = u q with
+// u = m / (127 * 2^24), up to the rounding of the product (2^-24 of |gs|) and of rint
+// (u / 2 = 2^-32.99 m): as close as an fp32 product for all but the smallest gs.
+// uh, ul = 2^16 u, u for the vector of this lane's D registers.
 template <int NV>
-__device__ __forceinline__ void bwd_pair(uint32_t v1, uint32_t v2, const float (&gsA)[NV],
-                                         const float (&gsB)[NV], float (&s)[NV]) {
-    auto sw = __builtin_amdgcn_permlane32_swap(v1, v2, false, false);
-    const uint32_t X = sw[0], Y = sw[1];
-    const float xa = cvt_f32_sbyte0(X), xb = cvt_f32_sbyte2(X);
-    const float ya = cvt_f32_sbyte0(Y), yb = cvt_f32_sbyte2(Y);
+struct GradDigits {
+    static constexpr int NA = NV > 4 ? 2 : 1;
+    mv_i32x4 a[NA];
+    float uh[NA], ul[NA];   // 2^16 u and u
+    __device__ __forceinline__ void init(const float (&gs)[NV], uint32_t lane, uint8_t *stage,
+                                         uint32_t tr_off) {
+        uint32_t pk[4 * NA];
+        float top[4 * NA];   // m / 127 = 2^24 u
 #pragma unroll
-    for (int v = 0; v < NV; v++) {
-        const float a = fmaf(ya, gsB[v], xa * gsA[v]);
-        const float b = fmaf(yb, gsB[v], xb * gsA[v]);
-        auto t = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false,
-                                                  false);
-        s[v] = __uint_as_float(t[0]) + __uint_as_float(t[1]);
-    }
-}
-
-// Sums s[d] over the 16 lanes of each DPP row for the four quads d of a group at once:
-// lanes i < 8 of a row keep quads 0, 1 and hand quads 2, 3 to lane 15 - i (row_mirror),
-// then lanes with bit 2 clear keep the lower of their two quads and hand the other to
-// lane i ^ 7 (row_half_mirror), then two quad_perm adds finish the row.  Lane i of row k
-// ends with the total of quad i >> 2, column 4 (i >> 2) + (0, 2, 1, 3)[k].
-template <int NV>
-__device__ __forceinline__ void bwd_group_reduce(const float (&s)[4][NV], uint32_t lane,
-                                                 float (&tot)[NV]) {
-    const bool hi8 = (lane & 8u) != 0, hi4 = (lane & 4u) != 0;
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-        const float u0 = (hi8 ? s[2][v] : s[0][v]) + dpp_f<0x140, 0xF>(hi8 ? s[0][v] : s[2][v]);
-        const float u1 = (hi8 ? s[3][v] : s[1][v]) + dpp_f<0x140, 0xF>(hi8 ? s[1][v] : s[3][v]);
-        float w = (hi4 ? u1 : u0) + dpp_f<0x141, 0xF>(hi4 ? u0 : u1);
-        w += dpp_f<0xB1, 0xF>(w);    // quad_perm [1,0,3,2]
-        w += dpp_f<0x4E, 0xF>(w);    // quad_perm [2,3,0,1]
-        tot[v] = w;
-    }
-}
-
-// Per-lane constants of the pair step and the group store.
-template <int NV>
-struct BwdLane {
-    float gsA[NV], gsB[NV];
-    uint32_t col;     // column of this lane's group total within the 16-column group
-    bool writer;      // lanes 4q of each 16-lane row hold quad q's total
-    __device__ __forceinline__ void init(const float (&gs)[NV], uint32_t lane) {
+        for (int v = 0; v < 4 * NA; v++) {
+            pk[v] = 0u;
+            top[v] = 0.0f;
+        }
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-            auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(gs[v]),
-                                                       __float_as_uint(gs[v]), false, false);
-            gsA[v] = __uint_as_float(sw[0]);
-            gsB[v] = __uint_as_float(sw[1]);
-        }
-        const uint32_t i = lane & 15, k = lane >> 4;   // rows of 16 lanes: columns 0, 2, 1, 3
-        col = 4 * (i >> 2) + ((k & 1) * 2 + (k >> 1));
-        writer = (i & 3) == 0;
-    }
-    __device__ __forceinline__ void store(const float (&tot)[NV], float *out, size_t plane,
-                                          uint32_t n_vec) const {
-        if (writer) {
+            float m = fabsf(gs[v]);
 #pragma unroll
-            for (int v = 0; v < NV; v++)
-                if (v < int(n_vec)) out[size_t(v) * plane + col] = tot[v];
+            for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+            int e = 0;
+            const float f = frexpf(m, &e);   // 0 for m = 0
+            const float inv = m > 0.0f ? 2130706432.0f / f : 0.0f;   // 127 * 2^24 / f
+            top[v] = ldexpf(f * (1.0f / 127.0f), e);
+            const int32_t q = int32_t(rintf(ldexpf(gs[v], -e) * inv));
+            const int32_t d3 = int32_t(uint32_t(q) << 24) >> 24;
+            const int32_t q1 = (q - d3) >> 8;
+            const int32_t d2 = int32_t(uint32_t(q1) << 24) >> 24;
+            const int32_t q2 = (q1 - d2) >> 8;
+            const int32_t d1 = int32_t(uint32_t(q2) << 24) >> 24;
+            const int32_t d0 = (q2 - d1) >> 8;
+            pk[v] = (uint32_t(d0) & 0xFFu) | ((uint32_t(d1) & 0xFFu) << 8) |
+                    ((uint32_t(d2) & 0xFFu) << 16) | (uint32_t(d3) << 24);
         }
-    }
-    // a group's four quads (next(v1, v2) decodes one, see quad_cols): pair steps, then
-    // the transposed row reduction
-    template <class Next>
-    __device__ __forceinline__ void group(Next next, uint32_t lane, float (&tot)[NV]) const {
-        float s[4][NV];
-        // quad d's pair step follows quad d+1's decode: independent work between the
-        // decode's dependent steps and the swaps' operand hazards
-        uint32_t p1, p2;
-        next(p1, p2);
+        const uint32_t k = lane >> 4;
 #pragma unroll
-        for (int d = 1; d < 4; d++) {
-            uint32_t c1, c2;
-            next(c1, c2);
-            bwd_pair<NV>(p1, p2, gsA, gsB, s[d - 1]);
-            p1 = c1;
-            p2 = c2;
-        }
-        bwd_pair<NV>(p1, p2, gsA, gsB, s[3]);
-        bwd_group_reduce<NV>(s, lane, tot);
-    }
-};
-
-// Matrix-core path for several gradient vectors.  A group's 16 columns of the 64 rows
-// form a 64 x 16 int8 block W_g (lane r holds row r: four packed quads).  The group sums
-// out[c][v] = sum_r W_g[r][c] gs[r][v] are W_g^T gs, a 16 x 64 by 64 x 16 product:
-// sixteen v_mfma_f32_16x16x4f32 steps over slabs of four rows.  In slab s, lane l supplies
-// A[l%16][l/16] = W_g[4s + l/16][l%16] (read back from the wave's LDS stage) and
-// B[l/16][l%16] = gs[4s + l/16][v = l%16] (MfmaGrad); the result D[i][j] = out[i][j] is
-// held by lane 16 (i/4) + j, register i%4, so each lane stores four consecutive columns of
-// one vector.  The hardware's summation order is fixed: deterministic like the pair steps.
-struct MfmaGrad {
-    float b[16];   // slab s: s_w[r] g[l%16][r], r = 4s + l/16 (0 past the rows / n_vec)
-    __device__ __forceinline__ void init(const BwdJob &J, uint32_t rb, uint32_t lane) {
-        const uint32_t v = lane & 15;
+        for (int h = 0; h < NA; h++) {
+            // the wave's own LDS accesses run in order: each read sees the write before it
+            *reinterpret_cast<uint4 *>(stage + 16u * lane) =
+                make_uint4(pk[4 * h], pk[4 * h + 1], pk[4 * h + 2], pk[4 * h + 3]);
+            a[h] = stage_operand(stage, tr_off);
+            float t = top[4 * h];
 #pragma unroll
-        for (int s = 0; s < 16; s++) {
-            const uint32_t r = rb * 64u + 4u * s + (lane >> 4);
-            b[s] = r < J.rows && v < J.n_vec ? J.s_w[r] * J.g[size_t(v) * J.g_stride + r] : 0.0f;
+            for (int j = 1; j < 4; j++) t = k == uint32_t(j) ? top[4 * h + j] : t;
+            uh[h] = t * 0x1p-8f;
+            ul[h] = t * 0x1p-24f;
         }
     }
 };
 
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ void bwd_group_mfma(const uint32_t (&P)[4], const float (&b)[16],
-                                               uint32_t lane, uint32_t *wst, float *out,
-                                               size_t plane, uint32_t n_vec) {
-    // stage row r = lane at bytes 16r .. 16r + 15 (the wave's LDS accesses run in order, so
-    // the reads below see this write and the next group's write follows these reads)
-    *reinterpret_cast<uint4 *>(wst + 4 * lane) = make_uint4(P[0], P[1], P[2], P[3]);
-    const int8_t *w8 = reinterpret_cast<const int8_t *>(wst);
-    const uint32_t i = lane & 15, k = lane >> 4;
-    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+// The group step, software-pipelined by the units.  After group t's decode: mma() issues
+// group t - 1's products (its B operand, read from the LDS stage before the decode, has
+// long arrived: no wait on the LDS queue in the decode's instruction stream), stage()
+// writes group t's packed quads P[0..3] (the lane's row, 16 columns in order) to the stage
+// and issues its transposed reads, and finish() scales and stores group t - 1's sums.  A
+// group's sums 2^24 D0 + 2^16 D1 + 2^8 D2 + D3 are scaled as
+// 2^16 u (2^8 D0 + D1) + u (2^8 D2 + D3), both pairs exact in int32 (|D| <= 2^20).
+// finish() stores vector v's 16 column sums to out + v * plane + 0 .. 15: out is
+// wave-uniform, the lane's offset fixed (loff).
+template <int NV>
+struct GroupPipe {
+    static constexpr int NA = GradDigits<NV>::NA;
+    mv_i32x4 d[NA];
+    uint32_t loff[NA];   // (4 h + k) * plane + i for lane 16 k + i
+    bool on[NA];         // 4 h + k < n_vec
+    __device__ __forceinline__ void init(uint32_t lane, size_t plane, uint32_t n_vec) {
+        const uint32_t i = lane & 15u, k = lane >> 4;
 #pragma unroll
-    for (int s = 0; s < 16; s++) {
-        const float a = static_cast<float>(w8[(4 * s + k) * 16 + i]);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[s], acc, 0, 0, 0);
+        for (int h = 0; h < NA; h++) {
+            loff[h] = uint32_t((4u * h + k) * plane) + i;
+            on[h] = 4u * h + k < n_vec;
+        }
     }
-    if (i < n_vec)   // lane l: columns 4 (l/16) .. +3 of vector l%16
-        *reinterpret_cast<float4 *>(out + size_t(i) * plane + 4 * k) =
-            make_float4(acc[0], acc[1], acc[2], acc[3]);
-}
+    __device__ __forceinline__ static mv_i32x4 stage(const uint32_t (&P)[4], uint32_t lane,
+                                                      uint8_t *stg, uint32_t tr_off) {
+        *reinterpret_cast<uint4 *>(stg + 16u * lane) = make_uint4(P[0], P[1], P[2], P[3]);
+        return stage_operand(stg, tr_off);
+    }
+    __device__ __forceinline__ void mma(const GradDigits<NV> &gd, const mv_i32x4 &b) {
+        const mv_i32x4 zero = {0, 0, 0, 0};
+#pragma unroll
+        for (int h = 0; h < NA; h++)
+            d[h] = __builtin_amdgcn_mfma_i32_16x16x64_i8(gd.a[h], b, zero, 0, 0, 0);
+    }
+    __device__ __forceinline__ void finish(const GradDigits<NV> &gd, float *out) const {
+#pragma unroll
+        for (int h = 0; h < NA; h++) {
+            const int32_t hi = int32_t(uint32_t(d[h][0]) << 8) + d[h][1];
+            const int32_t lo = int32_t(uint32_t(d[h][2]) << 8) + d[h][3];
+            const float t = fmaf(float(hi), gd.uh[h], float(lo) * gd.ul[h]);
+            if (on[h]) out[loff[h]] = t;
+        }
+    }
+};
 
 // .bin v0 unit: row block rb, streams 8 sg .. 8 sg + 7.  fresh (block-uniform): build the
 // table and synchronise the block first; a unit that follows one of the same matrix in
@@ -291,31 +291,24 @@ __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32
         return __builtin_amdgcn_perm(e[1], e[0], 0x0c0c0703u) |
                __builtin_amdgcn_perm(e[3], e[2], 0x07030c0cu);
     };
-    BwdLane<NV> L;
-    MfmaGrad M;
-    if constexpr (NV >= kMfmaNV) M.init(J, rb, lane);
-    else L.init(R.gs, lane);
+    uint8_t *stage = reinterpret_cast<uint8_t *>(sm.wstage + wave * 256);
+    const uint32_t tr_off = tr_offset(lane);
+    GradDigits<NV> gd;
+    gd.init(R.gs, lane, stage, tr_off);
+    GroupPipe<NV> pipe;
+    pipe.init(lane, plane, J.n_vec);
+    float *prev = nullptr;   // group t - 1's output, its B operand in b
+    mv_i32x4 b = {0, 0, 0, 0};
     auto group = [&](uint32_t t) {
         uint32_t wbefore = st.waddr;
-        if constexpr (NV >= kMfmaNV) {
-            uint32_t P[4];
+        uint32_t P[4];
 #pragma unroll
-            for (int d = 0; d < 4; d++) P[d] = next4();
-            bwd_group_mfma(P, M.b, lane, sm.wstage + wave * 256, base + t * 512u, plane,
-                           J.n_vec);
-        } else {
-            float tot[NV];
-            L.group(
-                [&](uint32_t &v1, uint32_t &v2) {
-                    uint32_t e[4];
-                    st.pair_packed(sm.lut, e[0], e[1]);
-                    st.pair_packed(sm.lut, e[2], e[3]);
-                    quad_cols(e, v1, v2);
-                },
-                lane, tot);
-            L.store(tot, base + t * 512u, plane, J.n_vec);
-        }
+        for (int d = 0; d < 4; d++) P[d] = next4();
         st.end_group(wbefore);
+        if (prev) pipe.mma(gd, b);
+        b = GroupPipe<NV>::stage(P, lane, stage, tr_off);
+        if (prev) pipe.finish(gd, prev);
+        prev = base + t * 512u;
     };
     uint32_t t = 0;
     for (; t + 1 < groups; t += 2) {
@@ -328,16 +321,22 @@ __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32
         st.refill_a();
         group(t);
     }
+    if (prev) {
+        pipe.mma(gd, b);
+        pipe.finish(gd, prev);
+    }
 }
 
 // Quad-coder unit (S = 32, 64 or 128 streams): row block rb, streams 8 sg .. 8 sg + 7;
 // fresh as for v0_bwd_unit.
-template <int NV, int S, int MODE = 0>
+template <int NV, int S, int MODE = 0, int SETW = bwd_set_words<S, MODE>()>
 __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint32_t sg,
-                                              uint32_t nrb, QuadBwdSmem<NV, S, MODE> &sm,
+                                              uint32_t nrb, QuadBwdSmem<NV, S, MODE, SETW> &sm,
                                               bool fresh = true) {
     static_assert(MODE == 0 || S == 64, "one table per row: 64 streams");
-    using QL = v1::QuadLane<128 / S>;
+    using QL = v1::QuadLane<128 / S, SETW>;
+    constexpr int GPS = SETW / 4;   // groups per set period
+    static_assert(GPS == 1 || GPS == 2, "set period of one or two groups");
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid >> 6, lane = tid & 63;
     // packed table T[q] = f | r << 8 | sym << 24: from the A / B tables (A = f | sym << 24,
@@ -376,41 +375,51 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
     const uint32_t groups = cols / (16u * S);
     const size_t plane = size_t(nrb) * cols;
     float *base = J.partial + size_t(rb) * cols + stream * 16u;
-    BwdLane<NV> L;
-    MfmaGrad M;
-    if constexpr (NV >= kMfmaNV) M.init(J, rb, lane);
-    else L.init(R.gs, lane);
+    uint8_t *stage = reinterpret_cast<uint8_t *>(sm.wstage + wave * 256);
+    const uint32_t tr_off = tr_offset(lane);
+    GradDigits<NV> gd;
+    gd.init(R.gs, lane, stage, tr_off);
+    GroupPipe<NV> pipe;
+    pipe.init(lane, plane, J.n_vec);
+    float *prev = nullptr;   // group t - 1's output, its B operand in b
+    mv_i32x4 b = {0, 0, 0, 0};
     auto group = [&](uint32_t t) {
-        uint32_t wbefore = st.waddr;
-        if constexpr (NV >= kMfmaNV) {
-            uint32_t P[4];
+        uint32_t P[4];
 #pragma unroll
-            for (int d = 0; d < 4; d++) P[d] = st.quad(tab);
-            bwd_group_mfma(P, M.b, lane, sm.wstage + wave * 256, base + t * (16u * S), plane,
-                           J.n_vec);
-        } else {
-            float tot[NV];
-            L.group(
-                [&](uint32_t &v1, uint32_t &v2) {
-                    uint32_t e[4];
-                    st.quad_entries(tab, e);
-                    quad_cols(e, v1, v2);
-                },
-                lane, tot);
-            L.store(tot, base + t * (16u * S), plane, J.n_vec);
+        for (int d = 0; d < 4; d++) P[d] = st.quad(tab);
+        if (prev) pipe.mma(gd, b);
+        b = GroupPipe<NV>::stage(P, lane, stage, tr_off);
+        if (prev) pipe.finish(gd, prev);
+        prev = base + t * (16u * S);
+    };
+    // a set period: GPS groups after one refill, the consumed words counted once (at most
+    // 4 * GPS < R4)
+    auto period = [&](uint32_t t0, uint32_t ng) {
+        uint32_t wbefore = st.waddr;
+        group(t0);
+        if constexpr (GPS == 2) {
+            if (ng > 1) group(t0 + 1);
         }
         st.end_group(wbefore);
     };
     uint32_t t = 0;
-    for (; t + 1 < groups; t += 2) {
+    for (; t + 2 * GPS <= groups; t += 2 * GPS) {
         st.refill_a();
-        group(t);
+        period(t, GPS);
         st.refill_b();
-        group(t + 1);
+        period(t + GPS, GPS);
     }
     if (t < groups) {
         st.refill_a();
-        group(t);
+        period(t, groups - t < uint32_t(GPS) ? groups - t : uint32_t(GPS));
+        if (t + GPS < groups) {
+            st.refill_b();
+            period(t + GPS, groups - t - GPS);
+        }
+    }
+    if (prev) {
+        pipe.mma(gd, b);
+        pipe.finish(gd, prev);
     }
 }
 
