@@ -117,6 +117,13 @@ int matvans_decompress_and_mult(const matvans_dev_matrix *m, int8_t *d_result8,
 int matvans_absmax_requant(const int32_t *d_acc, uint32_t n, int8_t *d_out8, float *d_delta,
                            int clamp, void *stream);
 
+/* A12 epilogue, the fp32 chain's: a = max|y| (absMaxWithThrustDevice,
+ * baselines/cuBLAS_baseline_float/baseline_matVec_cuBLAS.cu:48-57), delta = a / 127,
+ * y[i] = roundf(y[i] / delta) in place (normalizeAndRound, :59-64, launched at :129-135):
+ * IEEE fp32, rounding half away from zero; delta written to *d_delta (device).  As in the
+ * reference, y == 0 gives delta 0 and NaN outputs. */
+int matvans_f32_normalize_round(float *d_y, uint32_t n, float *d_delta, void *stream);
+
 /* Scaled fp32 output y[v*rows + r] = s_x[v] * s_w[r] * float(acc[v*rows + r]).
  * s_w is the per-row weight scale (device, [rows]), s_x per vector (device, [n_vec]).
  * No reference counterpart (the .bin grid_spacing is always 1.0,

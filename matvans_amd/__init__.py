@@ -24,7 +24,8 @@ from . import _check
 from .capi import DevMatrix, HostMatrix, MatvansError, call, lib
 
 __all__ = ["CompressedMatrix", "MatvansError", "decmpressAndMultiply", "decompress_and_multiply",
-           "int8_matvec", "f32_gemv", "absmax_requant", "scale_output", "hash_int8_array",
+           "int8_matvec", "f32_gemv", "absmax_requant", "f32_normalize_round", "scale_output",
+           "hash_int8_array",
            "build_model", "encode_rows", "read_bin", "write_bin", "run_chain", "lib",
            "quantize_vectors", "compressed_linear"]
 
@@ -379,6 +380,22 @@ def absmax_requant(acc, clamp=False, out8=None, delta=None, stream=None):
     call("matvans_absmax_requant", _ptr(acc), n, _ptr(out8), _ptr(delta), int(bool(clamp)),
          _stream(stream))
     return out8, delta
+
+
+def f32_normalize_round(y, delta=None, stream=None):
+    """The fp32 chain's epilogue, in place: y = roundf(y / delta), delta = max|y| / 127
+    (round half away from zero).  Returns (y, fp32 delta tensor[1])."""
+    torch = _torch()
+    _check.tensor("y", y, dtype=torch.float32)
+    if not y.is_contiguous():
+        raise ValueError("y: must be contiguous")
+    if y.numel() == 0:
+        raise ValueError("y: empty")
+    if delta is None:
+        delta = torch.empty(1, dtype=torch.float32, device=y.device)
+    _check.tensor("delta", delta, dtype=torch.float32, device=y.device, min_numel=1)
+    call("matvans_f32_normalize_round", _ptr(y), y.numel(), _ptr(delta), _stream(stream))
+    return y, delta
 
 
 def scale_output(acc, s_w, s_x, out=None, stream=None):

@@ -92,6 +92,7 @@ SIGNATURES = {
                                                    P, u32, P]),
     "matvans_decompress_and_mult": (i32, [ctypes.POINTER(DevMatrix), P, P, P, P, P]),
     "matvans_absmax_requant": (i32, [P, u32, P, P, i32, P]),
+    "matvans_f32_normalize_round": (i32, [P, u32, P, P]),
     "matvans_scale_output": (i32, [P, u32, u32, P, P, P, P]),
     "matvans_int8_matvec": (i32, [P, P, P, u32, u32, u32, u32, P]),
     "matvans_f32_gemv": (i32, [P, P, P, u32, u32, f32, P]),

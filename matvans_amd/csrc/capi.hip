@@ -229,6 +229,13 @@ extern "C" int matvans_absmax_requant(const int32_t *d_acc, uint32_t n, int8_t *
                       "absmax_requant launch");
 }
 
+extern "C" int matvans_f32_normalize_round(float *d_y, uint32_t n, float *d_delta, void *stream) {
+    if (!d_y || !d_delta) return fail(MATVANS_E_INVALID, "null device pointer");
+    if (n == 0) return fail(MATVANS_E_SHAPE, "empty vector");
+    return hip_status(mv::launch_f32_normalize_round(d_y, n, d_delta, as_stream(stream)),
+                      "f32_normalize_round launch");
+}
+
 extern "C" int matvans_scale_output(const int32_t *d_acc, uint32_t rows, uint32_t n_vec,
                                     const float *d_s_w, const float *d_s_x, float *d_y,
                                     void *stream) {

@@ -8,6 +8,9 @@
 //  absmax_requant  absMaxWithThrustDevice + normalizeAndRoundtoInt8
 //                (matvansWarpPerRow/matvansWarpPerRow.cu:51-72), in one stream-ordered
 //                launch that leaves delta on the device.
+//  f32_normalize_round  absMaxWithThrustDevice + normalizeAndRound of the fp32 chain
+//                (baselines/cuBLAS_baseline_float/baseline_matVec_cuBLAS.cu:48-64,129-135):
+//                y = roundf(y / (max|y| / 127)) in place, delta left on the device.
 //  scale_output  y = s_x * s_w[r] * float(acc).
 //
 // Both matvecs are HBM-bound: each wave streams whole rows with 16-byte nontemporal
@@ -227,6 +230,46 @@ __global__ __launch_bounds__(1024) void absmax_requant_kernel(const int32_t *__r
     }
 }
 
+// One block: block-wide max of |y| (fp32; the order of a max does not matter), delta =
+// max / 127, then y[i] = roundf(y[i] / delta) (IEEE division, half away from zero).
+__global__ __launch_bounds__(1024) void f32_normalize_round_kernel(float *__restrict__ y,
+                                                                   uint32_t n,
+                                                                   float *__restrict__ delta) {
+    __shared__ float red[16];
+    float m = 0.0f;
+    // four loads in flight per thread; an index past n re-reads element i (max unchanged)
+    for (uint32_t i = threadIdx.x; i < n; i += 4 * blockDim.x) {
+        float a[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint32_t j = i + k * blockDim.x;
+            a[k] = fabsf(y[j < n ? j : i]);
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) m = m < a[k] ? a[k] : m;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float t = __shfl_xor(m, o, 64);
+        m = m < t ? t : m;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        float t = threadIdx.x < (blockDim.x >> 6) ? red[threadIdx.x] : 0.0f;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const float u = __shfl_xor(t, o, 64);
+            t = t < u ? u : t;
+        }
+        if (threadIdx.x == 0) red[0] = t;
+    }
+    __syncthreads();
+    const float d = red[0] / 127.0f;
+    if (threadIdx.x == 0) *delta = d;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) y[i] = roundf(y[i] / d);
+}
+
 __global__ void scale_output_kernel(const int32_t *__restrict__ acc, uint32_t rows,
                                     uint32_t n_vec, const float *__restrict__ s_w,
                                     const float *__restrict__ s_x, float *__restrict__ y) {
@@ -338,6 +381,11 @@ hipError_t launch_absmax_requant(const int32_t *acc, uint32_t n, int8_t *out8, f
                                  int clamp, hipStream_t s) {
     hipLaunchKernelGGL(absmax_requant_kernel, dim3(1), dim3(1024), 0, s, acc, n, out8, delta,
                        clamp);
+    return hipGetLastError();
+}
+
+hipError_t launch_f32_normalize_round(float *y, uint32_t n, float *delta, hipStream_t s) {
+    hipLaunchKernelGGL(f32_normalize_round_kernel, dim3(1), dim3(1024), 0, s, y, n, delta);
     return hipGetLastError();
 }
 

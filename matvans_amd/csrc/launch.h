@@ -110,6 +110,7 @@ hipError_t launch_f32_gemv(float *y, const float *mat, const float *x, uint32_t 
                            uint32_t cols, float alpha, hipStream_t s);
 hipError_t launch_absmax_requant(const int32_t *acc, uint32_t n, int8_t *out8, float *delta,
                                  int clamp, hipStream_t s);
+hipError_t launch_f32_normalize_round(float *y, uint32_t n, float *delta, hipStream_t s);
 hipError_t launch_scale_output(const int32_t *acc, uint32_t rows, uint32_t n_vec,
                                const float *s_w, const float *s_x, float *y, hipStream_t s);
 

@@ -105,6 +105,22 @@ def requant_f32(acc, clamp=False):
     return q.astype(np.int8), delta
 
 
+def normalize_round_f32(y):
+    """The fp32 comparator chain's epilogue (baselines/cuBLAS_baseline_float/
+    baseline_matVec_cuBLAS.cu:48-64, 129-135): a = max|y| (thrust transform_reduce of fabs
+    with maximum), delta = a / 127 in fp32, y = roundf(y / delta) in fp32 (round half away
+    from zero).  Returns (float32 y, float32 delta)."""
+    y = np.asarray(y, dtype=np.float32)
+    a = np.abs(y).max() if y.size else np.float32(0)
+    delta = np.float32(np.float32(a) / np.float32(127))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        q = (y / delta).astype(np.float32)
+        t = np.trunc(q)
+        f = q - t                       # exact: q and trunc(q) share the exponent range
+        r = t + np.where(np.abs(f) >= np.float32(0.5), np.sign(q), np.float32(0))
+    return r.astype(np.float32), delta
+
+
 # ------------------------------------------------------------------ entropy model
 def create_entropy_model(data, precision: int = PRECISION):
     """8-bit quantised model (create_matrices_WarpPerRow.py:131-187).

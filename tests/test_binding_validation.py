@@ -47,6 +47,7 @@ def test_host_tensors_rejected(w, m0, mq):
                lambda: mv.int8_matvec(torch.from_numpy(w), x),
                lambda: mv.f32_gemv(torch.zeros(4, 8), torch.zeros(8)),
                lambda: mv.absmax_requant(torch.zeros(4, dtype=torch.int32)),
+               lambda: mv.f32_normalize_round(torch.zeros(4)),
                lambda: mv.scale_output(torch.zeros(4, dtype=torch.int32), torch.zeros(4),
                                        torch.zeros(1)),
                lambda: mv.quantize_vectors(torch.zeros(8)),
@@ -115,6 +116,8 @@ def test_dense_rejections(fake_device):
         mv.absmax_requant(torch.zeros(4, dtype=torch.int32), out8=torch.zeros(2, dtype=torch.int8))
     with pytest.raises(ValueError, match="delta: dtype"):
         mv.absmax_requant(torch.zeros(4, dtype=torch.int32), delta=torch.zeros(1, dtype=torch.int32))
+    with pytest.raises(ValueError, match="y: dtype"):
+        mv.f32_normalize_round(torch.zeros(4, dtype=torch.float64))
     with pytest.raises(ValueError, match="s_w: shape"):
         mv.scale_output(torch.zeros((2, 4), dtype=torch.int32), torch.zeros(3), torch.zeros(2))
     with pytest.raises(ValueError, match="s_x: shape"):

@@ -1,6 +1,7 @@
 """GPU parity of the comparators and epilogues: int8 sdot4 matvec (bit-exact),
-fp32 GEMV (vs FP64), absmax/requant (bit-exact vs the fp32 oracle epilogue) and the
-scaled fp32 output (1e-6 relative vs FP64)."""
+fp32 GEMV (vs FP64), absmax/requant (bit-exact vs the fp32 oracle epilogue), the fp32
+chain's normalize-and-round (bit-exact vs the oracle) and the scaled fp32 output (1e-6
+relative vs FP64)."""
 import sys
 
 import numpy as np
@@ -81,6 +82,27 @@ def test_absmax_requant(cuda, clamp):
             assert d.cpu().numpy()[0] == refd
         else:
             assert np.array_equal(out8.cpu().numpy(), np.zeros(7, np.int8))
+
+
+def test_f32_normalize_round(cuda):
+    import torch
+    note("f32 normalize and round")
+    rng = np.random.default_rng(3)
+    cases = [(rng.standard_normal(n) * sc).astype(np.float32)
+             for n, sc in [(1, 1.0), (1000, 3e4), (4096, 1e-3), (14336, 7.0), (100003, 1e6)]]
+    cases.append(np.array([127.0, 0.5, -0.5, 1.5, 2.5, -2.5, 0.49999997, -126.5], np.float32))
+    for y in cases:
+        d_y = torch.from_numpy(y.copy()).cuda()
+        _, d = mv.f32_normalize_round(d_y)
+        torch.cuda.synchronize()
+        ref, refd = O.normalize_round_f32(y)
+        assert d.cpu().numpy()[0] == refd
+        assert np.array_equal(d_y.cpu().numpy(), ref)
+    # all zeros: delta 0 and NaN, as in the reference
+    d_y = torch.zeros(5, device="cuda")
+    _, d = mv.f32_normalize_round(d_y)
+    torch.cuda.synchronize()
+    assert d.cpu().numpy()[0] == 0 and bool(torch.isnan(d_y).all())
 
 
 def test_scale_output(cuda):

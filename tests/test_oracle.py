@@ -97,6 +97,24 @@ def test_requant_closed_form():
     assert list(y) == [127, 0, 2, 2]   # 0.5 -> 0, 1.5 -> 2, 2.5 -> 2
 
 
+def test_normalize_round_f32_half_away_and_closed_form():
+    """The fp32 chain's epilogue: delta = max|y| / 127, roundf(y / delta) with ties away
+    from zero (np.rint would give 2 for 2.5, 0 for 0.5), against floor(|q| + 0.5) taken
+    in float64 (exact for float32 q) on random data."""
+    y = np.array([127.0, 0.5, -0.5, 1.5, 2.5, -2.5, 0.49999997, -126.5, 3.0], np.float32)
+    r, d = O.normalize_round_f32(y)
+    assert d == np.float32(1.0)
+    assert list(r) == [127, 1, -1, 2, 3, -3, 0, -127, 3]
+    rng = np.random.default_rng(4)
+    for n in (1, 1000, 14336):
+        y = (rng.standard_normal(n) * rng.choice([1e-3, 1.0, 3e4])).astype(np.float32)
+        r, d = O.normalize_round_f32(y)
+        assert d == np.float32(np.abs(y).max()) / np.float32(127)
+        q = (y / d).astype(np.float32).astype(np.float64)
+        want = np.sign(q) * np.floor(np.abs(q) + 0.5)
+        assert np.array_equal(r.astype(np.float64), want)
+
+
 # --------------------------------------------------------------- model and ppf
 @pytest.mark.parametrize("which", [0, 1])
 def test_model_and_ppf_match_golden(fx512, raw512, which):
