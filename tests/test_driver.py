@@ -68,13 +68,19 @@ def test_driver_chain_on_golden_bin(gpu_ready, fx):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("streams,precision,coder", [(32, 8, 1), (32, 12, 0), (32, 8, 0)])
-def test_driver_chain_on_mvx1(gpu_ready, tmp_path, streams, precision, coder):
-    """The golden raw matrices re-encoded as MVX1: same chain, same result_hash."""
+@pytest.mark.parametrize("streams,precision,coder,outliers",
+                         [(32, 8, 1, None), (32, 12, 0, None), (32, 8, 0, None), (32, 8, 1, 3)])
+def test_driver_chain_on_mvx1(gpu_ready, tmp_path, streams, precision, coder, outliers):
+    """The golden raw matrices re-encoded as MVX1: same chain, same result_hash; with a
+    narrow outlier band the file is a version-2 container whose outlier lists the driver
+    reads, uploads and applies."""
     from matvans_amd.mvx import CompressedMatrixV1, write_mvx1
     from oracle import oracle as O
     num, result_hash, vec, mats = O.read_raw_bin(os.path.join(GOLDEN, "raw-matrices_512.bin.gz"))
-    enc = [CompressedMatrixV1.encode(m, streams, precision, 0, coder=coder) for m in mats]
+    enc = [CompressedMatrixV1.encode(m, streams, precision, 0, coder=coder, outliers=outliers)
+           for m in mats]
+    if outliers is not None:
+        assert all(e.n_outliers > 0 for e in enc)
     p = tmp_path / "chain.mvx"
     write_mvx1(p, enc, vector=vec, result_hash=result_hash)
     r = run(str(p), "--iters", "2", "--quiet")
