@@ -27,19 +27,21 @@ namespace mv {
 namespace {
 
 using blk::BwdJob;
-using blk::kBwdWaves;
 using blk::kRedParts;
 
-template <int NV>
-__global__ __launch_bounds__(512) void v0_bwd_partial(BwdJob job) {
-    __shared__ blk::V0BwdSmem<NV> sm;
-    blk::v0_bwd_unit<NV>(job, blockIdx.x, blockIdx.y, gridDim.x, sm);
+// W waves (streams) per unit: 8, or 4 when the 8-wave grid would not fit on the GPU at
+// once (see fits_at_once)
+template <int NV, int W>
+__global__ __launch_bounds__(64 * W) void v0_bwd_partial(BwdJob job) {
+    __shared__ blk::V0BwdSmem<NV, W> sm;
+    blk::v0_bwd_unit<NV, W>(job, blockIdx.x, blockIdx.y, gridDim.x, sm);
 }
 
-template <int NV, int S, int MODE>
-__global__ __launch_bounds__(512) void quad_bwd_partial(BwdJob job) {
-    __shared__ blk::QuadBwdSmem<NV, S, MODE> sm;
-    blk::quad_bwd_unit<NV, S, MODE>(job, blockIdx.x, blockIdx.y, gridDim.x, sm);
+template <int NV, int S, int MODE, int W>
+__global__ __launch_bounds__(64 * W) void quad_bwd_partial(BwdJob job) {
+    __shared__ blk::QuadBwdSmem<NV, S, MODE, blk::bwd_set_words<S, MODE>(), W> sm;
+    blk::quad_bwd_unit<NV, S, MODE, blk::bwd_set_words<S, MODE>(), W>(job, blockIdx.x,
+                                                                       blockIdx.y, gridDim.x, sm);
 }
 
 __global__ __launch_bounds__(1024) void bwd_reduce(BwdJob job, uint32_t nrb) {
@@ -54,18 +56,40 @@ hipError_t launch_reduce(const BwdJob &j, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Whether `units` blocks of `kernel` (`threads` each) are resident on the current GPU at
+// once.  A grid that overflows by a fraction runs its last units in a nearly idle second
+// round; units of 4 waves (half the LDS, twice the units) then fit in one round.  Every
+// wave computes the same either way: the results are identical.
+template <class K>
+bool fits_at_once(K kernel, int threads, uint32_t units) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess)
+        return true;
+    return units <= uint32_t(cus) * uint32_t(per);
+}
+
 template <int NV>
 hipError_t bwd_nv(const BwdJob &j, hipStream_t s) {
-    dim3 grid((j.rows + 63) / 64, 32 / kBwdWaves), block(512);
-    hipLaunchKernelGGL((v0_bwd_partial<NV>), grid, block, 0, s, j);
+    const uint32_t nrb = (j.rows + 63) / 64;
+    if (fits_at_once(v0_bwd_partial<NV, 8>, 512, nrb * 4u))
+        hipLaunchKernelGGL((v0_bwd_partial<NV, 8>), dim3(nrb, 4), dim3(512), 0, s, j);
+    else
+        hipLaunchKernelGGL((v0_bwd_partial<NV, 4>), dim3(nrb, 8), dim3(256), 0, s, j);
     hipError_t e = hipGetLastError();
     return e != hipSuccess ? e : launch_reduce(j, s);
 }
 
 template <int NV, int S, int MODE>
 hipError_t quad_bwd_nv(const BwdJob &j, hipStream_t s) {
-    dim3 grid((j.rows + 63) / 64, S / kBwdWaves), block(512);
-    hipLaunchKernelGGL((quad_bwd_partial<NV, S, MODE>), grid, block, 0, s, j);
+    const uint32_t nrb = (j.rows + 63) / 64;
+    if (fits_at_once(quad_bwd_partial<NV, S, MODE, 8>, 512, nrb * uint32_t(S / 8)))
+        hipLaunchKernelGGL((quad_bwd_partial<NV, S, MODE, 8>), dim3(nrb, S / 8), dim3(512), 0,
+                           s, j);
+    else
+        hipLaunchKernelGGL((quad_bwd_partial<NV, S, MODE, 4>), dim3(nrb, S / 4), dim3(256), 0,
+                           s, j);
     hipError_t e = hipGetLastError();
     return e != hipSuccess ? e : launch_reduce(j, s);
 }

@@ -2,8 +2,8 @@
 // streams, shared by the one-matrix kernels (backward.hip) and the persistent batched
 // backward (batch.hip).  The reference has no backward pass.
 //
-// Mapping (stream-major): a unit is 8 waves = 8 consecutive streams of 64 consecutive
-// rows; a wave decodes one stream of the 64 rows, one row per lane.  A stream covers the
+// Mapping (stream-major): a unit is W waves (8, or 4) = W consecutive streams of 64
+// consecutive rows; a wave decodes one stream of the 64 rows, one row per lane.  A stream covers the
 // same columns in every row, so each 16-symbol group of the wave is a 64 x 16 int8 block
 // of W whose column sums, weighted by the rows' scaled gradients, are one
 // v_mfma_i32_16x16x64_i8 per four vectors (GroupPipe: the block and the gradient
@@ -56,11 +56,12 @@ struct BwdJob {
 // The decode table comes first, so that its lookups address it with the LDS instruction's
 // immediate offset (below 64 KiB) and one shift per symbol; the stage next, 16-byte
 // aligned for the transposed 8-byte loads and the 16-byte stage writes.
-template <int NV>
+// W: waves (streams) per unit, 8 or 4.
+template <int NV, int W = kBwdWaves>
 struct V0BwdSmem {
     alignas(16) uint32_t lut[256];
-    alignas(16) uint32_t wstage[kBwdWaves * 256];   // per wave: 64 rows x 16 int8
-    uint16_t ring[kBwdWaves * kRingWords * 64];
+    alignas(16) uint32_t wstage[W * 256];   // per wave: 64 rows x 16 int8
+    uint16_t ring[W * kRingWords * 64];
     uint16_t dummy[kDummyBytes / 2];
 };
 
@@ -72,13 +73,13 @@ constexpr int bwd_set_words() {
     return 4;
 }
 
-template <int NV, int S, int MODE = 0, int SETW = bwd_set_words<S, MODE>()>
+template <int NV, int S, int MODE = 0, int SETW = bwd_set_words<S, MODE>(), int W = kBwdWaves>
 struct QuadBwdSmem {
     alignas(16) uint32_t tab[MODE == 0 ? 256 : 4];
-    alignas(16) uint32_t wstage[kBwdWaves * 256];
-    uint32_t ring[kBwdWaves * v1::QuadLane<128 / S, SETW>::R4 * 64];
-    uint32_t rowtab[MODE == 1 ? 64 * 256 : 1];            // one packed table per row
-    uint32_t rowscratch[MODE == 1 ? kBwdWaves * 192 : 1];  // 768 bytes per wave
+    alignas(16) uint32_t wstage[W * 256];
+    uint32_t ring[W * v1::QuadLane<128 / S, SETW>::R4 * 64];
+    uint32_t rowtab[MODE == 1 ? 64 * 256 : 1];     // one packed table per row
+    uint32_t rowscratch[MODE == 1 ? W * 192 : 1];  // 768 bytes per wave
     uint32_t dummy[v1::QuadLane<128 / S, SETW>::M * 64];
 };
 
@@ -254,13 +255,14 @@ struct GroupPipe {
     }
 };
 
-// .bin v0 unit: row block rb, streams 8 sg .. 8 sg + 7.  fresh (block-uniform): build the
-// table and synchronise the block first; a unit that follows one of the same matrix in
-// the same block reuses the table, and its waves go on without a barrier (each wave's
-// ring and stage are its own).
-template <int NV>
+// .bin v0 unit: row block rb, streams W sg .. W sg + W - 1 (W waves).  fresh
+// (block-uniform): build the table and synchronise the block first; a unit that follows
+// one of the same matrix in the same block reuses the table, and its waves go on without
+// a barrier (each wave's ring and stage are its own).
+template <int NV, int W = kBwdWaves>
 __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32_t sg,
-                                            uint32_t nrb, V0BwdSmem<NV> &sm, bool fresh = true) {
+                                            uint32_t nrb, V0BwdSmem<NV, W> &sm,
+                                            bool fresh = true) {
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid >> 6, lane = tid & 63;
     if (fresh && tid < 256) {   // packed table f | bias << 8 | sym << 24
@@ -268,7 +270,7 @@ __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32
         v0_table_entry(tid, J.min_value, J.G, J.cdf, J.ppf, f, sym, bias);
         sm.lut[tid] = f | (bias << 8) | (sym << 24);
     }
-    const uint32_t stream = sg * kBwdWaves + wave;   // 0..31
+    const uint32_t stream = sg * W + wave;   // 0..31
     BwdRows<NV> R;
     R.init(J, rb, lane);
 
@@ -327,11 +329,12 @@ __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32
     }
 }
 
-// Quad-coder unit (S = 32, 64 or 128 streams): row block rb, streams 8 sg .. 8 sg + 7;
+// Quad-coder unit (S = 32, 64 or 128 streams): row block rb, streams W sg .. W sg + W - 1;
 // fresh as for v0_bwd_unit.
-template <int NV, int S, int MODE = 0, int SETW = bwd_set_words<S, MODE>()>
+template <int NV, int S, int MODE = 0, int SETW = bwd_set_words<S, MODE>(), int W = kBwdWaves>
 __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint32_t sg,
-                                              uint32_t nrb, QuadBwdSmem<NV, S, MODE, SETW> &sm,
+                                              uint32_t nrb,
+                                              QuadBwdSmem<NV, S, MODE, SETW, W> &sm,
                                               bool fresh = true) {
     static_assert(MODE == 0 || S == 64, "one table per row: 64 streams");
     using QL = v1::QuadLane<128 / S, SETW>;
@@ -349,8 +352,8 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
     } else {
         __syncthreads();   // every wave of the previous unit is done with the tables
         uint8_t *scratch = reinterpret_cast<uint8_t *>(sm.rowscratch + wave * 192);
-        for (uint32_t k = 0; k < 64u / kBwdWaves; k++) {   // the same count on every wave
-            const uint32_t r = wave * (64u / kBwdWaves) + k;
+        for (uint32_t k = 0; k < 64u / W; k++) {   // the same count on every wave
+            const uint32_t r = wave * (64u / W) + k;
             uint32_t *t = sm.rowtab + r * 256u;
             build_row_table(J.freq, rb * 64u + r, J.G, rb * 64u + r < J.rows, J.min_value,
                             lane, scratch,
@@ -360,7 +363,7 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
         }
         tab = sm.rowtab + lane * 256u;
     }
-    const uint32_t stream = sg * kBwdWaves + wave;   // 0..S-1
+    const uint32_t stream = sg * W + wave;   // 0..S-1
     BwdRows<NV> R;
     R.init(J, rb, lane);
 

@@ -136,3 +136,27 @@ def test_backward_eight_vectors_llm_shapes(cuda, fmt, rows, cols):
     ref = alpha.astype(np.float64)[:, None] * ((g.astype(np.float64) * s_w[None, :]) @
                                                w.astype(np.float64))
     assert np.linalg.norm(got - ref) <= 1e-5 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("fmt", ["v0", "q32", "q64"])
+@pytest.mark.parametrize("n_vec", [1, 3])
+def test_backward_one_matrix_equals_batch_at_14336_rows(cuda, fmt, n_vec):
+    """At 14336 rows the one-matrix backward's 8-wave grid would overflow the GPU, so it
+    runs units of 4 waves; the batched backward runs 8-wave units (and, at 3 vectors, its
+    4-vector instance).  Every wave computes the same: dx bitwise equal."""
+    import torch
+    from matvans_amd.batch import BatchBackward
+    note(f"backward one-matrix vs batch 14336x4096 {fmt} n_vec={n_vec}")
+    rng = np.random.default_rng(31 + n_vec)
+    w = sample_int8(rng, (14336, 4096), 6.0)
+    cm = (mv.CompressedMatrix.encode(w) if fmt == "v0"
+          else CompressedMatrixV1.encode(w, int(fmt[1:]), 8, 0, coder=1)).to_device()
+    g = torch.from_numpy(rng.standard_normal((n_vec, 14336)).astype(np.float32)).cuda()
+    s_w = torch.from_numpy((rng.random(14336) * 0.02 + 0.001).astype(np.float32)).cuda()
+    alpha = torch.from_numpy((rng.random(n_vec) + 0.5).astype(np.float32)).cuda()
+    single = cm.backward(g, s_w, alpha).cpu().numpy()
+    b = BatchBackward([(cm, g, s_w, alpha)])
+    b.launch()
+    torch.cuda.synchronize()
+    assert b.outputs[0].cpu().numpy().tobytes() == single.tobytes()
+    b.close()
