@@ -172,17 +172,26 @@ struct RTile {
     uint32_t item, v, cb, pad;
 };
 
+// Waves (streams) per backward unit: 4.  Twice the units of the one-matrix default (8),
+// half the LDS per block: finer work for the queue and more resident blocks (decoder
+// block, q32: 91.1 -> 84.5 us at one vector, 141.3 -> 136.3 at eight).  Every wave
+// computes the same whatever the unit width, so the results stay those of the
+// one-matrix backward.
+constexpr int kBatchBwdWaves = 4;
+
 template <int NV, uint32_t MASK>
 union BwdSmem {
-    Opt<(MASK & bit(kBwdV0)) != 0, blk::V0BwdSmem<NV>> v0;
-    Opt<(MASK & bit(kBwdQ32)) != 0, blk::QuadBwdSmem<NV, 32>> q32;
-    Opt<(MASK & bit(kBwdQ64)) != 0, blk::QuadBwdSmem<NV, 64>> q64;
+    Opt<(MASK & bit(kBwdV0)) != 0, blk::V0BwdSmem<NV, kBatchBwdWaves>> v0;
+    Opt<(MASK & bit(kBwdQ32)) != 0,
+        blk::QuadBwdSmem<NV, 32, 0, blk::bwd_set_words<32, 0>(), kBatchBwdWaves>> q32;
+    Opt<(MASK & bit(kBwdQ64)) != 0,
+        blk::QuadBwdSmem<NV, 64, 0, blk::bwd_set_words<64, 0>(), kBatchBwdWaves>> q64;
 };
 
 // Same work queue as batch_fwd (sharded counters, static first tile, last-ticket reset);
 // a unit is one (row block, stream group) of the one-matrix backward.
 template <int NV, uint32_t MASK>
-__global__ __launch_bounds__(512) void batch_bwd(const BwdItem *__restrict__ items,
+__global__ __launch_bounds__(64 * kBatchBwdWaves) void batch_bwd(const BwdItem *__restrict__ items,
                                                  const Tile *__restrict__ tiles, uint32_t n_tiles,
                                                  uint32_t *counters, uint32_t queue) {
     __shared__ BwdSmem<NV, MASK> sm;
@@ -207,15 +216,18 @@ __global__ __launch_bounds__(512) void batch_bwd(const BwdItem *__restrict__ ite
             const uint32_t sg = unit / nrb, rb = unit % nrb;
             const bool fresh = u == 0;
             if constexpr ((MASK & bit(kBwdV0)) != 0) {
-                if (path == kBwdV0) blk::v0_bwd_unit<NV>(it.job, rb, sg, nrb, sm.v0, fresh);
+                if (path == kBwdV0)
+                    blk::v0_bwd_unit<NV, kBatchBwdWaves>(it.job, rb, sg, nrb, sm.v0, fresh);
             }
             if constexpr ((MASK & bit(kBwdQ32)) != 0) {
                 if (path == kBwdQ32)
-                    blk::quad_bwd_unit<NV, 32>(it.job, rb, sg, nrb, sm.q32, fresh);
+                    blk::quad_bwd_unit<NV, 32, 0, blk::bwd_set_words<32, 0>(), kBatchBwdWaves>(
+                        it.job, rb, sg, nrb, sm.q32, fresh);
             }
             if constexpr ((MASK & bit(kBwdQ64)) != 0) {
                 if (path == kBwdQ64)
-                    blk::quad_bwd_unit<NV, 64>(it.job, rb, sg, nrb, sm.q64, fresh);
+                    blk::quad_bwd_unit<NV, 64, 0, blk::bwd_set_words<64, 0>(), kBatchBwdWaves>(
+                        it.job, rb, sg, nrb, sm.q64, fresh);
             }
         }
         __syncthreads();   // the next tile rebuilds the table
@@ -371,7 +383,7 @@ hipError_t set_max_bpc(matvans_batch *p) {
     int nb = 0;
     hipError_t e = p->backward
         ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-              &nb, reinterpret_cast<const void *>(bwd_kernel(p)), 512, 0)
+              &nb, reinterpret_cast<const void *>(bwd_kernel(p)), 64 * mv::kBatchBwdWaves, 0)
         : hipOccupancyMaxActiveBlocksPerMultiprocessor(
               &nb, reinterpret_cast<const void *>(fwd_kernel(p)), 256, 0);
     p->max_bpc = e == hipSuccess && nb > 0 ? uint32_t(nb) : 1u;
@@ -523,7 +535,7 @@ extern "C" int matvans_batch_create_backward(const matvans_batch_bwd_item *items
         mv::BwdItem b{};
         const bool v0 = it.format == MATVANS_BATCH_V0;
         b.path = v0 ? mv::kBwdV0 : it.streams == 32 ? mv::kBwdQ32 : mv::kBwdQ64;
-        b.sgs = (v0 ? 32u : it.streams) / mv::blk::kBwdWaves;
+        b.sgs = (v0 ? 32u : it.streams) / uint32_t(mv::kBatchBwdWaves);
         b.nrb = (it.rows + 63) / 64;
         b.units = b.nrb * b.sgs;
         b.job = mv::blk::BwdJob{p->d_partial + off, it.dx, it.g, it.s_w, it.alpha,
@@ -604,7 +616,8 @@ extern "C" int matvans_batch_launch(const matvans_batch *p, void *stream) {
     const uint32_t queue = p->cfg.schedule == 0 ? 1u : 0u;
     auto s = reinterpret_cast<hipStream_t>(stream);
     if (p->backward) {
-        hipLaunchKernelGGL(bwd_kernel(p), dim3(grid), dim3(512), 0, s, p->d_bitems, p->d_tiles,
+        hipLaunchKernelGGL(bwd_kernel(p), dim3(grid), dim3(64 * mv::kBatchBwdWaves), 0, s,
+                           p->d_bitems, p->d_tiles,
                            p->n_tiles, p->d_counters, queue);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, "backward batch launch");
