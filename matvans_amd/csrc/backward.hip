@@ -138,6 +138,7 @@ hipError_t launch_v0_backward(const V0BwdArgs &a, hipStream_t s) {
     BwdJob j{a.partial, a.dx, a.g, a.s_w, a.alpha, a.g_stride, a.n_vec, a.rows, a.cols,
              a.cursors, a.payload, a.payload_size, int32_t(a.min_value), uint32_t(a.G), a.cdf,
              a.ppf, nullptr};
+    if (!bwd_partials_fit(a.rows, a.cols)) return hipErrorInvalidValue;
     switch (a.n_vec) {
         case 1: return bwd_nv<1>(j, s);
         case 2: return bwd_nv<2>(j, s);
@@ -156,6 +157,7 @@ hipError_t launch_quad_backward(const QuadBwdArgs &a, hipStream_t s) {
              a.g_stride, a.n_vec,   a.rows,    a.cols,          a.cursors,
              a.payload,  a.payload_size, a.min_value, a.G,      nullptr,
              nullptr,    a.lut,     a.freq,    a.out_col_start, a.out_col_rec};
+    if (!bwd_partials_fit(a.rows, a.cols)) return hipErrorInvalidValue;
     if (a.table_mode == 1) return a.streams == 64 ? quad_bwd_s<64, 1>(j, s) : hipErrorInvalidValue;
     if (a.streams == 32) return quad_bwd_s<32>(j, s);
     if (a.streams == 64) return quad_bwd_s<64>(j, s);

@@ -29,6 +29,7 @@
 #include "blk_b.h"
 #include "blk_bwd.h"
 #include "blk_v1.h"
+#include "launch.h"
 #include "matvans_internal.h"
 #include "v0_stream.h"
 
@@ -490,6 +491,9 @@ int check_bwd_item(const matvans_batch_bwd_item &it, uint32_t k) {
         return fail(MATVANS_E_INVALID, "item %u: null device pointer", k);
     if (it.n_vec < 1 || it.n_vec > 8) return fail(MATVANS_E_INVALID, "item %u: n_vec %u", k, it.n_vec);
     if (!it.rows || !it.cols) return fail(MATVANS_E_SHAPE, "item %u: empty matrix", k);
+    if (!mv::bwd_partials_fit(it.rows, it.cols))
+        return fail(MATVANS_E_SHAPE, "item %u: %u x %u: partial sums beyond 4 GiB", k, it.rows,
+                    it.cols);
     if (it.n_vec > 1 && it.g_stride < it.rows)
         return fail(MATVANS_E_INVALID, "item %u: g_stride < rows", k);
     if (reinterpret_cast<uintptr_t>(it.payload) % 16)

@@ -223,13 +223,13 @@ template <int NV>
 struct GroupPipe {
     static constexpr int NA = GradDigits<NV>::NA;
     mv_i32x4 d[NA];
-    uint32_t loff[NA];   // (4 h + k) * plane + i for lane 16 k + i
+    uint32_t loff[NA];   // byte offset 4 ((4 h + k) * plane + i) for lane 16 k + i
     bool on[NA];         // 4 h + k < n_vec
     __device__ __forceinline__ void init(uint32_t lane, size_t plane, uint32_t n_vec) {
         const uint32_t i = lane & 15u, k = lane >> 4;
 #pragma unroll
         for (int h = 0; h < NA; h++) {
-            loff[h] = uint32_t((4u * h + k) * plane) + i;
+            loff[h] = uint32_t(((4u * h + k) * plane + i) * sizeof(float));
             on[h] = 4u * h + k < n_vec;
         }
     }
@@ -250,7 +250,7 @@ struct GroupPipe {
             const int32_t hi = int32_t(uint32_t(d[h][0]) << 8) + d[h][1];
             const int32_t lo = int32_t(uint32_t(d[h][2]) << 8) + d[h][3];
             const float t = fmaf(float(hi), gd.uh[h], float(lo) * gd.ul[h]);
-            if (on[h]) out[loff[h]] = t;
+            if (on[h]) *reinterpret_cast<float *>(reinterpret_cast<char *>(out) + loff[h]) = t;
         }
     }
 };
@@ -264,7 +264,8 @@ __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32
                                             uint32_t nrb, V0BwdSmem<NV, W> &sm,
                                             bool fresh = true) {
     const uint32_t tid = threadIdx.x;
-    const uint32_t wave = tid >> 6, lane = tid & 63;
+    // the wave index is the same on every lane: scalar, so are the stream and the output
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     if (fresh && tid < 256) {   // packed table f | bias << 8 | sym << 24
         uint32_t f, sym, bias;
         v0_table_entry(tid, J.min_value, J.G, J.cdf, J.ppf, f, sym, bias);
@@ -299,7 +300,7 @@ __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32
     gd.init(R.gs, lane, stage, tr_off);
     GroupPipe<NV> pipe;
     pipe.init(lane, plane, J.n_vec);
-    float *prev = nullptr;   // group t - 1's output, its B operand in b
+    float *prev = nullptr;   // group t - 1's output (wave-uniform), its B operand in b
     mv_i32x4 b = {0, 0, 0, 0};
     auto group = [&](uint32_t t) {
         uint32_t wbefore = st.waddr;
@@ -341,7 +342,8 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
     constexpr int GPS = SETW / 4;   // groups per set period
     static_assert(GPS == 1 || GPS == 2, "set period of one or two groups");
     const uint32_t tid = threadIdx.x;
-    const uint32_t wave = tid >> 6, lane = tid & 63;
+    // the wave index is the same on every lane: scalar, so are the stream and the output
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     // packed table T[q] = f | r << 8 | sym << 24: from the A / B tables (A = f | sym << 24,
     // B = r), or one per row of the unit, lane l reading row l's
     const uint32_t *tab;
@@ -384,7 +386,7 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
     gd.init(R.gs, lane, stage, tr_off);
     GroupPipe<NV> pipe;
     pipe.init(lane, plane, J.n_vec);
-    float *prev = nullptr;   // group t - 1's output, its B operand in b
+    float *prev = nullptr;   // group t - 1's output (wave-uniform), its B operand in b
     mv_i32x4 b = {0, 0, 0, 0};
     auto group = [&](uint32_t t) {
         uint32_t P[4];

@@ -292,6 +292,8 @@ extern "C" int matvans_decompress_backward(float *dx, const float *g, uint32_t n
     if (n_vec < 1 || n_vec > 8) return fail(MATVANS_E_INVALID, "n_vec %u outside 1..8", n_vec);
     if (rows == 0 || cols == 0 || cols % 512)
         return fail(MATVANS_E_SHAPE, "shape %u x %u (cols must be a multiple of 512)", rows, cols);
+    if (!mv::bwd_partials_fit(rows, cols))
+        return fail(MATVANS_E_SHAPE, "%u x %u: partial sums beyond 4 GiB", rows, cols);
     if (n_vec > 1 && g_stride < rows) return fail(MATVANS_E_INVALID, "g_stride %u < rows", g_stride);
     if (!aligned(payload, 16) || !aligned(workspace, 16) || !aligned(dx, 4))
         return fail(MATVANS_E_ALIGN, "payload and workspace must be 16-byte aligned");
@@ -322,6 +324,8 @@ int check_v1_backward(const matvans_v1_dev_matrix *m, const float *dx, const flo
     if (m->rows == 0 || m->cols == 0 || m->cols % (16 * m->streams))
         return fail(MATVANS_E_SHAPE, "shape %u x %u (cols must be a multiple of %u)", m->rows,
                     m->cols, 16 * m->streams);
+    if (!mv::bwd_partials_fit(m->rows, m->cols))
+        return fail(MATVANS_E_SHAPE, "%u x %u: partial sums beyond 4 GiB", m->rows, m->cols);
     if (n_vec > 1 && g_stride < m->rows)
         return fail(MATVANS_E_INVALID, "g_stride %u < rows", g_stride);
     if (!aligned(m->payload, 16) || !aligned(dx, 4))

@@ -73,6 +73,12 @@ struct V0BwdArgs {
 };
 
 size_t v0_bwd_workspace_bytes(uint32_t rows, uint32_t cols, uint32_t n_vec);
+// Whether the partial sums of a rows x cols matrix fit the 32-bit byte offsets the
+// backward's group store uses: eight vector planes of ceil(rows / 64) x cols floats
+// below 4 GiB.
+inline bool bwd_partials_fit(uint32_t rows, uint32_t cols) {
+    return 8ull * 4ull * ((rows + 63ull) / 64ull) * cols < (1ull << 32);
+}
 hipError_t launch_v0_backward(const V0BwdArgs &a, hipStream_t s);
 
 struct QuadBwdArgs {

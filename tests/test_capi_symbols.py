@@ -72,3 +72,21 @@ def test_last_error_is_per_thread(tmp_path):
         assert st != 0
         assert f"missing_{me}.bin" in msg and f"missing_{other}.bin" not in msg
     assert got["clean"] == ""
+
+
+def test_backward_refuses_partial_sums_beyond_4gib():
+    """The backward's group store addresses the partial sums with 32-bit byte offsets; a
+    matrix whose eight planes of 64-row partial sums would pass 4 GiB is refused with a
+    shape error by the argument checks, before anything reaches the GPU (the pointers
+    below are never dereferenced)."""
+    L = matvans_amd.lib()
+    fake = ctypes.c_void_p(256)
+    rows, cols = 1 << 24, 4096          # 8 x 4 B x 2^18 row blocks x 4096 = 2^35 bytes
+    m = capi.V1DevMatrix(rows=rows, cols=cols, streams=32, precision=8, table_mode=0, coder=1,
+                         payload_size=1 << 20, cursors=fake, lut=fake, payload=fake)
+    st = L.matvans_v1_decompress_backward_noalloc(ctypes.byref(m), fake, fake, 1, rows, fake,
+                                                  fake, None)
+    assert st == 2 and "4 GiB" in capi.last_error()     # MATVANS_E_SHAPE
+    st = L.matvans_decompress_backward(fake, fake, 1, rows, fake, fake, rows, cols, fake, 0, 8,
+                                       fake, fake, fake, 1 << 20, fake, 1 << 40, None)
+    assert st == 2 and "4 GiB" in capi.last_error()
