@@ -77,6 +77,10 @@ FORMATS = {
     "q64-rowcdf": (64, 8, 1, 1),
     "q32x": (32, 8, 0, 1, "auto"),
     "q64x": (64, 8, 0, 1, "auto"),
+    "l32": (32, 8, 0, 2),
+    "l64": (64, 8, 0, 2),
+    "l32x": (32, 8, 0, 2, "auto"),
+    "l64x": (64, 8, 0, 2, "auto"),
     "b-rows": "rows",
     "b-stripes": "stripes",
 }
@@ -94,11 +98,18 @@ FORMAT_TEXT = {
     "q32x": "MVX1 quad coder: 32 streams/row, global 8-bit CDF over a band, entries beyond "
             "it coded as 0 and listed per row (outlier split)",
     "q64x": "MVX1 quad coder: 64 streams/row, global 8-bit CDF over a band, outlier split",
+    "l32": "MVX1 coder 2: the q32 streams back to back behind 16-bit offsets (no per-stream "
+           "padding)",
+    "l64": "MVX1 coder 2: the q64 streams back to back behind 16-bit offsets",
+    "l32x": "MVX1 coder 2: 32 streams/row back to back, outlier split",
+    "l64x": "MVX1 coder 2: 64 streams/row back to back, outlier split",
     "b-rows": "family B (buffered) ans-warp-rows: quad rANS, 7-bit CDF, 32 coders/row",
     "b-stripes": "family B (buffered) ans-warp-stripes: quad rANS, 7-bit CDF, 32 rows/warp",
 }
-BACKWARD_FORMATS = ("v0", "q32", "q64", "q64-rowcdf", "q32x", "q64x")
-DEFAULT_FORMATS = ["v0", "q32", "q64", "q128", "q32x", "q64x", "q64-rowcdf", "s128-p8", "s64-p8",
+BACKWARD_FORMATS = ("v0", "q32", "q64", "q64-rowcdf", "q32x", "q64x", "l32", "l64", "l32x",
+                    "l64x")
+DEFAULT_FORMATS = ["v0", "q32", "q64", "q128", "q32x", "q64x", "l32", "l64", "l32x", "l64x",
+                   "q64-rowcdf", "s128-p8", "s64-p8",
                    "s64-p12",
                    "s64-p8-rowcdf",
                    "s32-p12", "b-rows"]
@@ -221,8 +232,8 @@ def _v0_copy(c):
                                c.ppf_data, c.payload)
 
 
-def bench_block(dev, steps, warmup, scale=1.0, nvecs=(1, 4, 8), fmts=("v0", "q32", "q64"),
-                replays=1):
+def bench_block(dev, steps, warmup, scale=1.0, nvecs=(1, 4, 8),
+                fmts=("v0", "q32", "q64", "l32"), replays=1):
     """One decoder block (q, k, v, o, gate, up, down; sigma=6 int8) with 1, 4 and 8 vectors
     in one persistent launch per step, per encoding; the tile size, tile order and grid
     are autotuned per (encoding, n_vec), and the fastest encoding (the streams-per-row
@@ -247,8 +258,9 @@ def bench_block(dev, steps, warmup, scale=1.0, nvecs=(1, 4, 8), fmts=("v0", "q32
             stored = sum(c.payload_size * 2 + 4 * c.rows + c.G + 1 + 256 for c in enc)
             make = _v0_copy
         else:
-            S = 32 if fmt == "q32" else 64
-            enc = [CompressedMatrixV1.encode(m, S, 8, 0, coder=1) for m in mats]
+            S = int(fmt[1:])
+            enc = [CompressedMatrixV1.encode(m, S, 8, 0, coder=2 if fmt[0] == "l" else 1)
+                   for m in mats]
             stored = sum(c.stored_bytes for c in enc)
             make = _v1_copy
         n_copies = min(16, max(2, -(-2 * MALL_BYTES // stored)))

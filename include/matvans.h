@@ -262,6 +262,11 @@ int matvans_encode_rows(const int8_t *mat, uint32_t rows, uint32_t cols, int8_t 
  *              buffered/encoder.py:198-237 with 8-bit quantiles and one stream per coder.
  *              Words are u32 (two u16 of the payload): word j of stream l at u16 index
  *              cursor + 256*(j/(128/S)) + 2*((128/S)*l + j%(128/S)).
+ *              2: the quad coder's streams (P = 8, table_mode 0) stored back to back: a
+ *              row at u16 index cursor (a multiple of 8) is S u16 end offsets e[l] (the
+ *              u32 words of streams 0..l), then the streams' u32 words (stream l: words
+ *              e[l-1] .. e[l]-1 after the offsets, e[-1] = 0), zero words to a multiple
+ *              of four.  No per-stream padding; a row holds at most 65535 stream words.
  * S = 32, P = 8, table_mode 0, coder 0 encodes exactly as the reference generator.
  * ----------------------------------------------------------------------------*/
 typedef struct {
@@ -270,7 +275,7 @@ typedef struct {
     uint32_t streams, precision, table_mode, coder;
     int32_t min_value;          /* symbols min_value .. min_value + G - 1 */
     uint32_t G;
-    uint32_t payload_size;      /* u16 words, a multiple of 256 */
+    uint32_t payload_size;      /* u16 words, a multiple of 256 (coder 2: of 8) */
     const uint32_t *cursors;    /* [rows] */
     const uint16_t *cdf;        /* table_mode 0: [G+1] */
     const uint8_t *freq;        /* table_mode 1: [rows][G] */
@@ -473,7 +478,9 @@ enum {
     MATVANS_BATCH_QUAD = 1,      /* MVX1 quad coder, 32, 64 or 128 streams: cursors, lut,
                                     payload (batched backward: 32 or 64) */
     MATVANS_BATCH_B_ROWS = 2,    /* family-B ans-warp-rows */
-    MATVANS_BATCH_B_STRIPES = 3  /* family-B ans-warp-stripes */
+    MATVANS_BATCH_B_STRIPES = 3, /* family-B ans-warp-stripes */
+    MATVANS_BATCH_QUAD_LANES = 4 /* MVX1 coder 2 (quad streams back to back), 32 or 64
+                                    streams: the fields of MATVANS_BATCH_QUAD */
 };
 
 typedef struct {
@@ -526,7 +533,7 @@ int matvans_batch_create(const matvans_batch_item *items, uint32_t n, matvans_ba
  * The plan owns the partial-sum workspace.  .bin v0 and quad (32 / 64 streams) items.
  * No reference counterpart. */
 typedef struct {
-    uint32_t format;             /* MATVANS_BATCH_V0 or MATVANS_BATCH_QUAD */
+    uint32_t format;             /* MATVANS_BATCH_V0, MATVANS_BATCH_QUAD or _QUAD_LANES */
     uint32_t rows, cols;
     uint32_t n_vec, g_stride;    /* 1..8 gradient vectors, g_stride floats apart */
     const float *g;              /* [n_vec][g_stride] */

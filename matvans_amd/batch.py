@@ -14,7 +14,19 @@ from .mvx import CompressedMatrixV1
 
 __all__ = ["Batch", "BatchBackward", "tune"]
 
-V0, QUAD, B_ROWS, B_STRIPES = 0, 1, 2, 3
+V0, QUAD, B_ROWS, B_STRIPES, QUAD_LANES = 0, 1, 2, 3, 4
+
+
+def _quad_format(m):
+    """Batch format of an MVX1 matrix: the quad coder with one table per matrix (coder 1,
+    or coder 2 with 32 or 64 streams), else None."""
+    if not isinstance(m, CompressedMatrixV1) or m.table_mode != 0:
+        return None
+    if m.coder == 1:
+        return QUAD
+    if m.coder == 2 and m.streams in (32, 64):
+        return QUAD_LANES
+    return None
 
 
 def _p(t):
@@ -24,7 +36,8 @@ def _p(t):
 class Batch:
     """entries: list of (matrix, vecs), (matrix, vecs, scale) or (matrix, vecs, scale,
     int32) where matrix is a CompressedMatrix (.bin v0), a CompressedMatrixV1 with
-    coder=1, one table per matrix and 32, 64 or 128 streams, or a (BufferedMatrices, k)
+    coder=1, one table per matrix and 32, 64 or 128 streams (coder=2: 32 or 64 streams),
+    or a (BufferedMatrices, k)
     pair (a ValueError names any other entry); vecs is an int8 device
     tensor [n_vec, cols] (or [cols]).  All matrices must be on the device.
     ``outputs[i]`` is the int32 [n_vec, rows] result of entry i (None when int32 is
@@ -52,8 +65,7 @@ class Batch:
                     raise ValueError(f"batch entry {i}: matrices are not on the device")
                 mdev, mcols = bm.d["compressed"].device, bm.shapes[k][1]
             else:
-                if not (isinstance(m, CompressedMatrix) or
-                        (isinstance(m, CompressedMatrixV1) and m.coder == 1 and m.table_mode == 0)):
+                if not (isinstance(m, CompressedMatrix) or _quad_format(m) is not None):
                     raise ValueError("batch entries: .bin v0, MVX1 quad (one table per "
                                      "matrix) or family-B matrices")
                 if getattr(m, "d_payload", None) is None:
@@ -95,8 +107,8 @@ class Batch:
                     it.format = V0
                     it.min_value, it.G = m.min_value, m.G
                     it.cdf, it.ppf = _p(m.d_cdf_data), _p(m.d_ppf_data)
-                elif isinstance(m, CompressedMatrixV1) and m.coder == 1 and m.table_mode == 0:
-                    it.format, it.streams, it.lut = QUAD, m.streams, _p(m.d_lut)
+                elif _quad_format(m) is not None:
+                    it.format, it.streams, it.lut = _quad_format(m), m.streams, _p(m.d_lut)
                     if m.d_out is not None:
                         it.out_start, it.out_rec = _p(m.d_out[0]), _p(m.d_out[1])
                 else:
@@ -171,8 +183,7 @@ class BatchBackward(Batch):
         self.outputs, self.out8 = [], []
         arr = (BatchBwdItem * len(entries))()
         for i, (m, g, s_w, alpha) in enumerate(entries):
-            if not (isinstance(m, CompressedMatrix) or
-                    (isinstance(m, CompressedMatrixV1) and m.coder == 1 and m.table_mode == 0)):
+            if not (isinstance(m, CompressedMatrix) or _quad_format(m) is not None):
                 raise ValueError("backward batch entries: .bin v0 or MVX1 quad matrices "
                                  "(one table per matrix)")
             if getattr(m, "d_payload", None) is None:
@@ -193,8 +204,8 @@ class BatchBackward(Batch):
                 it.format = V0
                 it.min_value, it.G = m.min_value, m.G
                 it.cdf, it.ppf = _p(m.d_cdf_data), _p(m.d_ppf_data)
-            elif isinstance(m, CompressedMatrixV1) and m.coder == 1 and m.table_mode == 0:
-                it.format, it.streams, it.lut = QUAD, m.streams, _p(m.d_lut)
+            elif _quad_format(m) is not None:
+                it.format, it.streams, it.lut = _quad_format(m), m.streams, _p(m.d_lut)
                 if m.d_out is not None:
                     it.out_col_start, it.out_col_rec = _p(m.d_out[2]), _p(m.d_out[3])
             else:
@@ -224,11 +235,12 @@ def tune(matrices, vecs, encodings=("q32", "q64", "q128", "v0"), reps=5, stream=
     for enc in encodings:
         if enc == "v0":
             mats = [CompressedMatrix.encode(m).to_device(dev) for m in matrices]
-        elif enc in ("q32", "q64", "q128"):
+        elif enc in ("q32", "q64", "q128", "l32", "l64"):
             S = int(enc[1:])
             if any(m.shape[1] % (16 * S) for m in matrices):
                 continue
-            mats = [CompressedMatrixV1.encode(m, S, 8, 0, coder=1).to_device(dev)
+            coder = 2 if enc[0] == "l" else 1
+            mats = [CompressedMatrixV1.encode(m, S, 8, 0, coder=coder).to_device(dev)
                     for m in matrices]
         else:
             raise ValueError(f"unknown encoding {enc}")

@@ -37,11 +37,12 @@ __global__ __launch_bounds__(64 * W) void v0_bwd_partial(BwdJob job) {
     blk::v0_bwd_unit<NV, W>(job, blockIdx.x, blockIdx.y, gridDim.x, sm);
 }
 
-template <int NV, int S, int MODE, int W>
+// LAY: 0 for coder 1 (granule layout), 1 for coder 2 (streams back to back)
+template <int NV, int S, int MODE, int W, int LAY = 0>
 __global__ __launch_bounds__(64 * W) void quad_bwd_partial(BwdJob job) {
     __shared__ blk::QuadBwdSmem<NV, S, MODE, blk::bwd_set_words<S, MODE>(), W> sm;
-    blk::quad_bwd_unit<NV, S, MODE, blk::bwd_set_words<S, MODE>(), W>(job, blockIdx.x,
-                                                                       blockIdx.y, gridDim.x, sm);
+    blk::quad_bwd_unit<NV, S, MODE, blk::bwd_set_words<S, MODE>(), W, LAY>(
+        job, blockIdx.x, blockIdx.y, gridDim.x, sm);
 }
 
 __global__ __launch_bounds__(1024) void bwd_reduce(BwdJob job, uint32_t nrb) {
@@ -81,30 +82,35 @@ hipError_t bwd_nv(const BwdJob &j, hipStream_t s) {
     return e != hipSuccess ? e : launch_reduce(j, s);
 }
 
-template <int NV, int S, int MODE>
+// One table per row (MODE 1) keeps 8-wave units: a unit builds its 64 rows' tables
+// whatever its wave count, so 4-wave units would build every table twice and hold as
+// much LDS per block (4096x14336: 53 us with 8-wave units in two rounds, 118 us with
+// 4-wave units).
+template <int NV, int S, int MODE, int LAY>
 hipError_t quad_bwd_nv(const BwdJob &j, hipStream_t s) {
     const uint32_t nrb = (j.rows + 63) / 64;
-    if (fits_at_once(quad_bwd_partial<NV, S, MODE, 8>, 512, nrb * uint32_t(S / 8)))
-        hipLaunchKernelGGL((quad_bwd_partial<NV, S, MODE, 8>), dim3(nrb, S / 8), dim3(512), 0,
-                           s, j);
+    if (MODE == 1 ||
+        fits_at_once(quad_bwd_partial<NV, S, MODE, 8, LAY>, 512, nrb * uint32_t(S / 8)))
+        hipLaunchKernelGGL((quad_bwd_partial<NV, S, MODE, 8, LAY>), dim3(nrb, S / 8), dim3(512),
+                           0, s, j);
     else
-        hipLaunchKernelGGL((quad_bwd_partial<NV, S, MODE, 4>), dim3(nrb, S / 4), dim3(256), 0,
-                           s, j);
+        hipLaunchKernelGGL((quad_bwd_partial<NV, S, MODE, 4, LAY>), dim3(nrb, S / 4), dim3(256),
+                           0, s, j);
     hipError_t e = hipGetLastError();
     return e != hipSuccess ? e : launch_reduce(j, s);
 }
 
-template <int S, int MODE = 0>
+template <int S, int MODE = 0, int LAY = 0>
 hipError_t quad_bwd_s(const BwdJob &j, hipStream_t s) {
     switch (j.n_vec) {
-        case 1: return quad_bwd_nv<1, S, MODE>(j, s);
-        case 2: return quad_bwd_nv<2, S, MODE>(j, s);
-        case 3: return quad_bwd_nv<3, S, MODE>(j, s);
-        case 4: return quad_bwd_nv<4, S, MODE>(j, s);
-        case 5: return quad_bwd_nv<5, S, MODE>(j, s);
-        case 6: return quad_bwd_nv<6, S, MODE>(j, s);
-        case 7: return quad_bwd_nv<7, S, MODE>(j, s);
-        case 8: return quad_bwd_nv<8, S, MODE>(j, s);
+        case 1: return quad_bwd_nv<1, S, MODE, LAY>(j, s);
+        case 2: return quad_bwd_nv<2, S, MODE, LAY>(j, s);
+        case 3: return quad_bwd_nv<3, S, MODE, LAY>(j, s);
+        case 4: return quad_bwd_nv<4, S, MODE, LAY>(j, s);
+        case 5: return quad_bwd_nv<5, S, MODE, LAY>(j, s);
+        case 6: return quad_bwd_nv<6, S, MODE, LAY>(j, s);
+        case 7: return quad_bwd_nv<7, S, MODE, LAY>(j, s);
+        case 8: return quad_bwd_nv<8, S, MODE, LAY>(j, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -158,6 +164,13 @@ hipError_t launch_quad_backward(const QuadBwdArgs &a, hipStream_t s) {
              a.payload,  a.payload_size, a.min_value, a.G,      nullptr,
              nullptr,    a.lut,     a.freq,    a.out_col_start, a.out_col_rec};
     if (!bwd_partials_fit(a.rows, a.cols)) return hipErrorInvalidValue;
+    if (a.coder == 2) {   // streams back to back, one table per matrix
+        if (a.table_mode != 0) return hipErrorInvalidValue;
+        if (a.streams == 32) return quad_bwd_s<32, 0, 1>(j, s);
+        if (a.streams == 64) return quad_bwd_s<64, 0, 1>(j, s);
+        if (a.streams == 128) return quad_bwd_s<128, 0, 1>(j, s);
+        return hipErrorInvalidValue;
+    }
     if (a.table_mode == 1) return a.streams == 64 ? quad_bwd_s<64, 1>(j, s) : hipErrorInvalidValue;
     if (a.streams == 32) return quad_bwd_s<32>(j, s);
     if (a.streams == 64) return quad_bwd_s<64>(j, s);

@@ -39,10 +39,11 @@ namespace mv {
 namespace {
 
 // decode path of an item: v0, quad with 32 / 64 / 128 streams, family-B rows / stripes,
-// quad with 32 streams and the products on the matrix cores (six or more vectors)
+// quad with 32 streams and the products on the matrix cores (six or more vectors), and the
+// quad streams back to back (coder 2) with 32 or 64 streams, or 32 on the matrix cores
 enum : uint32_t {
     kPathV0 = 0, kPathQ32 = 1, kPathQ64 = 2, kPathBRows = 3, kPathBStripes = 4, kPathQ128 = 5,
-    kPathQ32M = 6
+    kPathQ32M = 6, kPathL32 = 7, kPathL64 = 8, kPathL32M = 9
 };
 // from this many vectors on, quad items with 32 streams per row take the matrix-core path
 constexpr uint32_t kBatchMfmaMinVec = 6;
@@ -74,14 +75,20 @@ union BatchSmem {
     Opt<(MASK & bit(kPathQ128)) != 0, blk::QuadSmem<NV, 128>> q128;
     Opt<(MASK & (bit(kPathBRows) | bit(kPathBStripes))) != 0, blk::BSmem> b;
     Opt<(MASK & bit(kPathQ32M)) != 0, blk::QuadMfmaSmem<4>> q32m;
+    Opt<(MASK & bit(kPathL32)) != 0, blk::QuadSmem<NV, 32>> l32;
+    Opt<(MASK & bit(kPathL64)) != 0, blk::QuadSmem<NV, 64>> l64;
+    Opt<(MASK & bit(kPathL32M)) != 0, blk::QuadMfmaSmem<4>> l32m;
 };
 // kernel instances: the decode paths each one contains (a plan picks the smallest one
 // that covers its items; fewer paths leave fewer registers live)
 constexpr uint32_t kQuad = bit(kPathQ32) | bit(kPathQ64) | bit(kPathQ128);
+constexpr uint32_t kLanes = bit(kPathL32) | bit(kPathL64);
 constexpr uint32_t kMasks[] = {bit(kPathV0), bit(kPathQ32), bit(kPathQ64), bit(kPathQ128),
                                bit(kPathQ32) | bit(kPathQ64), kQuad,
                                bit(kPathBRows) | bit(kPathBStripes), bit(kPathQ32M),
-                               kQuad | bit(kPathQ32M), 0x7Fu};
+                               kQuad | bit(kPathQ32M), bit(kPathL32), bit(kPathL64), kLanes,
+                               bit(kPathL32M), kLanes | bit(kPathL32M), 0x3FFu};
+constexpr uint32_t kNumMasks = sizeof(kMasks) / sizeof(kMasks[0]);
 
 // work-queue shards: block b draws from shard b % kShards, whose tiles are t = g, g + 8,
 // g + 16, ...; each shard's counter sits on its own 128-byte line
@@ -123,6 +130,12 @@ __global__ __launch_bounds__(256) void batch_fwd(const Item *__restrict__ items,
                 if (path == kPathQ128) blk::quad_block<NV, 128>(it.v1, bid, sm.q128);
             if constexpr ((MASK & bit(kPathQ32M)) != 0)
                 if (path == kPathQ32M) blk::quad_block_mfma<NV, 4>(it.v1, bid, sm.q32m);
+            if constexpr ((MASK & bit(kPathL32)) != 0)
+                if (path == kPathL32) blk::quad_block<NV, 32, 0, 4, 1>(it.v1, bid, sm.l32);
+            if constexpr ((MASK & bit(kPathL64)) != 0)
+                if (path == kPathL64) blk::quad_block<NV, 64, 0, 4, 1>(it.v1, bid, sm.l64);
+            if constexpr ((MASK & bit(kPathL32M)) != 0)
+                if (path == kPathL32M) blk::quad_block_mfma<NV, 4, 1>(it.v1, bid, sm.l32m);
             if constexpr ((MASK & bit(kPathBRows)) != 0)
                 if (path == kPathBRows) blk::b_block<NV, 0>(it.b, bid, sm.b);
             if constexpr ((MASK & bit(kPathBStripes)) != 0)
@@ -156,9 +169,12 @@ __global__ void v0_tables(uint32_t *out, const uint8_t *cdf, const uint8_t *ppf,
 
 // ---- backward: dx = alpha * W^T (s_w * g) for every matrix of a batch (blk_bwd.h)
 
-enum : uint32_t { kBwdV0 = 0, kBwdQ32 = 1, kBwdQ64 = 2 };
+// v0, quad with 32 / 64 streams, and the quad streams back to back (coder 2), 32 / 64
+enum : uint32_t { kBwdV0 = 0, kBwdQ32 = 1, kBwdQ64 = 2, kBwdL32 = 3, kBwdL64 = 4 };
 constexpr uint32_t kBwdMasks[] = {bit(kBwdV0), bit(kBwdQ32), bit(kBwdQ64),
-                                  bit(kBwdQ32) | bit(kBwdQ64), 0x7u};
+                                  bit(kBwdQ32) | bit(kBwdQ64), bit(kBwdL32), bit(kBwdL64),
+                                  bit(kBwdL32) | bit(kBwdL64), 0x1Fu};
+constexpr uint32_t kNumBwdMasks = sizeof(kBwdMasks) / sizeof(kBwdMasks[0]);
 
 struct BwdItem {
     uint32_t path;
@@ -187,6 +203,10 @@ union BwdSmem {
         blk::QuadBwdSmem<NV, 32, 0, blk::bwd_set_words<32, 0>(), kBatchBwdWaves>> q32;
     Opt<(MASK & bit(kBwdQ64)) != 0,
         blk::QuadBwdSmem<NV, 64, 0, blk::bwd_set_words<64, 0>(), kBatchBwdWaves>> q64;
+    Opt<(MASK & bit(kBwdL32)) != 0,
+        blk::QuadBwdSmem<NV, 32, 0, blk::bwd_set_words<32, 0>(), kBatchBwdWaves>> l32;
+    Opt<(MASK & bit(kBwdL64)) != 0,
+        blk::QuadBwdSmem<NV, 64, 0, blk::bwd_set_words<64, 0>(), kBatchBwdWaves>> l64;
 };
 
 // Same work queue as batch_fwd (sharded counters, static first tile, last-ticket reset);
@@ -229,6 +249,16 @@ __global__ __launch_bounds__(64 * kBatchBwdWaves) void batch_bwd(const BwdItem *
                 if (path == kBwdQ64)
                     blk::quad_bwd_unit<NV, 64, 0, blk::bwd_set_words<64, 0>(), kBatchBwdWaves>(
                         it.job, rb, sg, nrb, sm.q64, fresh);
+            }
+            if constexpr ((MASK & bit(kBwdL32)) != 0) {
+                if (path == kBwdL32)
+                    blk::quad_bwd_unit<NV, 32, 0, blk::bwd_set_words<32, 0>(), kBatchBwdWaves, 1>(
+                        it.job, rb, sg, nrb, sm.l32, fresh);
+            }
+            if constexpr ((MASK & bit(kBwdL64)) != 0) {
+                if (path == kBwdL64)
+                    blk::quad_bwd_unit<NV, 64, 0, blk::bwd_set_words<64, 0>(), kBatchBwdWaves, 1>(
+                        it.job, rb, sg, nrb, sm.l64, fresh);
             }
         }
         __syncthreads();   // the next tile rebuilds the table
@@ -318,6 +348,10 @@ int check_item(const matvans_batch_item &it, uint32_t k) {
                 return fail(MATVANS_E_INVALID, "item %u: null v0 arrays", k);
             break;
         case MATVANS_BATCH_QUAD:
+        case MATVANS_BATCH_QUAD_LANES:
+            if (it.format == MATVANS_BATCH_QUAD_LANES && it.streams != 32 && it.streams != 64)
+                return fail(MATVANS_E_UNSUPPORTED,
+                            "item %u: coder-2 quad items need 32 or 64 streams", k);
             if (it.streams != 32 && it.streams != 64 && it.streams != 128)
                 return fail(MATVANS_E_UNSUPPORTED,
                             "item %u: quad items need 32, 64 or 128 streams", k);
@@ -363,8 +397,12 @@ FwdKernel fwd_kernel(const matvans_batch *p) {
                     mv::batch_fwd<NV, mv::kMasks[2]>, mv::batch_fwd<NV, mv::kMasks[3]>, \
                     mv::batch_fwd<NV, mv::kMasks[4]>, mv::batch_fwd<NV, mv::kMasks[5]>, \
                     mv::batch_fwd<NV, mv::kMasks[6]>, mv::batch_fwd<NV, mv::kMasks[7]>, \
-                    mv::batch_fwd<NV, mv::kMasks[8]>, mv::batch_fwd<NV, mv::kMasks[9]>}
-    static const FwdKernel table[4][10] = {MV_ROW(1), MV_ROW(2), MV_ROW(4), MV_ROW(8)};
+                    mv::batch_fwd<NV, mv::kMasks[8]>, mv::batch_fwd<NV, mv::kMasks[9]>, \
+                    mv::batch_fwd<NV, mv::kMasks[10]>, mv::batch_fwd<NV, mv::kMasks[11]>, \
+                    mv::batch_fwd<NV, mv::kMasks[12]>, mv::batch_fwd<NV, mv::kMasks[13]>, \
+                    mv::batch_fwd<NV, mv::kMasks[14]>}
+    static_assert(mv::kNumMasks == 15, "kernel table");
+    static const FwdKernel table[4][15] = {MV_ROW(1), MV_ROW(2), MV_ROW(4), MV_ROW(8)};
 #undef MV_ROW
     return table[nv_row(p->nv)][p->kernel];
 }
@@ -372,8 +410,10 @@ FwdKernel fwd_kernel(const matvans_batch *p) {
 BwdKernel bwd_kernel(const matvans_batch *p) {
 #define MV_ROW(NV) {mv::batch_bwd<NV, mv::kBwdMasks[0]>, mv::batch_bwd<NV, mv::kBwdMasks[1]>, \
                     mv::batch_bwd<NV, mv::kBwdMasks[2]>, mv::batch_bwd<NV, mv::kBwdMasks[3]>, \
-                    mv::batch_bwd<NV, mv::kBwdMasks[4]>}
-    static const BwdKernel table[4][5] = {MV_ROW(1), MV_ROW(2), MV_ROW(4), MV_ROW(8)};
+                    mv::batch_bwd<NV, mv::kBwdMasks[4]>, mv::batch_bwd<NV, mv::kBwdMasks[5]>, \
+                    mv::batch_bwd<NV, mv::kBwdMasks[6]>, mv::batch_bwd<NV, mv::kBwdMasks[7]>}
+    static_assert(mv::kNumBwdMasks == 8, "kernel table");
+    static const BwdKernel table[4][8] = {MV_ROW(1), MV_ROW(2), MV_ROW(4), MV_ROW(8)};
 #undef MV_ROW
     return table[nv_row(p->nv)][p->kernel];
 }
@@ -422,7 +462,8 @@ extern "C" int matvans_batch_create(const matvans_batch_item *items, uint32_t n,
         const auto &it = items[k];
         mv::Item m{};
         uint32_t stride = it.n_vec == 1 && it.vec_stride < it.cols ? it.cols : it.vec_stride;
-        if (it.format == MATVANS_BATCH_V0 || it.format == MATVANS_BATCH_QUAD) {
+        const bool quad = it.format == MATVANS_BATCH_QUAD || it.format == MATVANS_BATCH_QUAD_LANES;
+        if (it.format == MATVANS_BATCH_V0 || quad) {
             const uint32_t *lut = it.lut;
             if (it.format == MATVANS_BATCH_V0) {
                 uint32_t *tab = p->d_tables + size_t(v0_i++) * 512;
@@ -433,6 +474,12 @@ extern "C" int matvans_batch_create(const matvans_batch_item *items, uint32_t n,
                 m.path = mv::kPathV0;
                 m.units = (it.rows + 7) / 8;
                 p->chain.push_back(it.cols / 32);
+            } else if (it.format == MATVANS_BATCH_QUAD_LANES) {
+                m.path = it.streams == 32 ? (it.n_vec >= mv::kBatchMfmaMinVec ? mv::kPathL32M
+                                                                               : mv::kPathL32)
+                                          : mv::kPathL64;
+                m.units = (it.rows + 256u / it.streams - 1) / (256u / it.streams);
+                p->chain.push_back(it.cols / it.streams / 4);
             } else {
                 m.path = it.streams == 32 ? (it.n_vec >= mv::kBatchMfmaMinVec ? mv::kPathQ32M
                                                                                : mv::kPathQ32)
@@ -443,9 +490,8 @@ extern "C" int matvans_batch_create(const matvans_batch_item *items, uint32_t n,
             }
             m.v1 = mv::blk::V1Job{it.dst, it.vecs, stride, it.n_vec, it.rows, it.cols, it.cursors,
                                   it.min_value, it.G, lut, nullptr, it.payload, it.payload_size,
-                                  it.out8, it.scale,
-                                  it.format == MATVANS_BATCH_QUAD ? it.out_start : nullptr,
-                                  it.format == MATVANS_BATCH_QUAD ? it.out_rec : nullptr};
+                                  it.out8, it.scale, quad ? it.out_start : nullptr,
+                                  quad ? it.out_rec : nullptr};
         } else {
             const bool rows = it.format == MATVANS_BATCH_B_ROWS;
             m.path = rows ? mv::kPathBRows : mv::kPathBStripes;
@@ -501,7 +547,7 @@ int check_bwd_item(const matvans_batch_bwd_item &it, uint32_t k) {
     if (it.format == MATVANS_BATCH_V0) {
         if (it.cols % 512) return fail(MATVANS_E_SHAPE, "item %u: cols %% 512", k);
         if (!it.cdf || !it.ppf) return fail(MATVANS_E_INVALID, "item %u: null v0 tables", k);
-    } else if (it.format == MATVANS_BATCH_QUAD) {
+    } else if (it.format == MATVANS_BATCH_QUAD || it.format == MATVANS_BATCH_QUAD_LANES) {
         if (it.streams != 32 && it.streams != 64)
             return fail(MATVANS_E_UNSUPPORTED, "item %u: quad items need 32 or 64 streams", k);
         if (it.cols % (16 * it.streams)) return fail(MATVANS_E_SHAPE, "item %u: cols", k);
@@ -538,7 +584,10 @@ extern "C" int matvans_batch_create_backward(const matvans_batch_bwd_item *items
         const auto &it = items[k];
         mv::BwdItem b{};
         const bool v0 = it.format == MATVANS_BATCH_V0;
-        b.path = v0 ? mv::kBwdV0 : it.streams == 32 ? mv::kBwdQ32 : mv::kBwdQ64;
+        const bool lanes = it.format == MATVANS_BATCH_QUAD_LANES;
+        b.path = v0 ? mv::kBwdV0
+                    : it.streams == 32 ? (lanes ? mv::kBwdL32 : mv::kBwdQ32)
+                                       : (lanes ? mv::kBwdL64 : mv::kBwdQ64);
         b.sgs = (v0 ? 32u : it.streams) / uint32_t(mv::kBatchBwdWaves);
         b.nrb = (it.rows + 63) / 64;
         b.units = b.nrb * b.sgs;
@@ -568,8 +617,8 @@ extern "C" int matvans_batch_create_backward(const matvans_batch_bwd_item *items
     if (e == hipSuccess) e = hipMalloc(&p->d_counters, counter_bytes);
     if (e == hipSuccess) e = hipMemset(p->d_counters, 0, counter_bytes);
     p->max_tiles = max_tiles;
-    p->kernel = 4;
-    for (uint32_t i = 0; i < 5; i++)
+    p->kernel = mv::kNumBwdMasks - 1;
+    for (uint32_t i = 0; i < mv::kNumBwdMasks; i++)
         if ((mv::kBwdMasks[i] & p->mask) == p->mask) { p->kernel = i; break; }
     p->cfg.blocks_per_cu = 4;
     if (e == hipSuccess) e = set_max_bpc(p);

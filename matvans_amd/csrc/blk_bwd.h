@@ -332,13 +332,17 @@ __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32
 
 // Quad-coder unit (S = 32, 64 or 128 streams): row block rb, streams W sg .. W sg + W - 1;
 // fresh as for v0_bwd_unit.
-template <int NV, int S, int MODE = 0, int SETW = bwd_set_words<S, MODE>(), int W = kBwdWaves>
+// LAY 1: coder 2, the row's streams back to back (lane r finds its row's stream from the
+// row's end offsets).
+template <int NV, int S, int MODE = 0, int SETW = bwd_set_words<S, MODE>(), int W = kBwdWaves,
+          int LAY = 0>
 __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint32_t sg,
                                               uint32_t nrb,
                                               QuadBwdSmem<NV, S, MODE, SETW, W> &sm,
                                               bool fresh = true) {
     static_assert(MODE == 0 || S == 64, "one table per row: 64 streams");
-    using QL = v1::QuadLane<128 / S, SETW>;
+    static_assert(LAY == 0 || MODE == 0, "coder 2 has one table per matrix");
+    using QL = v1::QuadLane<128 / S, SETW, LAY>;
     constexpr int GPS = SETW / 4;   // groups per set period
     static_assert(GPS == 1 || GPS == 2, "set period of one or two groups");
     const uint32_t tid = threadIdx.x;
@@ -370,8 +374,15 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
     R.init(J, rb, lane);
 
     QL st;
-    st.init(J.payload, R.wbeg, R.wend, R.valid, R.rcur, R.rend, stream, sm.ring, sm.dummy, wave,
-            lane);
+    if constexpr (LAY == 0) {
+        st.init(J.payload, R.wbeg, R.wend, R.valid, R.rcur, R.rend, stream, sm.ring, sm.dummy,
+                wave, lane);
+    } else {
+        uint32_t first, n;
+        lane_stream<S>(J.payload, R.rcur, R.rend, stream, R.valid, first, n);
+        st.init_lanes(J.payload, R.wbeg, R.wend, R.valid, first, n, sm.ring, sm.dummy, wave,
+                      lane);
+    }
     st.prologue();
     if (fresh || MODE == 1) __syncthreads();
     st.start();

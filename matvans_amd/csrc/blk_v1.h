@@ -128,6 +128,24 @@ __device__ __forceinline__ void put_row(const V1Job &J, int v, uint32_t row, int
     }
 }
 
+// Coder 2 (lane-contiguous quad streams): stream sl of the row at u16 cursor rcur (row end
+// rend) from the row's u16 end offsets: its first u32 word in the payload and its word
+// count (0 when the row cannot hold its offsets).  Both loads stay inside the row.
+template <int S>
+__device__ __forceinline__ void lane_stream(const uint16_t *__restrict__ payload, uint32_t rcur,
+                                            uint32_t rend, uint32_t sl, bool valid,
+                                            uint32_t &first, uint32_t &n) {
+    first = 0;
+    n = 0;
+    if (valid && rcur + uint32_t(S) <= rend) {
+        const uint32_t e1 = payload[rcur + sl];
+        const uint32_t e0 = payload[rcur + (sl ? sl - 1u : 0u)];
+        const uint32_t b = sl ? e0 : 0u;
+        first = rcur / 2u + uint32_t(S) / 2u + b;
+        n = e1 >= b ? e1 - b : 0u;
+    }
+}
+
 // Rows a block covers for S streams per row.
 template <int S>
 constexpr uint32_t rows_per_block() { return S > 64 ? kWaves / (S / 64) : kWaves * (64 / S); }
@@ -378,15 +396,17 @@ __device__ __forceinline__ void v1_block(const V1Job &J, uint32_t bid, V1Smem<NV
 // Quad coder (MVX1 coder 1, P = 8, one table per matrix): same mapping, per-lane 64-bit
 // state; a 16-symbol group is four quads, each one v_dot4 per vector.  SETW words per
 // lane per ring set: 4 (a set per group) or 8 (a set per two groups, twice the ring).
-template <int NV, int S, int MODE = 0, int SETW = 4>
+// LAY 1: coder 2, the same streams stored back to back (QuadLane LAY 1).
+template <int NV, int S, int MODE = 0, int SETW = 4, int LAY = 0>
 __device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid,
                                            QuadSmem<NV, S, MODE, SETW> &sm) {
     static_assert(MODE == 0 || S == 64, "one table per row needs one row per wave");
+    static_assert(LAY == 0 || MODE == 0, "coder 2 has one table per matrix");
     constexpr int RPW = S < 64 ? 64 / S : 1;
     constexpr int WPR = S > 64 ? S / 64 : 1;
     constexpr int G4 = 128 / S;
     constexpr int GPS = SETW / 4;      // groups per set period
-    using QL = v1::QuadLane<G4, SETW>;
+    using QL = v1::QuadLane<G4, SETW, LAY>;
     uint32_t *ring = sm.ring;
     uint32_t *dummy = sm.dummy;
     int32_t *part = sm.part;
@@ -434,7 +454,13 @@ __device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid,
     outl.offsets(J, row, valid);
 
     QL st;
-    st.init(payload, wbeg, wend, valid, rcur, rend, sl, ring, dummy, wave, lane);
+    if constexpr (LAY == 0) {
+        st.init(payload, wbeg, wend, valid, rcur, rend, sl, ring, dummy, wave, lane);
+    } else {
+        uint32_t first, n;
+        lane_stream<S>(payload, rcur, rend, sl, valid, first, n);
+        st.init_lanes(payload, wbeg, wend, valid, first, n, ring, dummy, wave, lane);
+    }
     outl.records(J, sl);
     st.prologue();
 
@@ -490,13 +516,17 @@ __device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid,
             vload(tg + 2, vc1);
         }
     };
-    static_assert(GPS == 1 || GPS == 2, "set period of one or two groups");
+    static_assert(GPS == 1 || GPS == 2 || GPS == 4, "set period of one, two or four groups");
     auto period = [&](auto par, uint32_t t0, uint32_t ng) {
         constexpr int P = decltype(par)::value;
         uint32_t wbefore = st.waddr;
         one(std::integral_constant<int, P % 2>{}, t0);
-        if constexpr (GPS == 2) {
+        if constexpr (GPS >= 2) {
             if (ng > 1) one(std::integral_constant<int, (P + 1) % 2>{}, t0 + 1);
+        }
+        if constexpr (GPS == 4) {
+            if (ng > 2) one(std::integral_constant<int, P % 2>{}, t0 + 2);
+            if (ng > 3) one(std::integral_constant<int, (P + 1) % 2>{}, t0 + 3);
         }
         st.end_group(wbefore);
     };
@@ -583,14 +613,14 @@ struct QuadMfmaSmem {
 
 typedef int mv_i32x4 __attribute__((ext_vector_type(4)));
 
-template <int NV, int MW>
+template <int NV, int MW, int LAY = 0>
 __device__ __forceinline__ void quad_block_mfma(const V1Job &J, uint32_t bid,
                                                 QuadMfmaSmem<MW> &sm) {
     static_assert(MW == 4 || MW == 8, "4 or 8 waves");
     constexpr int S = 32;
     constexpr uint32_t kRows = 2 * MW;
     constexpr int KS = 8 / MW;             // 64-column slices per wave and group
-    using QL = v1::QuadLane<4, 4>;
+    using QL = v1::QuadLane<4, 4, LAY>;
     const uint32_t rows = J.rows, cols = J.cols, vec_stride = J.vec_stride;
     const uint32_t payload_size = J.payload_size;
     const uint32_t *__restrict__ cursors = J.cursors;
@@ -625,7 +655,13 @@ __device__ __forceinline__ void quad_block_mfma(const V1Job &J, uint32_t bid,
     outl.offsets(J, row, valid);
 
     QL st;
-    st.init(payload, wbeg, wend, valid, rcur, rend, sl, sm.ring, sm.dummy, wave, lane);
+    if constexpr (LAY == 0) {
+        st.init(payload, wbeg, wend, valid, rcur, rend, sl, sm.ring, sm.dummy, wave, lane);
+    } else {
+        uint32_t first, n;
+        lane_stream<S>(payload, rcur, rend, sl, valid, first, n);
+        st.init_lanes(payload, wbeg, wend, valid, first, n, sm.ring, sm.dummy, wave, lane);
+    }
     outl.records(J, sl);
     st.prologue();
 

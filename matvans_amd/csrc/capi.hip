@@ -133,9 +133,11 @@ extern "C" int matvans_v1_decompress_and_multiply(const matvans_v1_dev_matrix *m
                     m->precision, m->table_mode);
     if (m->table_mode == 1 && (m->streams != 64 || m->precision != 8))
         return fail(MATVANS_E_UNSUPPORTED, "one table per row needs 64 streams and 8-bit tables");
-    if (m->coder > 1) return fail(MATVANS_E_INVALID, "coder %u", m->coder);
-    if (m->coder == 1 && m->precision != 8)
+    if (m->coder > 2) return fail(MATVANS_E_INVALID, "coder %u", m->coder);
+    if (m->coder >= 1 && m->precision != 8)
         return fail(MATVANS_E_UNSUPPORTED, "the quad coder needs 8-bit probabilities");
+    if (m->coder == 2 && m->table_mode != 0)
+        return fail(MATVANS_E_UNSUPPORTED, "coder 2 has one table per matrix");
     if (m->table_mode == 0 && !m->lut) return fail(MATVANS_E_INVALID, "null decode table");
     if (m->table_mode == 1 && (!m->freq || m->G < 2 || m->G > 256))
         return fail(MATVANS_E_INVALID, "per-row tables need freq and 2 <= G <= 256");
@@ -148,7 +150,7 @@ extern "C" int matvans_v1_decompress_and_multiply(const matvans_v1_dev_matrix *m
     if (!aligned(vecs, 16) || vec_stride % 16)
         return fail(MATVANS_E_ALIGN, "vectors must be 16-byte aligned (stride %u)", vec_stride);
     if (vec_stride < m->cols) return fail(MATVANS_E_INVALID, "vec_stride %u < cols %u", vec_stride, m->cols);
-    if (m->n_outliers && (m->coder != 1 || m->table_mode != 0))
+    if (m->n_outliers && (m->coder == 0 || m->table_mode != 0))
         return fail(MATVANS_E_UNSUPPORTED, "outliers need the quad coder with one table");
     if (m->n_outliers && (!m->out_start || !m->out_rec))
         return fail(MATVANS_E_INVALID, "outliers without device offsets or records");
@@ -310,12 +312,12 @@ namespace {
 int check_v1_backward(const matvans_v1_dev_matrix *m, const float *dx, const float *g,
                       uint32_t n_vec, uint32_t g_stride, const float *s_w, const float *alpha) {
     if (!m || !dx || !g || !s_w || !alpha) return fail(MATVANS_E_INVALID, "null argument");
-    if (m->coder != 1 || m->precision != 8 || m->table_mode > 1 ||
+    if ((m->coder != 1 && m->coder != 2) || m->precision != 8 || m->table_mode > 1 ||
         (m->streams != 32 && m->streams != 64 && m->streams != 128) ||
-        (m->table_mode == 1 && m->streams != 64))
+        (m->table_mode == 1 && (m->streams != 64 || m->coder != 1)))
         return fail(MATVANS_E_UNSUPPORTED,
                     "backward needs the quad coder with 8-bit tables: one per matrix with 32, "
-                    "64 or 128 streams, or one per row with 64");
+                    "64 or 128 streams, or one per row with 64 (coder 1)");
     if (!m->cursors || !m->payload || (m->table_mode == 0 && !m->lut))
         return fail(MATVANS_E_INVALID, "null device pointer");
     if (m->table_mode == 1 && (!m->freq || m->G < 2 || m->G > 256))
@@ -343,7 +345,7 @@ mv::QuadBwdArgs v1_backward_args(const matvans_v1_dev_matrix *m, float *dx, cons
                            m->rows, m->cols, m->streams, m->cursors, m->lut, m->payload,
                            m->payload_size, m->table_mode, m->freq, int32_t(m->min_value),
                            uint32_t(m->G), m->n_outliers ? m->out_col_start : nullptr,
-                           m->n_outliers ? m->out_col_rec : nullptr};
+                           m->n_outliers ? m->out_col_rec : nullptr, m->coder};
 }
 }  // namespace
 

@@ -46,20 +46,24 @@ __global__ __launch_bounds__(256) void v1_fwd(V1Job job) {
 // ring words per lane per set of the one-matrix quad kernels: two groups per set for 32
 // streams per row (halves the ring bookkeeping; 2 waves per SIMD at 4096 rows leave LDS
 // to spare), one group per set for 64 and 128 streams (more resident waves per CU)
-template <int S>
-constexpr int quad_set_words() { return S == 32 ? 8 : 4; }
+// Coder 2 reads each lane's own run of words, 8 per set whatever the stream count (16-word
+// sets, a 64-byte line per lane and set, measured slower for 32 streams: 26.5 vs 24.6 us
+// at 4096x14336).
+template <int S, int LAY = 0>
+constexpr int quad_set_words() { return LAY ? 8 : (S == 32 ? 8 : 4); }
 
-template <int NV, int S, int MODE>
+// LAY: 0 for coder 1 (granule layout), 1 for coder 2 (streams back to back)
+template <int NV, int S, int MODE, int LAY = 0>
 __global__ __launch_bounds__(256) void v1q_fwd(V1Job job) {
-    constexpr int W = quad_set_words<S>();
+    constexpr int W = quad_set_words<S, LAY>();
     __shared__ blk::QuadSmem<NV, S, MODE, W> sm;
-    blk::quad_block<NV, S, MODE, W>(job, blockIdx.x, sm);
+    blk::quad_block<NV, S, MODE, W, LAY>(job, blockIdx.x, sm);
 }
 
-template <int NV, int MW>
+template <int NV, int MW, int LAY = 0>
 __global__ __launch_bounds__(64 * MW) void v1q_mfma(V1Job job) {
     __shared__ blk::QuadMfmaSmem<MW> sm;
-    blk::quad_block_mfma<NV, MW>(job, blockIdx.x, sm);
+    blk::quad_block_mfma<NV, MW, LAY>(job, blockIdx.x, sm);
 }
 
 V1Job job_of(const V1Args &a) {
@@ -68,11 +72,19 @@ V1Job job_of(const V1Args &a) {
                  a.payload_size, nullptr, 0.0f,        a.out_start,    a.out_rec};
 }
 
-template <int NV, int S, int MODE = 0>
+template <int NV, int S, int MODE = 0, int LAY = 0>
 hipError_t launch_quad(const V1Args &a, hipStream_t s) {
     constexpr uint32_t rows_per_block = S > 64 ? kWaves / (S / 64) : kWaves * (64 / S);
     dim3 grid((a.rows + rows_per_block - 1) / rows_per_block), block(256);
-    hipLaunchKernelGGL((v1q_fwd<NV, S, MODE>), grid, block, 0, s, job_of(a));
+    hipLaunchKernelGGL((v1q_fwd<NV, S, MODE, LAY>), grid, block, 0, s, job_of(a));
+    return hipGetLastError();
+}
+
+template <int NV, int LAY>
+hipError_t launch_mfma(const V1Args &a, hipStream_t s) {
+    constexpr int MW = MV_MFMA_WAVES;
+    dim3 grid((a.rows + 2 * MW - 1) / (2 * MW)), block(64 * MW);
+    hipLaunchKernelGGL((v1q_mfma<NV, MW, LAY>), grid, block, 0, s, job_of(a));
     return hipGetLastError();
 }
 
@@ -90,14 +102,17 @@ constexpr uint32_t kMfmaMinVec = MV_MFMA_MIN_VEC;
 
 template <int NV>
 hipError_t launch_v1_nv(const V1Args &a, hipStream_t s) {
+    if (a.coder == 2) {   // quad streams back to back: one table per matrix
+        if (a.precision != 8 || a.table_mode != 0) return hipErrorInvalidValue;
+        if (a.streams == 32 && a.n_vec >= kMfmaMinVec) return launch_mfma<NV, 1>(a, s);
+        if (a.streams == 128) return launch_quad<NV, 128, 0, 1>(a, s);
+        if (a.streams == 64) return launch_quad<NV, 64, 0, 1>(a, s);
+        return launch_quad<NV, 32, 0, 1>(a, s);
+    }
     if (a.coder == 1) {
         if (a.precision != 8) return hipErrorInvalidValue;
-        if (a.table_mode == 0 && a.streams == 32 && a.n_vec >= kMfmaMinVec) {
-            constexpr int MW = MV_MFMA_WAVES;
-            dim3 grid((a.rows + 2 * MW - 1) / (2 * MW)), block(64 * MW);
-            hipLaunchKernelGGL((v1q_mfma<NV, MW>), grid, block, 0, s, job_of(a));
-            return hipGetLastError();
-        }
+        if (a.table_mode == 0 && a.streams == 32 && a.n_vec >= kMfmaMinVec)
+            return launch_mfma<NV, 0>(a, s);
         if (a.table_mode == 1)
             return a.streams == 64 ? launch_quad<NV, 64, 1>(a, s) : hipErrorInvalidValue;
         if (a.streams == 128) return launch_quad<NV, 128>(a, s);

@@ -7,7 +7,8 @@
 //
 // Stream l of a row decodes columns 16*S*(k/16) + 16*l + k%16.  A row is K granules of
 // 512 bytes; every stream owns GW = 256/S consecutive u16 words of each granule, so word
-// j of stream l is at cursor + 256*(j/GW) + GW*l + j%GW.
+// j of stream l is at cursor + 256*(j/GW) + GW*l + j%GW.  Coder 2 keeps the quad coder's
+// streams but stores them back to back behind S u16 end offsets (no per-stream padding).
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -64,7 +65,14 @@ int quantize_counts(const uint64_t *counts, uint32_t n_sym, uint32_t total, uint
 bool valid_params(uint32_t streams, uint32_t precision, uint32_t table_mode, uint32_t coder = 0) {
     return (streams == 32 || streams == 64 || streams == 128) && (precision == 8 || precision == 12) &&
            (table_mode == 0 || (table_mode == 1 && precision == 8)) &&
-           (coder == 0 || (coder == 1 && precision == 8));
+           (coder == 0 || (coder == 1 && precision == 8) ||
+            (coder == 2 && precision == 8 && table_mode == 0));
+}
+
+// Coder 2 row: S u16 end offsets (S / 2 u32 words), the streams' u32 words back to back,
+// zero words to a multiple of four; its size in u16 words.
+uint64_t lanes_row_u16(uint32_t S, uint64_t stream_words) {
+    return 2 * ((uint64_t(S) / 2 + stream_words + 3) / 4 * 4);
 }
 
 // Quad coder (coder 1): 64-bit state, 32-bit words, four 8-bit quantiles per step (the
@@ -298,7 +306,7 @@ extern "C" int matvans_v1_encode(const int8_t *mat, uint32_t rows, uint32_t cols
     const uint32_t S = streams, GW = 256 / S, G4 = 128 / S;
     std::vector<uint32_t> kgr(rows);
     std::vector<std::vector<uint16_t>> st(coder == 0 ? size_t(rows) * S : 0);
-    std::vector<std::vector<uint32_t>> st4(coder == 1 ? size_t(rows) * S : 0);
+    std::vector<std::vector<uint32_t>> st4(coder != 0 ? size_t(rows) * S : 0);
     std::atomic<int> bad{-1};
     auto pass1 = [&](uint32_t t) {
         std::vector<uint16_t> tmp;
@@ -311,6 +319,7 @@ extern "C" int matvans_v1_encode(const int8_t *mat, uint32_t rows, uint32_t cols
                 for (uint32_t s = 0; s < G; s++) c[s + 1] = c[s] + freq[size_t(r) * G + s];
             }
             uint32_t mx = 0;
+            uint64_t sum = 0;
             for (uint32_t l = 0; l < S; l++) {
                 int e;
                 size_t len;
@@ -330,8 +339,15 @@ extern "C" int matvans_v1_encode(const int8_t *mat, uint32_t rows, uint32_t cols
                     return;
                 }
                 mx = std::max<uint32_t>(mx, uint32_t(len));
+                sum += len;
             }
-            kgr[r] = coder == 0 ? (mx + GW - 1) / GW : (mx + G4 - 1) / G4;
+            if (coder == 2 && sum > 0xFFFFu) {   // the end offsets are u16
+                bad = int(r);
+                return;
+            }
+            // coder 2: the row's size in u16 words; otherwise its granules
+            kgr[r] = coder == 0 ? (mx + GW - 1) / GW
+                   : coder == 1 ? (mx + G4 - 1) / G4 : uint32_t(lanes_row_u16(S, sum));
         }
     };
     std::vector<std::thread> th;
@@ -339,12 +355,21 @@ extern "C" int matvans_v1_encode(const int8_t *mat, uint32_t rows, uint32_t cols
     pass1(0);
     for (auto &x : th) x.join();
     th.clear();
-    if (bad >= 0) return fail(MATVANS_E_RANGE, "row %d holds a symbol the model gives no probability", int(bad));
+    if (bad >= 0) {
+        if (coder == 2) {   // a symbol without probability, or more than 65535 words
+            size_t r = size_t(bad), total = 0;
+            for (uint32_t l = 0; l < S; l++) total += st4[r * S + l].size();
+            if (total > 0xFFFFu)
+                return fail(MATVANS_E_SHAPE, "row %d: %zu stream words exceed the 16-bit "
+                            "offsets of the lane-contiguous layout", bad.load(), total);
+        }
+        return fail(MATVANS_E_RANGE, "row %d holds a symbol the model gives no probability", int(bad));
+    }
     uint64_t pos = 0;
     for (uint32_t r = 0; r < rows; r++) {
         if (pos > 0xFFFFFFFFull) return fail(MATVANS_E_CAPACITY, "payload exceeds 2^32 words");
         cursors[r] = uint32_t(pos);
-        pos += uint64_t(kgr[r]) * 256;
+        pos += coder == 2 ? uint64_t(kgr[r]) : uint64_t(kgr[r]) * 256;
     }
     if (pos > 0xFFFFFFFFull) return fail(MATVANS_E_CAPACITY, "payload exceeds 2^32 words");
     if (pos > payload_cap) return fail(MATVANS_E_CAPACITY, "payload needs %llu words, cap %llu",
@@ -352,6 +377,21 @@ extern "C" int matvans_v1_encode(const int8_t *mat, uint32_t rows, uint32_t cols
     auto pass2 = [&](uint32_t t) {
         for (uint32_t r = t; r < rows; r += n_threads) {
             uint16_t *dst = payload + cursors[r];
+            if (coder == 2) {
+                std::memset(dst, 0, size_t(kgr[r]) * 2);
+                uint32_t end = 0;
+                uint16_t *w = dst + S;   // stream words after the S u16 offsets
+                for (uint32_t l = 0; l < S; l++) {
+                    const auto &s = st4[size_t(r) * S + l];
+                    for (size_t j = 0; j < s.size(); j++, w += 2) {
+                        w[0] = uint16_t(s[j]);
+                        w[1] = uint16_t(s[j] >> 16);
+                    }
+                    end += uint32_t(s.size());
+                    dst[l] = uint16_t(end);
+                }
+                continue;
+            }
             std::memset(dst, 0, size_t(kgr[r]) * 512);
             for (uint32_t l = 0; l < S; l++) {
                 if (coder == 0) {
@@ -469,7 +509,7 @@ extern "C" int matvans_v1_read(const char *path, matvans_v1_file **out) {
         if (version == 2) {
             if (!rd.take(&m.n_outliers, 4)) return bad("truncated outlier count", k);
             if (m.n_outliers) {
-                if (m.coder != 1 || m.table_mode != 0) return bad("outliers need the quad coder", k);
+                if (m.coder == 0 || m.table_mode != 0) return bad("outliers need the quad coder", k);
                 ost_p = rd.p;
                 if (!rd.take(nullptr, (size_t(m.rows) + 1) * 4)) return bad("truncated outliers", k);
                 orec_p = rd.p;
@@ -515,13 +555,27 @@ extern "C" int matvans_v1_read(const char *path, matvans_v1_file **out) {
                 if (sum != 256) return bad("row frequencies do not sum to 256", k);
             }
         }
+        const uint32_t unit = m.coder == 2 ? 8u : 256u;   // 16-byte rows, or granules
         for (uint32_t r = 0; r < m.rows; r++) {
             uint32_t end = r + 1 < m.rows ? m.cursors[r + 1] : m.payload_size;
-            if (m.cursors[r] % 256 || end < m.cursors[r] || end > m.payload_size ||
+            if (m.cursors[r] % unit || end < m.cursors[r] || end > m.payload_size ||
                 (r == 0 && m.cursors[0] != 0))
                 return bad("bad cursors", k);
+            if (m.coder == 2) {
+                // end offsets: every stream holds at least its two state words, and the
+                // streams end within the row
+                const uint16_t *e = m.payload + m.cursors[r];
+                const uint64_t room = (uint64_t(end) - m.cursors[r]) / 2;
+                if (room < m.streams / 2) return bad("row shorter than its stream offsets", k);
+                uint32_t prev = 0;
+                for (uint32_t l = 0; l < m.streams; l++) {
+                    if (uint32_t(e[l]) < prev + 2) return bad("stream shorter than its state", k);
+                    prev = e[l];
+                }
+                if (m.streams / 2 + uint64_t(prev) > room) return bad("streams overrun the row", k);
+            }
         }
-        if (m.rows && m.payload_size % 256) return bad("payload not a whole number of granules", k);
+        if (m.rows && m.payload_size % unit) return bad("payload not a whole number of granules", k);
         v1->blobs.push_back(std::move(blob));
         v1->mats.push_back(m);
     }
@@ -560,7 +614,7 @@ extern "C" int matvans_v1_write(const char *path, uint32_t result_hash, const in
         if ((m.table_mode == 0 && !m.cdf) || (m.table_mode == 1 && !m.freq) || !m.payload ||
             (m.rows && !m.cursors))
             return fail(MATVANS_E_INVALID, "matrix %u: null table, cursors or payload", k);
-        if (m.n_outliers && (!m.out_start || !m.out_rec || m.coder != 1 || m.table_mode != 0))
+        if (m.n_outliers && (!m.out_start || !m.out_rec || m.coder == 0 || m.table_mode != 0))
             return fail(MATVANS_E_INVALID, "matrix %u: outliers need offsets, records and the "
                         "quad coder with one table", k);
     }

@@ -99,6 +99,7 @@ struct QuadBwdArgs {
     uint32_t G;
     const uint32_t *out_col_start;   // outlier split by column (table mode 0), null: none
     const uint32_t *out_col_rec;
+    uint32_t coder;          // 1: granule layout; 2: streams back to back (table mode 0)
 };
 
 hipError_t launch_quad_backward(const QuadBwdArgs &a, hipStream_t s);

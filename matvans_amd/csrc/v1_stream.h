@@ -274,23 +274,32 @@ struct Lane {
 // (R4 = 4M >= 3M + M).  A set may run past the row's last granule: those loads read the
 // next row's words inside the window or return zeros past it (buffer range check), into
 // ring slots the fit check has shown to be free, and the lane never consumes them.
-template <int G4, int SETW = 4>
+//
+// LAY 1 is the lane-contiguous layout of coder 2: the lane's stream is its own run of u32
+// words (no granules), read as 16-byte chunks from the aligned address at or below its
+// first word; that word lands in ring slot a0 = (first word) & 3, the slots before it hold
+// the previous stream's tail and are never read.  The fit rule and the bounds above hold
+// with the read index starting at a0 + 2 instead of 2 (a0 <= 3 < M).
+template <int G4, int SETW = 4, int LAY = 0>
 struct QuadLane {
+    static constexpr int CW = LAY ? 4 : G4;               // u32 words per chunk
+    static constexpr uint32_t kStride = LAY ? 16u : 512u; // bytes between a lane's chunks
     static constexpr int M = SETW;
     static constexpr int R4 = 4 * SETW;
     static constexpr uint32_t kRingWaveBytes = R4 * 64 * 4;
-    static constexpr int NG = M / G4;
+    static constexpr int NG = M / CW;
     static_assert(R4 >= 3 * M + M, "ring too small");
     static_assert(NG >= 1 && (R4 & (R4 - 1)) == 0, "set shape");
     static constexpr int kPreSets = (R4 * 3 / 4) / M;
 
     __amdgpu_buffer_rsrc_t rsrc;
-    uint32_t lane_off, K, gs, j;   // K granules in the row; gs next set to load
+    uint32_t lane_off, K, gs, j;   // K chunks of the lane (granules of the row); gs next set
+    uint32_t a0;           // LAY 1: ring slot of the state's low word
     uint32_t lo, hi;       // state
     uint32_t W;            // next unread word
     uint32_t waddr;        // LDS byte offset of that word within the ring array
     uint32_t wave_base;
-    Chunk<2 * G4> pa[NG], pb[NG];   // a set: NG chunks of 2*G4 u16 = G4 u32 words
+    Chunk<2 * CW> pa[NG], pb[NG];   // a set: NG chunks of 2*CW u16 = CW u32 words
     uint32_t sa, sb;                // their set index, or ~0 when not loaded
     uint32_t *ring, *dummy;         // dummy: M slots (M*64 words), written, never read
     uint32_t lane;
@@ -311,6 +320,31 @@ struct QuadLane {
             }
         }
         lane_off = roff + slot * (G4 * 4u);
+        a0 = 0;
+        ring = ring_array;
+        dummy = dummy_rows;
+        lane = lane_in_wave;
+        wave_base = wave * kRingWaveBytes;
+        sa = sb = 0xFFFFFFFFu;
+    }
+
+    // LAY 1: the lane's stream is n u32 words from u32 index `first` of the payload,
+    // inside the u16 window [win_beg, win_end) (win_beg a multiple of 8).
+    __device__ void init_lanes(const uint16_t *payload, uint32_t win_beg, uint32_t win_end,
+                               bool valid, uint32_t first, uint32_t n, uint32_t *ring_array,
+                               uint32_t *dummy_rows, uint32_t wave, uint32_t lane_in_wave) {
+        static_assert(LAY == 1, "lane-contiguous layout");
+        rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(payload) + win_beg, 0,
+                                                 static_cast<int>((win_end - win_beg) * 2u),
+                                                 0x00020000);
+        K = 0;
+        a0 = 0;
+        lane_off = 0;
+        if (valid && n >= 2 && 2u * first >= win_beg && 2u * (first + n) <= win_end) {
+            a0 = first & 3u;
+            K = (a0 + n + 3u) >> 2;
+            lane_off = ((first & ~3u) * 2u - win_beg) * 2u;
+        }
         ring = ring_array;
         dummy = dummy_rows;
         lane = lane_in_wave;
@@ -322,33 +356,33 @@ struct QuadLane {
         return ring[(wave_base >> 2) + w * 64 + lane];
     }
 
-    // loads set `set` (granules set*NG ..): one byte offset, the chunks at +512 apart
-    __device__ __forceinline__ void load_set(Chunk<2 * G4> (&p)[NG], uint32_t base) {
+    // loads set `set` (chunks set*NG ..): one byte offset, the chunks kStride apart
+    __device__ __forceinline__ void load_set(Chunk<2 * CW> (&p)[NG], uint32_t base) {
 #pragma unroll
-        for (int i = 0; i < NG; i++) p[i] = load_chunk<2 * G4>(rsrc, base + uint32_t(i) * 512u);
+        for (int i = 0; i < NG; i++) p[i] = load_chunk<2 * CW>(rsrc, base + uint32_t(i) * kStride);
     }
 
-    __device__ __forceinline__ void write_set(const Chunk<2 * G4> (&p)[NG], uint32_t set) {
+    __device__ __forceinline__ void write_set(const Chunk<2 * CW> (&p)[NG], uint32_t set) {
         uint32_t *q = set != 0xFFFFFFFFu ? &ring_at((set * M) & (R4 - 1)) : dummy + lane;
 #pragma unroll
         for (int i = 0; i < NG; i++)
 #pragma unroll
-            for (int w = 0; w < G4; w++) q[(i * G4 + w) * 64] = p[i].d[w];
+            for (int w = 0; w < CW; w++) q[(i * CW + w) * 64] = p[i].d[w];
     }
 
-    __device__ __forceinline__ void issue(Chunk<2 * G4> (&p)[NG], uint32_t &sp) {
+    __device__ __forceinline__ void issue(Chunk<2 * CW> (&p)[NG], uint32_t &sp) {
         const uint32_t g0 = gs * NG;
         const bool ok = g0 < K && (gs + 1) * M - j <= uint32_t(R4);
-        load_set(p, ok ? lane_off + g0 * 512u : kOOB);
+        load_set(p, ok ? lane_off + g0 * kStride : kOOB);
         sp = ok ? gs : 0xFFFFFFFFu;
         gs += ok ? 1u : 0u;
     }
 
     __device__ void prologue() {
-        Chunk<2 * G4> pre[kPreSets][NG];
+        Chunk<2 * CW> pre[kPreSets][NG];
 #pragma unroll
         for (uint32_t t = 0; t < uint32_t(kPreSets); t++)
-            load_set(pre[t], t * NG < K ? lane_off + t * NG * 512u : kOOB);
+            load_set(pre[t], t * NG < K ? lane_off + t * NG * kStride : kOOB);
 #pragma unroll
         for (uint32_t t = 0; t < uint32_t(kPreSets); t++) write_set(pre[t], t);
         const uint32_t k_sets = (K + NG - 1) / NG;
@@ -356,11 +390,12 @@ struct QuadLane {
     }
 
     __device__ void start() {
-        lo = ring_at(0);
-        hi = ring_at(1);
-        j = 2;
-        W = ring_at(2);
-        waddr = wave_base | (2u * 256u) | (lane * 4u);
+        const uint32_t a = LAY ? a0 : 0u;
+        lo = ring_at(a);
+        hi = ring_at(a + 1);
+        j = a + 2;
+        W = ring_at(a + 2);
+        waddr = wave_base | ((a + 2u) * 256u) | (lane * 4u);
         issue(pa, sa);
         issue(pb, sb);
     }

@@ -92,13 +92,15 @@ class CompressedMatrixV1:
     @classmethod
     def encode(cls, matrix, streams=64, precision=8, table_mode=0, n_threads=0, coder=0,
                outliers=None):
-        """outliers (quad coder, one table): None or False, no split; "auto", the band the
+        """coder 0: one symbol per step; 1: quad coder (P = 8); 2: quad coder with the
+        streams back to back (P = 8, one table; no per-stream padding).
+        outliers (quad coder, one table): None or False, no split; "auto", the band the
         library picks; an int, that band."""
         m = np.ascontiguousarray(np.asarray(matrix), dtype=np.int8)
         rows, cols = m.shape
         out_start = out_rec = None
         if outliers is not None and outliers is not False:
-            if coder != 1 or table_mode != 0 or precision != 8:
+            if coder not in (1, 2) or table_mode != 0 or precision != 8:
                 raise ValueError("the outlier split needs the quad coder with one 8-bit table")
             band = cls.outlier_band(m) if outliers == "auto" else int(outliers)
             if band < 127:
@@ -184,8 +186,8 @@ class CompressedMatrixV1:
 
     def backward(self, g, s_w, alpha=None, out=None, workspace=None, stream=None):
         """dx[v] = alpha[v] * W^T (s_w * g[v]) re-decoded from the quad-coder streams
-        (coder 1, 32, 64 or 128 streams), g [rows] or [n_vec, rows] fp32 (n_vec <= 8); bitwise
-        reproducible."""
+        (coder 1 or 2, 32, 64 or 128 streams), g [rows] or [n_vec, rows] fp32 (n_vec <= 8);
+        bitwise reproducible."""
         import torch
         s = self.dev_struct()
         dev = self.d_payload.device

@@ -43,6 +43,15 @@ payload; word j of stream l is at u16 index cursor + 256*(j // G4) + 2*(G4*l + j
 with G4 = 128 / S words per stream per 512-byte granule; words 0 and 1 are the low and
 high words of the initial state.
 
+Quad coder, lane-contiguous layout (coder = 2; P = 8, one table, S = 32, 64 or 128): the
+quad coder's streams, word for word as coder 1, without per-lane padding.  A row at u16
+index cursor (a multiple of 8, so 16-byte aligned) is S u16 end offsets e[0..S-1]
+(e[l] = the u32 words of streams 0..l), then the streams' words back to back (stream l:
+u32 words e[l-1] .. e[l] - 1 after the offsets, e[-1] = 0), then zero u32 words to a
+multiple of four.  Coder 1 pads every stream of a row to the longest one and to its
+granule slot (~0.3 bits per entry at 4096 x 14336, ~1 at 14336 x 4096); here a stream
+costs 16 bits of offset.
+
 Outlier split (quad coder, one table): entries with |w| > band are coded as 0 in the
 streams and listed per row as records col | (uint8)w << 24 (out_start[rows + 1] offsets,
 row-major order); W = decode(streams) + scatter(records).  Pinned by the round trip on
@@ -304,6 +313,92 @@ def encode_rows(matrix, streams, precision, min_value, cdf=None, freq=None):
     return np.asarray(cursors, np.uint32), np.concatenate(out)
 
 
+def encode_rows_quad_lanes(matrix, streams, min_value, cdf):
+    """Quad-coder encode of a matrix in the lane-contiguous layout (coder 2, one table).
+    Returns (cursors uint32[rows] in u16 units, payload uint16)."""
+    m = np.asarray(matrix, np.int8)
+    rows, cols = m.shape
+    assert streams in (32, 64, 128) and cols % (16 * streams) == 0
+    out, cursors, pos = [], [], 0
+    for r in range(rows):
+        sym = m[r].astype(np.int64) - min_value
+        lanes = [encode_stream_quad(sym[stream_columns(cols, streams, l)], cdf)
+                 for l in range(streams)]
+        ends = np.cumsum([len(s) for s in lanes])
+        if ends[-1] > 0xFFFF:
+            raise ValueError("row %d: %d words do not fit 16-bit offsets" % (r, ends[-1]))
+        words = np.concatenate([ends.astype(np.uint16).view(np.uint32),
+                                np.asarray(sum(lanes, []), np.uint32)])
+        row = np.zeros((len(words) + 3) // 4 * 4, np.uint32)
+        row[:len(words)] = words
+        cursors.append(pos)
+        pos += 2 * len(row)
+        out.append(row.view(np.uint16))
+    return np.asarray(cursors, np.uint32), np.concatenate(out)
+
+
+def lane_ranges(payload, cursor, streams):
+    """(first u32 word, word count) of every stream of the coder-2 row at u16 ``cursor``."""
+    ends = np.asarray(payload, np.uint16)[cursor:cursor + streams].astype(np.int64)
+    begs = np.concatenate([[0], ends[:-1]])
+    if np.any(ends < begs):
+        raise ValueError("stream offsets decrease")
+    return cursor // 2 + streams // 2 + begs, ends - begs
+
+
+def decode_rows_quad_lanes(rows, cols, streams, cursors, payload, min_value, cdf):
+    """Decode a coder-2 matrix (one table), vectorised over the rows x streams streams.
+    A stream reading past its own words raises."""
+    pay32 = np.asarray(payload, np.uint16).view(np.uint32)
+    S = streams
+    beg, cnt = [], []
+    for r in range(rows):
+        b, n = lane_ranges(payload, int(cursors[r]), S)
+        beg.append(b)
+        cnt.append(n)
+    beg, cnt = np.concatenate(beg), np.concatenate(cnt)
+    if beg.size and (beg + cnt).max() > len(pay32):
+        raise IndexError("stream past the payload")
+
+    def word(i, j):
+        if np.any(j >= cnt[i]):
+            raise IndexError("stream reads past its words")
+        return pay32[beg[i] + j].astype(np.uint64)
+
+    return _decode_quad(rows, cols, S, np.asarray(cdf, np.int64), min_value, word)
+
+
+def _decode_quad(rows, cols, S, cdf, min_value, word):
+    """Quad-coder decode of rows x S streams (one table) with word(i, j) = word j of
+    streams i (row-major stream index)."""
+    n = rows * S
+    ppf = ppf_of(cdf, 8)
+    lane = np.tile(np.arange(S), rows)
+    row = np.repeat(np.arange(rows), S)
+    allst = np.arange(n)
+    x = (word(allst, np.ones(n, np.int64)) << np.uint64(32)) | word(allst, np.zeros(n, np.int64))
+    ptr = np.full(n, 2, np.int64)
+    out = np.empty((rows, cols), np.int8)
+    for j in range(cols // S // 4):
+        need = x < (np.uint64(1) << np.uint64(32))
+        if need.any():
+            i = np.nonzero(need)[0]
+            x[i] = (x[i] << np.uint64(32)) | word(i, ptr[i])
+            ptr[i] += 1
+        F = np.ones(n, np.uint64)
+        R = np.zeros(n, np.uint64)
+        for i in range(4):
+            q = ((x >> np.uint64(8 * i)) & np.uint64(0xFF)).astype(np.int64)
+            s = ppf[q]
+            k = 4 * j + i
+            col = 16 * S * (k // 16) + 16 * lane + k % 16
+            out[row, col] = ((min_value + s) & 0xFF).astype(np.uint8).view(np.int8)
+            R = R + F * (q - cdf[s]).astype(np.uint64)
+            F = F * (cdf[s + 1] - cdf[s]).astype(np.uint64)
+        x = (x >> np.uint64(32)) * F + R
+    return out
+
+
 def decode_rows(rows, cols, streams, precision, cursors, payload, min_value, cdf=None,
                 freq=None, payload_size=None):
     """Decode every row, vectorised over the rows x streams streams.  Raises IndexError
@@ -436,6 +531,9 @@ class RecordV1:
         self.grid_spacing = grid_spacing
 
     def decode(self):
+        if self.coder == 2:
+            return decode_rows_quad_lanes(self.rows, self.cols, self.streams, self.cursors,
+                                          self.payload, self.min_value, self.cdf)
         if self.coder == 1:
             return decode_rows_quad(self.rows, self.cols, self.streams, self.cursors,
                                     self.payload, self.min_value, self.cdf, self.freq)
@@ -445,6 +543,12 @@ class RecordV1:
 
 def encode_record(matrix, streams=64, precision=8, table_mode=0, coder=0):
     m = np.asarray(matrix, np.int8)
+    if coder == 2:
+        assert precision == 8 and table_mode == 0
+        mn, G, cdf = global_model(m, 8)
+        cur, pay = encode_rows_quad_lanes(m, streams, mn, cdf)
+        return RecordV1(m.shape[0], m.shape[1], streams, 8, 0, mn, G, cur, pay, cdf=cdf,
+                        coder=2)
     if coder == 1:
         assert precision == 8
         if table_mode == 0:

@@ -21,18 +21,19 @@ def note(msg):
     sys.stdout.flush()
 
 
-FORMATS = ["v0", "q32", "q64", "q128", "q64r"]   # q64r: 64 streams, one table per row
+# q64r: 64 streams, one table per row; l32 / l64: the quad streams back to back (coder 2)
+FORMATS = ["v0", "q32", "q64", "q128", "q64r", "l32", "l64"]
 
 
 def encode(fmt, w):
     """The matrix on the device in one of the encodings with a backward: .bin v0, or the
     MVX1 quad coder with 32 / 64 / 128 streams and one table per matrix, or 64 streams and
-    one table per row."""
+    one table per row, or the quad streams back to back with 32 / 64 streams."""
     if fmt == "v0":
         return mv.CompressedMatrix.encode(w, n_threads=16).to_device()
     assert _fits(fmt, w.shape[1])
     return CompressedMatrixV1.encode(w, _streams(fmt), 8, int(fmt.endswith("r")),
-                                     coder=1).to_device()
+                                     n_threads=16, coder=2 if fmt[0] == "l" else 1).to_device()
 
 
 def dense_ref(w, s_w, g, alpha):

@@ -37,12 +37,14 @@ BLOCK = [(2048, 2048), (512, 2048), (512, 2048), (2048, 2048), (7168, 2048), (71
 
 
 def make(fmt, m):
+    """v0, q32 / q64 (quad coder), l32 / l64 (quad streams back to back, coder 2)."""
     if fmt == "v0":
         return mv.CompressedMatrix.encode(m).to_device()
-    return CompressedMatrixV1.encode(m, int(fmt[1:]), 8, 0, coder=1).to_device()
+    return CompressedMatrixV1.encode(m, int(fmt[1:]), 8, 0,
+                                     coder=2 if fmt[0] == "l" else 1).to_device()
 
 
-@pytest.mark.parametrize("fmt", ["v0", "q32", "q64"])
+@pytest.mark.parametrize("fmt", ["v0", "q32", "q64", "l32", "l64"])
 @pytest.mark.parametrize("n_vec", [1, 4, 8])
 def test_decoder_block(cuda, fmt, n_vec):
     import torch
@@ -238,7 +240,7 @@ def bwd_inputs(m, n_vec, seed):
     return g, s_w, alpha
 
 
-@pytest.mark.parametrize("fmt", ["v0", "q32", "q64"])
+@pytest.mark.parametrize("fmt", ["v0", "q32", "q64", "l32", "l64"])
 @pytest.mark.parametrize("n_vec", [1, 8])
 def test_backward_batch_block(cuda, fmt, n_vec):
     import torch

@@ -52,12 +52,13 @@ def block_encoded(fmt, mats):
         if fmt == "v0":
             _ENCODED[fmt] = [mv.CompressedMatrix.encode(m).to_device() for m in mats]
         else:
-            _ENCODED[fmt] = [CompressedMatrixV1.encode(m, int(fmt[1:]), 8, 0, coder=1).to_device()
-                             for m in mats]
+            coder = 2 if fmt[0] == "l" else 1
+            _ENCODED[fmt] = [CompressedMatrixV1.encode(m, int(fmt[1:]), 8, 0, n_threads=16,
+                                                       coder=coder).to_device() for m in mats]
     return _ENCODED[fmt]
 
 
-@pytest.mark.parametrize("fmt", ["v0", "q32", "q64"])
+@pytest.mark.parametrize("fmt", ["v0", "q32", "q64", "l32"])
 @pytest.mark.parametrize("n_vec", [1, 4, 8])
 def test_decoder_block_full_size(cuda, block, fmt, n_vec):
     import torch
@@ -115,7 +116,7 @@ def test_uneven_batch_full_size(cuda):
     b.close()
 
 
-@pytest.mark.parametrize("fmt", ["v0", "q32", "q64"])
+@pytest.mark.parametrize("fmt", ["v0", "q32", "q64", "l32", "l64"])
 @pytest.mark.parametrize("rows,cols", [(4096, 14336), (14336, 4096)])
 def test_backward_eight_vectors_llm_shapes(cuda, fmt, rows, cols):
     """dx = alpha * W^T (s_w * g) for 8 gradient vectors from the compressed streams,
@@ -125,7 +126,8 @@ def test_backward_eight_vectors_llm_shapes(cuda, fmt, rows, cols):
     rng = np.random.default_rng(rows + cols)
     w = sample_int8(rng, (rows, cols), 6.0)
     cm = (mv.CompressedMatrix.encode(w) if fmt == "v0"
-          else CompressedMatrixV1.encode(w, int(fmt[1:]), 8, 0, coder=1)).to_device()
+          else CompressedMatrixV1.encode(w, int(fmt[1:]), 8, 0, n_threads=16,
+                                         coder=2 if fmt[0] == "l" else 1)).to_device()
     g = rng.standard_normal((8, rows)).astype(np.float32)
     s_w = (rng.random(rows) * 0.02 + 0.001).astype(np.float32)
     alpha = (rng.random(8) + 0.5).astype(np.float32)
@@ -138,7 +140,7 @@ def test_backward_eight_vectors_llm_shapes(cuda, fmt, rows, cols):
     assert np.linalg.norm(got - ref) <= 1e-5 * np.linalg.norm(ref)
 
 
-@pytest.mark.parametrize("fmt", ["v0", "q32", "q64"])
+@pytest.mark.parametrize("fmt", ["v0", "q32", "q64", "l32"])
 @pytest.mark.parametrize("n_vec", [1, 3])
 def test_backward_one_matrix_equals_batch_at_14336_rows(cuda, fmt, n_vec):
     """At 14336 rows the one-matrix backward's 8-wave grid would overflow the GPU, so it
@@ -150,7 +152,8 @@ def test_backward_one_matrix_equals_batch_at_14336_rows(cuda, fmt, n_vec):
     rng = np.random.default_rng(31 + n_vec)
     w = sample_int8(rng, (14336, 4096), 6.0)
     cm = (mv.CompressedMatrix.encode(w) if fmt == "v0"
-          else CompressedMatrixV1.encode(w, int(fmt[1:]), 8, 0, coder=1)).to_device()
+          else CompressedMatrixV1.encode(w, int(fmt[1:]), 8, 0, n_threads=16,
+                                         coder=2 if fmt[0] == "l" else 1)).to_device()
     g = torch.from_numpy(rng.standard_normal((n_vec, 14336)).astype(np.float32)).cuda()
     s_w = torch.from_numpy((rng.random(14336) * 0.02 + 0.001).astype(np.float32)).cuda()
     alpha = torch.from_numpy((rng.random(n_vec) + 0.5).astype(np.float32)).cuda()

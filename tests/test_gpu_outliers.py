@@ -39,15 +39,16 @@ def matrix(rows, cols, kind, seed):
     return f(rows, cols, 4.0 if kind == "laplace" else 6.0, seed=seed)[0]
 
 
+@pytest.mark.parametrize("coder", [1, 2])
 @pytest.mark.parametrize("S", [32, 64, 128])
 @pytest.mark.parametrize("rows,cols,kind,band", CASES)
-def test_forward_with_outliers(cuda, S, rows, cols, kind, band):
+def test_forward_with_outliers(cuda, S, rows, cols, kind, band, coder):
     import torch
     if cols % (16 * S):
         pytest.skip("shape does not fit the stream count")
-    note(f"outliers forward q{S} {rows}x{cols} {kind} band={band}")
+    note(f"outliers forward coder {coder} S={S} {rows}x{cols} {kind} band={band}")
     m = matrix(rows, cols, kind, rows + cols)
-    c = CompressedMatrixV1.encode(m, S, 8, 0, coder=1,
+    c = CompressedMatrixV1.encode(m, S, 8, 0, coder=coder, n_threads=16,
                                   outliers="auto" if band is None else band).to_device()
     assert c.n_outliers > 0
     per_row = np.diff(c.out_start).max()
@@ -92,15 +93,16 @@ def dense(w, s_w, g, alpha):
                                                 w.astype(np.float64))
 
 
+@pytest.mark.parametrize("coder", [1, 2])
 @pytest.mark.parametrize("S", [32, 64])
 @pytest.mark.parametrize("rows,cols,kind,band,n_vec", [
     (4096, 14336, "laplace", None, 1), (4096, 14336, "laplace", None, 8),
     (14336, 4096, "laplace", None, 2), (300, 2048, "gauss", 2, 3)])
-def test_backward_with_outliers(cuda, S, rows, cols, kind, band, n_vec):
+def test_backward_with_outliers(cuda, S, rows, cols, kind, band, n_vec, coder):
     import torch
-    note(f"outliers backward q{S} {rows}x{cols} band={band} n_vec={n_vec}")
+    note(f"outliers backward coder {coder} S={S} {rows}x{cols} band={band} n_vec={n_vec}")
     m = matrix(rows, cols, kind, 3 * rows + cols)
-    c = CompressedMatrixV1.encode(m, S, 8, 0, coder=1,
+    c = CompressedMatrixV1.encode(m, S, 8, 0, coder=coder, n_threads=16,
                                   outliers="auto" if band is None else band).to_device()
     rng = np.random.default_rng(9)
     g = rng.standard_normal((n_vec, rows)).astype(np.float32)

@@ -1,5 +1,5 @@
 """GPU parity of the MVX1 fused decode-matvec (64 streams per row, 12-bit probabilities,
-one table per row) against the brute-force integer product and the oracle decoder.
+one table per row, the quad coder in both layouts) against the brute-force integer product and the oracle decoder.
 Every launch goes through the C ABI (matvans_v1_decompress_and_multiply)."""
 import ctypes
 import sys
@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 VARIANTS = [(32, 8, 0, 0), (64, 8, 0, 0), (128, 8, 0, 0), (32, 12, 0, 0), (64, 12, 0, 0),
             (128, 12, 0, 0), (64, 8, 1, 0), (32, 8, 0, 1), (64, 8, 0, 1), (128, 8, 0, 1),
-            (64, 8, 1, 1)]
+            (64, 8, 1, 1), (32, 8, 0, 2), (64, 8, 0, 2), (128, 8, 0, 2)]
 
 
 def note(msg):
@@ -80,7 +80,7 @@ LLM = [(4096, 14336, "l", 4.0), (14336, 4096, "l", 4.0), (4096, 4096, "g", 6.0)]
 @pytest.mark.parametrize("S,P,mode,coder", [(128, 8, 0, 0), (64, 8, 0, 0), (32, 8, 0, 0),
                                             (64, 12, 0, 0), (64, 8, 1, 0), (32, 12, 0, 0),
                                             (32, 8, 0, 1), (64, 8, 0, 1), (128, 8, 0, 1),
-                                            (64, 8, 1, 1)])
+                                            (64, 8, 1, 1), (32, 8, 0, 2), (64, 8, 0, 2)])
 @pytest.mark.parametrize("rows,cols,kind,p", LLM)
 def test_llm_shapes(cuda, S, P, mode, coder, rows, cols, kind, p):
     note(f"v1 LLM S={S} P={P} mode={mode} coder={coder} {rows}x{cols} {kind}{p}")

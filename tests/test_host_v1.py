@@ -184,3 +184,68 @@ def test_outlier_container_round_trip(tmp_path):
     r.write_bytes(data[:-(4 * plain.stored_bytes // 8)])
     with pytest.raises(MatvansError):
         read_mvx1(r)
+
+
+# --------------------------------------------- quad coder, streams back to back (coder 2)
+@pytest.mark.parametrize("S", [32, 64, 128])
+def test_lanes_encoder_matches_oracle(S):
+    for m in _mats():
+        c = CompressedMatrixV1.encode(m, S, 8, 0, n_threads=3, coder=2)
+        r = E.encode_record(m, S, 8, 0, coder=2)
+        assert np.array_equal(c.cursors, r.cursors) and np.array_equal(c.payload, r.payload)
+        assert np.array_equal(c.cdf, r.cdf)
+        assert np.array_equal(r.decode(), m)
+        # the Gaussian and Laplace matrices (near-constant and uniform rows leave coder 1
+        # little padding, and the offsets cost more there)
+        if S <= 64 and 4 < len(np.unique(m)) < 200:
+            c1 = CompressedMatrixV1.encode(m, S, 8, 0, coder=1)
+            assert c.payload_size < c1.payload_size
+
+
+def test_lanes_container_round_trip_with_outliers(tmp_path):
+    m = laplace_int8(6, 4096, 4.0, seed=21)[0]
+    cs = [CompressedMatrixV1.encode(m, 32, coder=2, outliers="auto"),
+          CompressedMatrixV1.encode(m, 64, coder=2)]
+    assert cs[0].n_outliers > 0
+    p = tmp_path / "l.mvx"
+    write_mvx1(p, cs)
+    _, _, back = read_mvx1(p)
+    _, _, orecs = E.read_mvx1(p)
+    for c, b, o in zip(cs, back, orecs):
+        assert b.coder == 2 and np.array_equal(c.payload, b.payload)
+        assert np.array_equal(c.cursors, b.cursors) and b.n_outliers == c.n_outliers
+        w = o.decode()
+        if c.n_outliers:
+            w = E.merge_outliers(w, c.out_start, c.out_rec)
+        assert np.array_equal(w, m)
+
+
+def test_lanes_reader_rejects_bad_offsets(tmp_path):
+    c = CompressedMatrixV1.encode(_mats()[0], 32, 8, 0, coder=2)
+    for name, edit in [("decreasing", lambda pay, cur: pay.__setitem__(cur + 3, 0)),
+                       ("overrun", lambda pay, cur: pay.__setitem__(cur + 31, 0xFFFF)),
+                       ("state", lambda pay, cur: pay.__setitem__(cur + 1, pay[cur] + 1))]:
+        pay = c.payload.copy()
+        edit(pay, int(c.cursors[1]))
+        bad = CompressedMatrixV1(c.rows, c.cols, 32, 8, 0, c.min_value, c.G, c.cursors, pay,
+                                 cdf=c.cdf, coder=2)
+        p = tmp_path / f"{name}.mvx"
+        write_mvx1(p, [bad])
+        with pytest.raises(MatvansError) as e:
+            read_mvx1(p)
+        assert e.value.status == 4, name
+
+
+def test_lanes_row_beyond_16bit_offsets_is_refused():
+    m = np.random.default_rng(3).integers(-128, 128, (1, 1 << 19)).astype(np.int8)
+    with pytest.raises(MatvansError) as e:
+        CompressedMatrixV1.encode(m, 32, 8, 0, coder=2)
+    assert e.value.status == 2
+
+
+def test_lanes_need_one_table_and_8_bits():
+    m = _mats()[0]
+    for kw in [dict(precision=12), dict(table_mode=1)]:
+        with pytest.raises(MatvansError) as e:
+            CompressedMatrixV1.encode(m, 64, coder=2, **{"precision": 8, "table_mode": 0, **kw})
+        assert e.value.status == 8

@@ -213,3 +213,64 @@ def test_container_round_trip(tmp_path):
     q = tmp_path / "b.mvx"
     E.write_mvx1(q, back, v, hdr["result_hash"])
     assert open(p, "rb").read() == open(q, "rb").read()
+
+
+# --------------------------------------------- quad coder, streams back to back (coder 2)
+@pytest.mark.parametrize("S", [32, 64, 128])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_lanes_round_trip(S, name):
+    m = CASES[name]
+    if m.shape[1] % (16 * S):
+        m = m[:, : m.shape[1] // (16 * S) * 16 * S]
+    if m.shape[1] == 0:
+        pytest.skip("narrower than one group of the stream count")
+    r = E.encode_record(m, S, 8, 0, coder=2)
+    assert r.cursors[0] == 0 and np.all(r.cursors % 8 == 0)
+    assert np.array_equal(r.decode(), m)
+
+
+@pytest.mark.parametrize("S", [32, 64])
+def test_lanes_hold_the_quad_streams(S):
+    """Coder 2 stores exactly coder 1's streams: stream l's words gathered with coder 1's
+    granule formula equal the words between coder 2's end offsets, and a textbook decode of
+    those words gives the stream's columns."""
+    m = CASES["laplace4"][:, :2048]
+    r1 = E.encode_record(m, S, 8, 0, coder=1)
+    r2 = E.encode_record(m, S, 8, 0, coder=2)
+    p1, p2 = r1.payload.view(np.uint32), r2.payload.view(np.uint32)
+    G4 = 128 // S
+    for row in range(m.shape[0]):
+        beg, cnt = E.lane_ranges(r2.payload, int(r2.cursors[row]), S)
+        for lane in range(S):
+            w2 = [int(v) for v in p2[beg[lane]:beg[lane] + cnt[lane]]]
+            w1 = [int(p1[int(r1.cursors[row]) // 2 + 128 * (j // G4) + G4 * lane + j % G4])
+                  for j in range(len(w2))]
+            assert w1 == w2
+            syms = E.decode_stream_quad_textbook(w2, m.shape[1] // S // 4, r2.cdf)
+            want = m[row, E.stream_columns(m.shape[1], S, lane)].astype(np.int64) - r2.min_value
+            assert syms == want.tolist()
+
+
+def test_lanes_smaller_than_granules():
+    """Without the per-stream padding a row takes fewer words, most so on short rows."""
+    for S, m in [(32, laplace_int8(16, 4096, 4.0, seed=5)[0]),
+                 (64, laplace_int8(8, 8192, 4.0, seed=6)[0])]:
+        n1 = len(E.encode_record(m, S, 8, 0, coder=1).payload)
+        n2 = len(E.encode_record(m, S, 8, 0, coder=2).payload)
+        assert n2 < 0.96 * n1
+
+
+def test_lanes_decoder_catches_a_short_stream():
+    """A planted mistake in the end offsets (one stream a word shorter) is caught: the
+    decoder reads past the stream or decodes other values."""
+    m = CASES["gauss6"][:, :1024]
+    r = E.encode_record(m, 32, 8, 0, coder=2)
+    pay = r.payload.copy()
+    pay[int(r.cursors[2]) + 5] -= 1
+    bad = E.RecordV1(r.rows, r.cols, 32, 8, 0, r.min_value, r.G, r.cursors, pay, cdf=r.cdf,
+                     coder=2)
+    try:
+        out = bad.decode()
+    except (IndexError, ValueError):
+        return
+    assert not np.array_equal(out, m)
