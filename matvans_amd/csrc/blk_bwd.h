@@ -90,23 +90,33 @@ struct BwdRows {
     uint32_t row, wbeg, wend, rcur, rend;
     bool valid;
     float gs[NV];
+    // Every load takes a clamped index instead of a branch, so that all of them issue
+    // back to back and cost one memory round trip.
     __device__ __forceinline__ void init(const BwdJob &J, uint32_t rb, uint32_t lane) {
+        const uint32_t rows = J.rows, rmax = rows - 1u, ps = J.payload_size;
         const uint32_t row0 = rb * 64u;
         row = row0 + lane;
-        valid = row < J.rows;
-        const uint32_t last = row0 + 63u < J.rows ? row0 + 63u : J.rows - 1u;
-        wbeg = J.cursors[row0];
-        wend = last + 1 < J.rows ? J.cursors[last + 1] : J.payload_size;
-        if (wend > J.payload_size) wend = J.payload_size;
+        valid = row < rows;
+        const uint32_t last = row0 + 63u < rows ? row0 + 63u : rmax;
+        const uint32_t b = last + 1 < rows ? last + 1 : rmax;
+        const uint32_t r = valid ? row : rmax, r1 = row + 1 < rows ? row + 1 : rmax;
+        const uint32_t c0 = J.cursors[row0], cb = J.cursors[b], cr = J.cursors[r],
+                       cr1 = J.cursors[r1];
+        const float sw = J.s_w[r];
+        float gv[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+            gv[v] = J.g[size_t(v < int(J.n_vec) ? v : 0) * J.g_stride + r];
+        wbeg = c0;
+        wend = last + 1 < rows ? cb : ps;
+        if (wend > ps) wend = ps;
         if (wend < wbeg) wend = wbeg;
         wbeg = __builtin_amdgcn_readfirstlane(wbeg);
         wend = __builtin_amdgcn_readfirstlane(wend);
-        rcur = valid ? J.cursors[row] : 0u;
-        rend = valid ? (row + 1 < J.rows ? J.cursors[row + 1] : J.payload_size) : 0u;
+        rcur = valid ? cr : 0u;
+        rend = valid ? (row + 1 < rows ? cr1 : ps) : 0u;
 #pragma unroll
-        for (int v = 0; v < NV; v++)
-            gs[v] = valid && v < int(J.n_vec) ? J.s_w[row] * J.g[size_t(v) * J.g_stride + row]
-                                              : 0.0f;
+        for (int v = 0; v < NV; v++) gs[v] = valid && v < int(J.n_vec) ? sw * gv[v] : 0.0f;
     }
 };
 
@@ -348,12 +358,26 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
     const uint32_t tid = threadIdx.x;
     // the wave index is the same on every lane: scalar, so are the stream and the output
     const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    // the table entries, the coder-2 row table and the rows' indices load together (one
+    // memory round trip before the payload)
+    const uint32_t stream = sg * W + wave;   // 0..S-1
+    uint32_t ga = 0, gb = 0;
+    if constexpr (MODE == 0) {
+        ga = J.glut[tid & 255u];
+        gb = J.glut[256u + (tid & 255u)];
+    }
+    LaneTable lt{0, 0, 0};
+    if constexpr (LAY == 1) {
+        const uint32_t r = rb * 64u + lane;
+        lt = lane_table<S>(J.payload, r < J.rows ? r : J.rows - 1u, stream);
+    }
+    BwdRows<NV> R;
+    R.init(J, rb, lane);
     // packed table T[q] = f | r << 8 | sym << 24: from the A / B tables (A = f | sym << 24,
     // B = r), or one per row of the unit, lane l reading row l's
     const uint32_t *tab;
     if constexpr (MODE == 0) {
-        if (fresh && tid < 256)
-            sm.tab[tid] = (J.glut[tid] & 0xFF0000FFu) | ((J.glut[256 + tid] & 0xFFu) << 8);
+        if (fresh && tid < 256) sm.tab[tid] = (ga & 0xFF0000FFu) | ((gb & 0xFFu) << 8);
         tab = sm.tab;
     } else {
         __syncthreads();   // every wave of the previous unit is done with the tables
@@ -369,19 +393,15 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
         }
         tab = sm.rowtab + lane * 256u;
     }
-    const uint32_t stream = sg * W + wave;   // 0..S-1
-    BwdRows<NV> R;
-    R.init(J, rb, lane);
 
     QL st;
     if constexpr (LAY == 0) {
         st.init(J.payload, R.wbeg, R.wend, R.valid, R.rcur, R.rend, stream, sm.ring, sm.dummy,
                 wave, lane);
     } else {
-        uint32_t first, n;
-        lane_stream<S>(J.payload, R.rcur, R.rend, stream, R.valid, first, n);
-        st.init_lanes(J.payload, R.wbeg, R.wend, R.valid, first, n, sm.ring, sm.dummy, wave,
-                      lane);
+        const LaneStream ls = lane_stream<S>(lt, R.rcur, R.valid);
+        st.init_lanes(J.payload, R.wbeg, R.wend, R.valid, ls.body, ls.kb, S, stream, ls.tail,
+                      ls.tail_n, sm.ring, sm.dummy, wave, lane);
     }
     st.prologue();
     if (fresh || MODE == 1) __syncthreads();

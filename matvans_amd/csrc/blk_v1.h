@@ -82,6 +82,10 @@ struct RowOutliers {
             end = J.out_start[row + 1];
         }
     }
+    __device__ __forceinline__ void set_offsets(uint32_t b, uint32_t e) {
+        beg = b;
+        end = e;
+    }
     __device__ __forceinline__ void records(const V1Job &J, uint32_t sl) {
 #pragma unroll
         for (int k = 0; k < kSlots; k++) {
@@ -128,23 +132,67 @@ __device__ __forceinline__ void put_row(const V1Job &J, int v, uint32_t row, int
     }
 }
 
-// Coder 2 (lane-contiguous quad streams): stream sl of the row at u16 cursor rcur (row end
-// rend) from the row's u16 end offsets: its first u32 word in the payload and its word
-// count (0 when the row cannot hold its offsets).  Both loads stay inside the row.
+// Coder 2 (quad streams in body granules and tails): stream sl of a row, from the row's
+// table at the head of the payload (K as two u16, then the tails' u16 end offsets).  The
+// table's position depends on the row index only, so it loads with the row's cursor.
+struct LaneTable {
+    uint32_t kb, e0, e1;   // body chunks; the tail's start and end offsets (u32 words)
+};
 template <int S>
-__device__ __forceinline__ void lane_stream(const uint16_t *__restrict__ payload, uint32_t rcur,
-                                            uint32_t rend, uint32_t sl, bool valid,
-                                            uint32_t &first, uint32_t &n) {
-    first = 0;
-    n = 0;
-    if (valid && rcur + uint32_t(S) <= rend) {
-        const uint32_t e1 = payload[rcur + sl];
-        const uint32_t e0 = payload[rcur + (sl ? sl - 1u : 0u)];
-        const uint32_t b = sl ? e0 : 0u;
-        first = rcur / 2u + uint32_t(S) / 2u + b;
-        n = e1 >= b ? e1 - b : 0u;
-    }
+__device__ __forceinline__ LaneTable lane_table(const uint16_t *__restrict__ payload,
+                                                uint32_t row, uint32_t sl) {
+    const uint16_t *__restrict__ t = payload + size_t(row) * (S + 2);
+    const uint32_t t0 = t[0], t1 = t[1], e1 = t[2 + sl], e0 = t[2 + (sl ? sl - 1u : 0u)];
+    return LaneTable{t0 | t1 << 16, sl ? e0 : 0u, e1};
 }
+struct LaneStream {
+    uint32_t body, kb, tail, tail_n;
+};
+// the stream's body (u32 index), body chunks, tail (u32 index) and tail words, for the row
+// at u16 cursor rcur
+template <int S>
+__device__ __forceinline__ LaneStream lane_stream(const LaneTable &t, uint32_t rcur, bool valid) {
+    if (!valid) return LaneStream{0, 0, 0, 0};
+    return LaneStream{rcur / 2u, t.kb, rcur / 2u + 4u * t.kb * uint32_t(S) + t.e0,
+                      t.e1 >= t.e0 ? t.e1 - t.e0 : 0u};
+}
+
+// Everything a lane of a quad-coder block reads before its payload: the wave's window of
+// cursors, its row's cursor range, outlier offsets and (coder 2) row table.  The loads
+// take clamped indices instead of branches, so they issue back to back and cost one
+// memory round trip before the payload loads (behind branches each waited on its own).
+template <int S, int RPW, int LAY>
+struct RowIndex {
+    uint32_t wbeg, wend, rcur, rend, obeg, oend;
+    LaneStream ls;
+    __device__ __forceinline__ void load(const V1Job &J, uint32_t row0, uint32_t row,
+                                         uint32_t sl, bool valid) {
+        const uint32_t rows = J.rows, rmax = rows - 1u, ps = J.payload_size;
+        const uint32_t *__restrict__ cur = J.cursors;
+        const uint32_t a = row0 < rows ? row0 : rmax;
+        const uint32_t b = row0 + RPW < rows ? row0 + RPW : rmax;
+        const uint32_t r = valid ? row : rmax;
+        const uint32_t r1 = row + 1 < rows ? row + 1 : rmax;
+        const bool outl = J.out_start != nullptr;
+        const uint32_t *__restrict__ os = outl ? J.out_start : cur;   // out_start: rows + 1
+        const uint32_t ca = cur[a], cb = cur[b], cr = cur[r], cr1 = cur[r1];
+        const uint32_t o0 = os[r], o1 = os[outl ? r + 1 : r1];
+        LaneTable lt{0, 0, 0};
+        if constexpr (LAY == 1) lt = lane_table<S>(J.payload, r, sl);
+        wbeg = row0 < rows ? ca : 0u;
+        wend = row0 < rows ? (row0 + RPW < rows ? cb : ps) : 0u;
+        if (wend > ps) wend = ps;
+        if (wend < wbeg) wend = wbeg;
+        wbeg = __builtin_amdgcn_readfirstlane(wbeg);
+        wend = __builtin_amdgcn_readfirstlane(wend);
+        rcur = valid ? cr : 0u;
+        rend = valid ? (row + 1 < rows ? cr1 : ps) : 0u;
+        obeg = valid && outl ? o0 : 0u;
+        oend = valid && outl ? o1 : 0u;
+        ls = LaneStream{0, 0, 0, 0};
+        if constexpr (LAY == 1) ls = lane_stream<S>(lt, rcur, valid);
+    }
+};
 
 // Rows a block covers for S streams per row.
 template <int S>
@@ -424,12 +472,20 @@ __device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid,
                                   : (bid * kWaves + wave) * RPW;
     const uint32_t row = row0 + (S >= 64 ? 0u : lane / S);
     const bool valid = row < rows;
+    // the decode table's entries and every index, in one round trip
+    uint32_t ga = 0, gb = 0;
+    if constexpr (MODE == 0) {
+        ga = glut[tid];
+        gb = glut[256 + tid];
+    }
+    RowIndex<S, RPW, LAY> ix;
+    ix.load(J, row0, row, sl, valid);
     // packed table T[q] = f | r << 8 | sym << 24 (r = q - cdf[s]); f <= 255 and r < f for
     // every table the encoder accepts
     uint32_t *tab;
     if constexpr (MODE == 0) {   // from the A / B tables (A = f | sym << 24, B = r)
         tab = sm.tab;
-        tab[tid] = (glut[tid] & 0xFF0000FFu) | ((glut[256 + tid] & 0xFFu) << 8);
+        tab[tid] = (ga & 0xFF0000FFu) | ((gb & 0xFFu) << 8);
     } else {                     // this wave's row's own table
         tab = sm.rowtab + wave * 448;
         build_row_table(J.freq, row, J.G, valid, J.min_value, lane,
@@ -438,31 +494,19 @@ __device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid,
                             tab[q] = f | (b << 8) | (sy << 24);
                         });
     }
-    uint32_t wbeg = 0, wend = 0;
-    if (row0 < rows) {
-        wbeg = cursors[row0];
-        uint32_t last = row0 + RPW - 1 < rows ? row0 + RPW - 1 : rows - 1;
-        wend = last + 1 < rows ? cursors[last + 1] : payload_size;
-        if (wend > payload_size) wend = payload_size;
-        if (wend < wbeg) wend = wbeg;
-    }
-    wbeg = __builtin_amdgcn_readfirstlane(wbeg);
-    wend = __builtin_amdgcn_readfirstlane(wend);
-    const uint32_t rcur = valid ? cursors[row] : 0u;
-    const uint32_t rend = valid ? (row + 1 < rows ? cursors[row + 1] : payload_size) : 0u;
+    const uint32_t wbeg = ix.wbeg, wend = ix.wend, rcur = ix.rcur, rend = ix.rend;
     RowOutliers<NV, S> outl;
-    outl.offsets(J, row, valid);
+    outl.set_offsets(ix.obeg, ix.oend);
 
     QL st;
     if constexpr (LAY == 0) {
         st.init(payload, wbeg, wend, valid, rcur, rend, sl, ring, dummy, wave, lane);
     } else {
-        uint32_t first, n;
-        lane_stream<S>(payload, rcur, rend, sl, valid, first, n);
-        st.init_lanes(payload, wbeg, wend, valid, first, n, ring, dummy, wave, lane);
+        st.init_lanes(payload, wbeg, wend, valid, ix.ls.body, ix.ls.kb, S, sl, ix.ls.tail,
+                      ix.ls.tail_n, ring, dummy, wave, lane);
     }
-    outl.records(J, sl);
     st.prologue();
+    outl.records(J, sl);
 
     const uint32_t groups = cols / (16u * S);
     // the vectors through a buffer resource: the lane's byte offset is fixed, the group's
@@ -634,36 +678,28 @@ __device__ __forceinline__ void quad_block_mfma(const V1Job &J, uint32_t bid,
     const uint32_t row = bid * kRows + rloc;
     const bool valid = row < rows;
     uint32_t *tab = sm.tab;
-    if (tid < 256) tab[tid] = (glut[tid] & 0xFF0000FFu) | ((glut[256 + tid] & 0xFFu) << 8);
+    // the decode table's entries and every index, in one round trip
+    const uint32_t ga = glut[tid & 255u], gb = glut[256u + (tid & 255u)];
+    RowIndex<S, 2, LAY> ix;
+    ix.load(J, row0, row, sl, valid);
+    if (tid < 256) tab[tid] = (ga & 0xFF0000FFu) | ((gb & 0xFFu) << 8);
     if constexpr (MW == 4) {   // operand rows 8..15 of both stages stay zero
         uint32_t *z0 = sm.stage[0] + 8 * kStageRow / 4, *z1 = sm.stage[1] + 8 * kStageRow / 4;
         for (uint32_t k = tid; k < 8 * kStageRow / 4; k += 64 * MW) z0[k] = z1[k] = 0;
     }
-    uint32_t wbeg = 0, wend = 0;
-    if (row0 < rows) {
-        wbeg = cursors[row0];
-        const uint32_t last = row0 + 1 < rows ? row0 + 1 : rows - 1;
-        wend = last + 1 < rows ? cursors[last + 1] : payload_size;
-        if (wend > payload_size) wend = payload_size;
-        if (wend < wbeg) wend = wbeg;
-    }
-    wbeg = __builtin_amdgcn_readfirstlane(wbeg);
-    wend = __builtin_amdgcn_readfirstlane(wend);
-    const uint32_t rcur = valid ? cursors[row] : 0u;
-    const uint32_t rend = valid ? (row + 1 < rows ? cursors[row + 1] : payload_size) : 0u;
+    const uint32_t wbeg = ix.wbeg, wend = ix.wend, rcur = ix.rcur, rend = ix.rend;
     RowOutliers<NV, S> outl;
-    outl.offsets(J, row, valid);
+    outl.set_offsets(ix.obeg, ix.oend);
 
     QL st;
     if constexpr (LAY == 0) {
         st.init(payload, wbeg, wend, valid, rcur, rend, sl, sm.ring, sm.dummy, wave, lane);
     } else {
-        uint32_t first, n;
-        lane_stream<S>(payload, rcur, rend, sl, valid, first, n);
-        st.init_lanes(payload, wbeg, wend, valid, first, n, sm.ring, sm.dummy, wave, lane);
+        st.init_lanes(payload, wbeg, wend, valid, ix.ls.body, ix.ls.kb, S, sl, ix.ls.tail,
+                      ix.ls.tail_n, sm.ring, sm.dummy, wave, lane);
     }
-    outl.records(J, sl);
     st.prologue();
+    outl.records(J, sl);
 
     const uint32_t groups = cols / (16u * S);
     const uint32_t nv_eff = J.n_vec < uint32_t(NV) ? J.n_vec : uint32_t(NV);

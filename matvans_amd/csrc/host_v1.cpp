@@ -8,7 +8,10 @@
 // Stream l of a row decodes columns 16*S*(k/16) + 16*l + k%16.  A row is K granules of
 // 512 bytes; every stream owns GW = 256/S consecutive u16 words of each granule, so word
 // j of stream l is at cursor + 256*(j/GW) + GW*l + j%GW.  Coder 2 keeps the quad coder's
-// streams but stores them back to back behind S u16 end offsets (no per-stream padding).
+// streams without per-stream padding: per row the first 4 K words of every stream in K
+// 16-byte-per-stream granules (K = the shortest stream's whole 4-word chunks), the rest
+// of each stream back to back, with K and the tails' end offsets in row tables at the
+// head of the payload.
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -69,10 +72,15 @@ bool valid_params(uint32_t streams, uint32_t precision, uint32_t table_mode, uin
             (coder == 2 && precision == 8 && table_mode == 0));
 }
 
-// Coder 2 row: S u16 end offsets (S / 2 u32 words), the streams' u32 words back to back,
-// zero words to a multiple of four; its size in u16 words.
-uint64_t lanes_row_u16(uint32_t S, uint64_t stream_words) {
-    return 2 * ((uint64_t(S) / 2 + stream_words + 3) / 4 * 4);
+// Coder 2: the row tables at the head of the payload (S + 2 u16 per row: K as two u16,
+// then the S tail end offsets), zero-padded to a multiple of 8 u16.
+uint64_t lanes_table_u16(uint32_t rows, uint32_t S) {
+    return (uint64_t(rows) * (S + 2) + 7) / 8 * 8;
+}
+// A coder-2 row: K granules of S x 4 words, the tails, zero words to a multiple of four;
+// its size in u16 words.
+uint64_t lanes_row_u16(uint32_t S, uint64_t K, uint64_t tail_words) {
+    return 2 * ((4 * K * S + tail_words + 3) / 4 * 4);
 }
 
 // Quad coder (coder 1): 64-bit state, 32-bit words, four 8-bit quantiles per step (the
@@ -318,7 +326,7 @@ extern "C" int matvans_v1_encode(const int8_t *mat, uint32_t rows, uint32_t cols
                 c[0] = 0;
                 for (uint32_t s = 0; s < G; s++) c[s + 1] = c[s] + freq[size_t(r) * G + s];
             }
-            uint32_t mx = 0;
+            uint32_t mx = 0, mn_len = 0xFFFFFFFFu;
             uint64_t sum = 0;
             for (uint32_t l = 0; l < S; l++) {
                 int e;
@@ -339,15 +347,18 @@ extern "C" int matvans_v1_encode(const int8_t *mat, uint32_t rows, uint32_t cols
                     return;
                 }
                 mx = std::max<uint32_t>(mx, uint32_t(len));
+                mn_len = std::min<uint32_t>(mn_len, uint32_t(len));
                 sum += len;
             }
-            if (coder == 2 && sum > 0xFFFFu) {   // the end offsets are u16
+            // coder 2: the tails' words must fit the u16 end offsets
+            const uint64_t K2 = mn_len / 4, tails = sum - 4 * K2 * S;
+            if (coder == 2 && tails > 0xFFFFu) {
                 bad = int(r);
                 return;
             }
             // coder 2: the row's size in u16 words; otherwise its granules
             kgr[r] = coder == 0 ? (mx + GW - 1) / GW
-                   : coder == 1 ? (mx + G4 - 1) / G4 : uint32_t(lanes_row_u16(S, sum));
+                   : coder == 1 ? (mx + G4 - 1) / G4 : uint32_t(lanes_row_u16(S, K2, tails));
         }
     };
     std::vector<std::thread> th;
@@ -356,21 +367,26 @@ extern "C" int matvans_v1_encode(const int8_t *mat, uint32_t rows, uint32_t cols
     for (auto &x : th) x.join();
     th.clear();
     if (bad >= 0) {
-        if (coder == 2) {   // a symbol without probability, or more than 65535 words
-            size_t r = size_t(bad), total = 0;
-            for (uint32_t l = 0; l < S; l++) total += st4[r * S + l].size();
-            if (total > 0xFFFFu)
-                return fail(MATVANS_E_SHAPE, "row %d: %zu stream words exceed the 16-bit "
-                            "offsets of the lane-contiguous layout", bad.load(), total);
+        if (coder == 2) {   // a symbol without probability, or tails beyond 65535 words
+            size_t r = size_t(bad), total = 0, mn_len = ~size_t(0);
+            for (uint32_t l = 0; l < S; l++) {
+                total += st4[r * S + l].size();
+                mn_len = std::min(mn_len, st4[r * S + l].size());
+            }
+            if (mn_len != ~size_t(0) && total - 4 * (mn_len / 4) * S > 0xFFFFu)
+                return fail(MATVANS_E_SHAPE, "row %d: %zu tail words exceed the 16-bit "
+                            "offsets of coder 2", bad.load(), total - 4 * (mn_len / 4) * S);
         }
         return fail(MATVANS_E_RANGE, "row %d holds a symbol the model gives no probability", int(bad));
     }
-    uint64_t pos = 0;
+    const uint64_t head = coder == 2 ? lanes_table_u16(rows, S) : 0;
+    uint64_t pos = head;
     for (uint32_t r = 0; r < rows; r++) {
         if (pos > 0xFFFFFFFFull) return fail(MATVANS_E_CAPACITY, "payload exceeds 2^32 words");
         cursors[r] = uint32_t(pos);
         pos += coder == 2 ? uint64_t(kgr[r]) : uint64_t(kgr[r]) * 256;
     }
+    if (coder == 2 && pos <= payload_cap) std::memset(payload, 0, size_t(head) * 2);
     if (pos > 0xFFFFFFFFull) return fail(MATVANS_E_CAPACITY, "payload exceeds 2^32 words");
     if (pos > payload_cap) return fail(MATVANS_E_CAPACITY, "payload needs %llu words, cap %llu",
                                        (unsigned long long)pos, (unsigned long long)payload_cap);
@@ -379,16 +395,20 @@ extern "C" int matvans_v1_encode(const int8_t *mat, uint32_t rows, uint32_t cols
             uint16_t *dst = payload + cursors[r];
             if (coder == 2) {
                 std::memset(dst, 0, size_t(kgr[r]) * 2);
+                size_t K = ~size_t(0);
+                for (uint32_t l = 0; l < S; l++) K = std::min(K, st4[size_t(r) * S + l].size());
+                K /= 4;
+                uint16_t *tab = payload + size_t(r) * (S + 2);
+                tab[0] = uint16_t(K);
+                tab[1] = uint16_t(K >> 16);
+                auto put = [](uint16_t *w, uint32_t v) { w[0] = uint16_t(v); w[1] = uint16_t(v >> 16); };
+                uint16_t *tail = dst + 8 * K * S;   // tails after the K granules
                 uint32_t end = 0;
-                uint16_t *w = dst + S;   // stream words after the S u16 offsets
                 for (uint32_t l = 0; l < S; l++) {
                     const auto &s = st4[size_t(r) * S + l];
-                    for (size_t j = 0; j < s.size(); j++, w += 2) {
-                        w[0] = uint16_t(s[j]);
-                        w[1] = uint16_t(s[j] >> 16);
-                    }
-                    end += uint32_t(s.size());
-                    dst[l] = uint16_t(end);
+                    for (size_t j = 0; j < 4 * K; j++) put(dst + 2 * (4 * ((j / 4) * S + l) + j % 4), s[j]);
+                    for (size_t j = 4 * K; j < s.size(); j++, end++) put(tail + 2 * size_t(end), s[j]);
+                    tab[2 + l] = uint16_t(end);
                 }
                 continue;
             }
@@ -556,23 +576,27 @@ extern "C" int matvans_v1_read(const char *path, matvans_v1_file **out) {
             }
         }
         const uint32_t unit = m.coder == 2 ? 8u : 256u;   // 16-byte rows, or granules
+        // coder 2: the row tables come first
+        const uint64_t head = m.coder == 2 ? lanes_table_u16(m.rows, m.streams) : 0;
+        if (head > m.payload_size) return bad("payload shorter than its row tables", k);
         for (uint32_t r = 0; r < m.rows; r++) {
             uint32_t end = r + 1 < m.rows ? m.cursors[r + 1] : m.payload_size;
             if (m.cursors[r] % unit || end < m.cursors[r] || end > m.payload_size ||
-                (r == 0 && m.cursors[0] != 0))
+                (r == 0 && m.cursors[0] != head))
                 return bad("bad cursors", k);
             if (m.coder == 2) {
-                // end offsets: every stream holds at least its two state words, and the
-                // streams end within the row
-                const uint16_t *e = m.payload + m.cursors[r];
+                // K granules and the tails' end offsets: every stream holds at least its
+                // two state words, and the granules and tails end within the row
+                const uint16_t *t = m.payload + size_t(r) * (m.streams + 2);
+                const uint64_t K = uint64_t(t[0]) | uint64_t(t[1]) << 16;
                 const uint64_t room = (uint64_t(end) - m.cursors[r]) / 2;
-                if (room < m.streams / 2) return bad("row shorter than its stream offsets", k);
                 uint32_t prev = 0;
                 for (uint32_t l = 0; l < m.streams; l++) {
-                    if (uint32_t(e[l]) < prev + 2) return bad("stream shorter than its state", k);
-                    prev = e[l];
+                    const uint32_t e = t[2 + l];
+                    if (e < prev || 4 * K + (e - prev) < 2) return bad("bad stream offsets", k);
+                    prev = e;
                 }
-                if (m.streams / 2 + uint64_t(prev) > room) return bad("streams overrun the row", k);
+                if (4 * K * m.streams + uint64_t(prev) > room) return bad("streams overrun the row", k);
             }
         }
         if (m.rows && m.payload_size % unit) return bad("payload not a whole number of granules", k);

@@ -275,11 +275,11 @@ struct Lane {
 // next row's words inside the window or return zeros past it (buffer range check), into
 // ring slots the fit check has shown to be free, and the lane never consumes them.
 //
-// LAY 1 is the lane-contiguous layout of coder 2: the lane's stream is its own run of u32
-// words (no granules), read as 16-byte chunks from the aligned address at or below its
-// first word; that word lands in ring slot a0 = (first word) & 3, the slots before it hold
-// the previous stream's tail and are never read.  The fit rule and the bounds above hold
-// with the read index starting at a0 + 2 instead of 2 (a0 <= 3 < M).
+// LAY 1 is coder 2's layout: a chunk is 4 consecutive words (16 bytes) of the lane's
+// stream; the row's first Kb chunks of every stream sit in its body granules (chunk c of
+// stream l at byte c * S * 16 + l * 16 of the body: one coalesced load per wave), the
+// rest in the stream's tail, chunk Kb + t at 16 t bytes past the tail's first word (a
+// dword-aligned 16-byte load).  The fit rule and the bounds above are unchanged.
 template <int G4, int SETW = 4, int LAY = 0>
 struct QuadLane {
     static constexpr int CW = LAY ? 4 : G4;               // u32 words per chunk
@@ -294,7 +294,7 @@ struct QuadLane {
 
     __amdgpu_buffer_rsrc_t rsrc;
     uint32_t lane_off, K, gs, j;   // K chunks of the lane (granules of the row); gs next set
-    uint32_t a0;           // LAY 1: ring slot of the state's low word
+    uint32_t Kb, gstride, tail_off;   // LAY 1: body chunks, body granule bytes, tail offset
     uint32_t lo, hi;       // state
     uint32_t W;            // next unread word
     uint32_t waddr;        // LDS byte offset of that word within the ring array
@@ -320,7 +320,7 @@ struct QuadLane {
             }
         }
         lane_off = roff + slot * (G4 * 4u);
-        a0 = 0;
+        Kb = gstride = tail_off = 0;
         ring = ring_array;
         dummy = dummy_rows;
         lane = lane_in_wave;
@@ -328,22 +328,27 @@ struct QuadLane {
         sa = sb = 0xFFFFFFFFu;
     }
 
-    // LAY 1: the lane's stream is n u32 words from u32 index `first` of the payload,
-    // inside the u16 window [win_beg, win_end) (win_beg a multiple of 8).
+    // LAY 1: stream `slot` of a row whose body starts at u32 index `body` of the payload
+    // (kb chunks per stream, S streams) and whose own tail is tail_n words from u32 index
+    // `tail`, inside the u16 window [win_beg, win_end) (win_beg a multiple of 8).
     __device__ void init_lanes(const uint16_t *payload, uint32_t win_beg, uint32_t win_end,
-                               bool valid, uint32_t first, uint32_t n, uint32_t *ring_array,
+                               bool valid, uint32_t body, uint32_t kb, uint32_t S, uint32_t slot,
+                               uint32_t tail, uint32_t tail_n, uint32_t *ring_array,
                                uint32_t *dummy_rows, uint32_t wave, uint32_t lane_in_wave) {
-        static_assert(LAY == 1, "lane-contiguous layout");
+        static_assert(LAY == 1, "coder 2 layout");
         rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(payload) + win_beg, 0,
                                                  static_cast<int>((win_end - win_beg) * 2u),
                                                  0x00020000);
-        K = 0;
-        a0 = 0;
-        lane_off = 0;
-        if (valid && n >= 2 && 2u * first >= win_beg && 2u * (first + n) <= win_end) {
-            a0 = first & 3u;
-            K = (a0 + n + 3u) >> 2;
-            lane_off = ((first & ~3u) * 2u - win_beg) * 2u;
+        K = Kb = 0;
+        lane_off = tail_off = 0;
+        gstride = S * 16u;
+        const uint64_t body_end = uint64_t(body) + 4ull * kb * S;
+        if (valid && 4u * kb + tail_n >= 2u && 2u * body >= win_beg && 2 * body_end <= win_end &&
+            2u * tail >= win_beg && 2ull * (uint64_t(tail) + tail_n) <= win_end) {
+            Kb = kb;
+            K = kb + (tail_n + 3u) / 4u;
+            lane_off = (2u * body - win_beg) * 2u + slot * 16u;
+            tail_off = (2u * tail - win_beg) * 2u;
         }
         ring = ring_array;
         dummy = dummy_rows;
@@ -352,14 +357,21 @@ struct QuadLane {
         sa = sb = 0xFFFFFFFFu;
     }
 
+    // byte offset of chunk c within the window
+    __device__ __forceinline__ uint32_t chunk_off(uint32_t c) const {
+        if constexpr (LAY == 1) return c < Kb ? lane_off + c * gstride : tail_off + (c - Kb) * 16u;
+        return lane_off + c * kStride;
+    }
+
     __device__ __forceinline__ uint32_t &ring_at(uint32_t w) {
         return ring[(wave_base >> 2) + w * 64 + lane];
     }
 
-    // loads set `set` (chunks set*NG ..): one byte offset, the chunks kStride apart
-    __device__ __forceinline__ void load_set(Chunk<2 * CW> (&p)[NG], uint32_t base) {
+    // loads the NG chunks of set `set` (or, when !ok, out-of-range zeros)
+    __device__ __forceinline__ void load_set(Chunk<2 * CW> (&p)[NG], uint32_t set, bool ok) {
 #pragma unroll
-        for (int i = 0; i < NG; i++) p[i] = load_chunk<2 * CW>(rsrc, base + uint32_t(i) * kStride);
+        for (int i = 0; i < NG; i++)
+            p[i] = load_chunk<2 * CW>(rsrc, ok ? chunk_off(set * NG + uint32_t(i)) : kOOB);
     }
 
     __device__ __forceinline__ void write_set(const Chunk<2 * CW> (&p)[NG], uint32_t set) {
@@ -373,7 +385,7 @@ struct QuadLane {
     __device__ __forceinline__ void issue(Chunk<2 * CW> (&p)[NG], uint32_t &sp) {
         const uint32_t g0 = gs * NG;
         const bool ok = g0 < K && (gs + 1) * M - j <= uint32_t(R4);
-        load_set(p, ok ? lane_off + g0 * kStride : kOOB);
+        load_set(p, gs, ok);
         sp = ok ? gs : 0xFFFFFFFFu;
         gs += ok ? 1u : 0u;
     }
@@ -381,8 +393,7 @@ struct QuadLane {
     __device__ void prologue() {
         Chunk<2 * CW> pre[kPreSets][NG];
 #pragma unroll
-        for (uint32_t t = 0; t < uint32_t(kPreSets); t++)
-            load_set(pre[t], t * NG < K ? lane_off + t * NG * kStride : kOOB);
+        for (uint32_t t = 0; t < uint32_t(kPreSets); t++) load_set(pre[t], t, t * NG < K);
 #pragma unroll
         for (uint32_t t = 0; t < uint32_t(kPreSets); t++) write_set(pre[t], t);
         const uint32_t k_sets = (K + NG - 1) / NG;
@@ -390,12 +401,11 @@ struct QuadLane {
     }
 
     __device__ void start() {
-        const uint32_t a = LAY ? a0 : 0u;
-        lo = ring_at(a);
-        hi = ring_at(a + 1);
-        j = a + 2;
-        W = ring_at(a + 2);
-        waddr = wave_base | ((a + 2u) * 256u) | (lane * 4u);
+        lo = ring_at(0);
+        hi = ring_at(1);
+        j = 2;
+        W = ring_at(2);
+        waddr = wave_base | (2u * 256u) | (lane * 4u);
         issue(pa, sa);
         issue(pb, sb);
     }

@@ -43,14 +43,16 @@ payload; word j of stream l is at u16 index cursor + 256*(j // G4) + 2*(G4*l + j
 with G4 = 128 / S words per stream per 512-byte granule; words 0 and 1 are the low and
 high words of the initial state.
 
-Quad coder, lane-contiguous layout (coder = 2; P = 8, one table, S = 32, 64 or 128): the
-quad coder's streams, word for word as coder 1, without per-lane padding.  A row at u16
-index cursor (a multiple of 8, so 16-byte aligned) is S u16 end offsets e[0..S-1]
-(e[l] = the u32 words of streams 0..l), then the streams' words back to back (stream l:
-u32 words e[l-1] .. e[l] - 1 after the offsets, e[-1] = 0), then zero u32 words to a
-multiple of four.  Coder 1 pads every stream of a row to the longest one and to its
-granule slot (~0.3 bits per entry at 4096 x 14336, ~1 at 14336 x 4096); here a stream
-costs 16 bits of offset.
+Quad coder, body and tails (coder = 2; P = 8, one table, S = 32, 64 or 128): the quad
+coder's streams, word for word as coder 1, without per-stream padding.  The payload
+starts with the row tables, S + 2 u16 per row (zero-padded to 8 u16 in all): K (u32, as
+two u16) and the S tail end offsets e[l].  Row r at u16 index cursor (a multiple of 8)
+holds the body, the first 4 K words of every stream in K 16-byte-per-stream granules
+(word 4 g + i of stream l at u32 cursor/2 + 4 (g S + l) + i; K = the shortest stream's
+whole 4-word chunks), then the tails, words 4 K .. of every stream back to back (stream
+l's at u32 cursor/2 + 4 K S + e[l-1], e[-1] = 0), zero words to a multiple of four.
+Coder 1 pads every stream to the longest of its row and to its granule slot (~0.3 bits
+per entry at 4096 x 14336, ~1 at 14336 x 4096); here a stream costs 16 bits of offset.
 
 Outlier split (quad coder, one table): entries with |w| > band are coded as 0 in the
 streams and listed per row as records col | (uint8)w << 24 (out_start[rows + 1] offsets,
@@ -313,37 +315,65 @@ def encode_rows(matrix, streams, precision, min_value, cdf=None, freq=None):
     return np.asarray(cursors, np.uint32), np.concatenate(out)
 
 
+def lanes_table_u16(rows, streams):
+    """u16 words of a coder-2 matrix's row tables (rows x (S + 2), zero-padded to 8)."""
+    return (rows * (streams + 2) + 7) // 8 * 8
+
+
 def encode_rows_quad_lanes(matrix, streams, min_value, cdf):
-    """Quad-coder encode of a matrix in the lane-contiguous layout (coder 2, one table).
-    Returns (cursors uint32[rows] in u16 units, payload uint16)."""
+    """Quad-coder encode of a matrix in coder 2's layout (one table): per row, the first
+    4 K words of every stream in K granules (K = the shortest stream's whole 4-word
+    chunks), the rest of each stream back to back.  Returns (cursors uint32[rows] in u16
+    units, payload uint16)."""
     m = np.asarray(matrix, np.int8)
     rows, cols = m.shape
-    assert streams in (32, 64, 128) and cols % (16 * streams) == 0
-    out, cursors, pos = [], [], 0
+    S = streams
+    assert S in (32, 64, 128) and cols % (16 * S) == 0
+    table = np.zeros(lanes_table_u16(rows, S), np.uint16)
+    out, cursors, pos = [], [], len(table)
     for r in range(rows):
         sym = m[r].astype(np.int64) - min_value
-        lanes = [encode_stream_quad(sym[stream_columns(cols, streams, l)], cdf)
-                 for l in range(streams)]
-        ends = np.cumsum([len(s) for s in lanes])
-        if ends[-1] > 0xFFFF:
-            raise ValueError("row %d: %d words do not fit 16-bit offsets" % (r, ends[-1]))
-        words = np.concatenate([ends.astype(np.uint16).view(np.uint32),
-                                np.asarray(sum(lanes, []), np.uint32)])
+        lanes = [encode_stream_quad(sym[stream_columns(cols, S, l)], cdf) for l in range(S)]
+        K = min(len(s) for s in lanes) // 4
+        body = np.array([s[:4 * K] for s in lanes], np.uint32).reshape(S, K, 4)
+        tails = [s[4 * K:] for s in lanes]
+        ends = np.cumsum([len(t) for t in tails])
+        if ends[-1] > 0xFFFF or K > 0xFFFF:
+            raise ValueError("row %d: tails do not fit 16-bit offsets" % r)
+        t = table[r * (S + 2):(r + 1) * (S + 2)]
+        t[0], t[1], t[2:] = K & 0xFFFF, K >> 16, ends
+        words = np.concatenate([body.transpose(1, 0, 2).ravel(),
+                                np.asarray(sum(tails, []), np.uint32)])
         row = np.zeros((len(words) + 3) // 4 * 4, np.uint32)
         row[:len(words)] = words
         cursors.append(pos)
         pos += 2 * len(row)
         out.append(row.view(np.uint16))
-    return np.asarray(cursors, np.uint32), np.concatenate(out)
+    return np.asarray(cursors, np.uint32), np.concatenate([table] + out)
 
 
-def lane_ranges(payload, cursor, streams):
-    """(first u32 word, word count) of every stream of the coder-2 row at u16 ``cursor``."""
-    ends = np.asarray(payload, np.uint16)[cursor:cursor + streams].astype(np.int64)
+def lanes_row(payload, cursors, rows, streams, r):
+    """(K, tail starts, tail lengths) of row r of a coder-2 matrix; the starts are u32
+    indices into the payload."""
+    S = streams
+    pay = np.asarray(payload, np.uint16)
+    t = pay[r * (S + 2):(r + 1) * (S + 2)].astype(np.int64)
+    K = int(t[0]) | int(t[1]) << 16
+    ends = t[2:]
     begs = np.concatenate([[0], ends[:-1]])
     if np.any(ends < begs):
-        raise ValueError("stream offsets decrease")
-    return cursor // 2 + streams // 2 + begs, ends - begs
+        raise ValueError("tail offsets decrease")
+    tail0 = int(cursors[r]) // 2 + K * S * 4
+    return K, tail0 + begs, ends - begs
+
+
+def lane_words(payload, cursors, rows, streams, r, l):
+    """The u32 words of stream l of row r of a coder-2 matrix, in decode order."""
+    pay32 = np.asarray(payload, np.uint16).view(np.uint32)
+    K, tb, tn = lanes_row(payload, cursors, rows, streams, r)
+    base = int(cursors[r]) // 2
+    body = [int(pay32[base + (g * streams + l) * 4 + i]) for g in range(K) for i in range(4)]
+    return body + [int(w) for w in pay32[tb[l]:tb[l] + tn[l]]]
 
 
 def decode_rows_quad_lanes(rows, cols, streams, cursors, payload, min_value, cdf):
@@ -351,19 +381,23 @@ def decode_rows_quad_lanes(rows, cols, streams, cursors, payload, min_value, cdf
     A stream reading past its own words raises."""
     pay32 = np.asarray(payload, np.uint16).view(np.uint32)
     S = streams
-    beg, cnt = [], []
+    Ks, tb, tn, base = [], [], [], []
     for r in range(rows):
-        b, n = lane_ranges(payload, int(cursors[r]), S)
-        beg.append(b)
-        cnt.append(n)
-    beg, cnt = np.concatenate(beg), np.concatenate(cnt)
-    if beg.size and (beg + cnt).max() > len(pay32):
+        K, b, n = lanes_row(payload, cursors, rows, S, r)
+        Ks.append(np.full(S, K))
+        tb.append(b)
+        tn.append(n)
+        base.append(int(cursors[r]) // 2 + 4 * np.arange(S))
+    Ks, tb, tn, base = (np.concatenate(a).astype(np.int64) for a in (Ks, tb, tn, base))
+    if tb.size and (tb + tn).max() > len(pay32):   # tails follow the bodies
         raise IndexError("stream past the payload")
 
     def word(i, j):
-        if np.any(j >= cnt[i]):
+        if np.any(j >= 4 * Ks[i] + tn[i]):
             raise IndexError("stream reads past its words")
-        return pay32[beg[i] + j].astype(np.uint64)
+        body = j < 4 * Ks[i]
+        p = np.where(body, base[i] + (j // 4) * 4 * S + j % 4, tb[i] + j - 4 * Ks[i])
+        return pay32[p].astype(np.uint64)
 
     return _decode_quad(rows, cols, S, np.asarray(cdf, np.int64), min_value, word)
 

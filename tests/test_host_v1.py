@@ -222,11 +222,12 @@ def test_lanes_container_round_trip_with_outliers(tmp_path):
 
 def test_lanes_reader_rejects_bad_offsets(tmp_path):
     c = CompressedMatrixV1.encode(_mats()[0], 32, 8, 0, coder=2)
-    for name, edit in [("decreasing", lambda pay, cur: pay.__setitem__(cur + 3, 0)),
-                       ("overrun", lambda pay, cur: pay.__setitem__(cur + 31, 0xFFFF)),
-                       ("state", lambda pay, cur: pay.__setitem__(cur + 1, pay[cur] + 1))]:
+    t = 34   # row 1's table: K (two u16), then 32 tail end offsets
+    for name, edit in [("decreasing", lambda pay: pay.__setitem__(t + 2 + 3, 0)),
+                       ("overrun", lambda pay: pay.__setitem__(t + 2 + 31, 0xFFFF)),
+                       ("granules", lambda pay: pay.__setitem__(t, pay[t] + 50))]:
         pay = c.payload.copy()
-        edit(pay, int(c.cursors[1]))
+        edit(pay)
         bad = CompressedMatrixV1(c.rows, c.cols, 32, 8, 0, c.min_value, c.G, c.cursors, pay,
                                  cdf=c.cdf, coder=2)
         p = tmp_path / f"{name}.mvx"
@@ -236,11 +237,18 @@ def test_lanes_reader_rejects_bad_offsets(tmp_path):
         assert e.value.status == 4, name
 
 
-def test_lanes_row_beyond_16bit_offsets_is_refused():
-    m = np.random.default_rng(3).integers(-128, 128, (1, 1 << 19)).astype(np.int8)
+def test_lanes_tails_beyond_16bit_offsets_are_refused():
+    """The tails' end offsets are u16: a row whose stream 0 is much shorter than the others
+    (its columns constant, the rest uniform) leaves > 65535 tail words and is refused."""
+    n = 1 << 21
+    m = np.random.default_rng(3).integers(-64, 64, (1, n)).astype(np.int8)
+    m[0, (np.arange(n) % 512) < 16] = 0            # stream 0's columns
     with pytest.raises(MatvansError) as e:
         CompressedMatrixV1.encode(m, 32, 8, 0, coder=2)
     assert e.value.status == 2
+    ok = CompressedMatrixV1.encode(m[:, : 1 << 16], 32, 8, 0, coder=2)   # short rows fit
+    assert np.array_equal(E.RecordV1(1, 1 << 16, 32, 8, 0, ok.min_value, ok.G, ok.cursors,
+                                     ok.payload, cdf=ok.cdf, coder=2).decode(), m[:, : 1 << 16])
 
 
 def test_lanes_need_one_table_and_8_bits():

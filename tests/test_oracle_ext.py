@@ -225,30 +225,34 @@ def test_lanes_round_trip(S, name):
     if m.shape[1] == 0:
         pytest.skip("narrower than one group of the stream count")
     r = E.encode_record(m, S, 8, 0, coder=2)
-    assert r.cursors[0] == 0 and np.all(r.cursors % 8 == 0)
+    assert r.cursors[0] == E.lanes_table_u16(m.shape[0], S) and np.all(r.cursors % 8 == 0)
     assert np.array_equal(r.decode(), m)
 
 
 @pytest.mark.parametrize("S", [32, 64])
 def test_lanes_hold_the_quad_streams(S):
     """Coder 2 stores exactly coder 1's streams: stream l's words gathered with coder 1's
-    granule formula equal the words between coder 2's end offsets, and a textbook decode of
-    those words gives the stream's columns."""
+    granule formula equal those gathered from coder 2's body granules and tail, and a
+    textbook decode of those words gives the stream's columns."""
     m = CASES["laplace4"][:, :2048]
     r1 = E.encode_record(m, S, 8, 0, coder=1)
     r2 = E.encode_record(m, S, 8, 0, coder=2)
-    p1, p2 = r1.payload.view(np.uint32), r2.payload.view(np.uint32)
+    p1 = r1.payload.view(np.uint32)
     G4 = 128 // S
+    assert r2.cursors[0] == E.lanes_table_u16(m.shape[0], S)
     for row in range(m.shape[0]):
-        beg, cnt = E.lane_ranges(r2.payload, int(r2.cursors[row]), S)
+        K, _, _ = E.lanes_row(r2.payload, r2.cursors, m.shape[0], S, row)
+        lens = []
         for lane in range(S):
-            w2 = [int(v) for v in p2[beg[lane]:beg[lane] + cnt[lane]]]
+            w2 = E.lane_words(r2.payload, r2.cursors, m.shape[0], S, row, lane)
             w1 = [int(p1[int(r1.cursors[row]) // 2 + 128 * (j // G4) + G4 * lane + j % G4])
                   for j in range(len(w2))]
             assert w1 == w2
+            lens.append(len(w2))
             syms = E.decode_stream_quad_textbook(w2, m.shape[1] // S // 4, r2.cdf)
             want = m[row, E.stream_columns(m.shape[1], S, lane)].astype(np.int64) - r2.min_value
             assert syms == want.tolist()
+        assert K == min(lens) // 4
 
 
 def test_lanes_smaller_than_granules():
@@ -266,7 +270,7 @@ def test_lanes_decoder_catches_a_short_stream():
     m = CASES["gauss6"][:, :1024]
     r = E.encode_record(m, 32, 8, 0, coder=2)
     pay = r.payload.copy()
-    pay[int(r.cursors[2]) + 5] -= 1
+    pay[2 * 34 + 2 + 5] -= 1   # row 2's tail end offset of stream 5
     bad = E.RecordV1(r.rows, r.cols, 32, 8, 0, r.min_value, r.G, r.cursors, pay, cdf=r.cdf,
                      coder=2)
     try:
