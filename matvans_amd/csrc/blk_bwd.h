@@ -75,7 +75,7 @@ constexpr int bwd_set_words() {
 
 template <int NV, int S, int MODE = 0, int SETW = bwd_set_words<S, MODE>(), int W = kBwdWaves>
 struct QuadBwdSmem {
-    alignas(16) uint32_t tab[MODE == 0 ? 256 : 4];
+    alignas(16) uint2 tab[MODE == 0 ? 256 : 2];   // two-word table (f | sym << 24, r)
     alignas(16) uint32_t wstage[W * 256];
     uint32_t ring[W * v1::QuadLane<128 / S, SETW>::R4 * 64];
     uint32_t rowtab[MODE == 1 ? 64 * 256 : 1];     // one packed table per row
@@ -373,12 +373,11 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
     }
     BwdRows<NV> R;
     R.init(J, rb, lane);
-    // packed table T[q] = f | r << 8 | sym << 24: from the A / B tables (A = f | sym << 24,
-    // B = r), or one per row of the unit, lane l reading row l's
-    const uint32_t *tab;
+    // one table per matrix: the two-word entries (f | sym << 24, r) of the A / B tables;
+    // one per row: packed T[q] = f | r << 8 | sym << 24, lane l reading row l's
+    const uint32_t *tab = nullptr;
     if constexpr (MODE == 0) {
-        if (fresh && tid < 256) sm.tab[tid] = (ga & 0xFF0000FFu) | ((gb & 0xFFu) << 8);
-        tab = sm.tab;
+        if (fresh && tid < 256) sm.tab[tid] = make_uint2(ga, gb);
     } else {
         __syncthreads();   // every wave of the previous unit is done with the tables
         uint8_t *scratch = reinterpret_cast<uint8_t *>(sm.rowscratch + wave * 192);
@@ -422,7 +421,10 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
     auto group = [&](uint32_t t) {
         uint32_t P[4];
 #pragma unroll
-        for (int d = 0; d < 4; d++) P[d] = st.quad(tab);
+        for (int d = 0; d < 4; d++) {
+            if constexpr (MODE == 0) P[d] = st.quad(sm.tab);
+            else P[d] = st.quad(tab);
+        }
         if (prev) pipe.mma(gd, b);
         b = GroupPipe<NV>::stage(P, lane, stage, tr_off);
         if (prev) pipe.finish(gd, prev);

@@ -213,7 +213,7 @@ template <int NV, int S, int MODE = 0, int SETW = 4>
 struct QuadSmem {
     static constexpr int WPR = S > 64 ? S / 64 : 1;
     uint32_t ring[kWaves * 4 * SETW * 64];
-    uint32_t tab[MODE == 0 ? 256 : 1];
+    uint2 tab[MODE == 0 ? 256 : 1];                  // two-word table (f | sym << 24, r)
     uint32_t rowtab[MODE == 1 ? kWaves * 448 : 1];   // per wave: packed table + scratch
     uint32_t dummy[SETW * 64];
     int32_t part[WPR > 1 ? kWaves * NV : 1];
@@ -480,12 +480,12 @@ __device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid,
     }
     RowIndex<S, RPW, LAY> ix;
     ix.load(J, row0, row, sl, valid);
-    // packed table T[q] = f | r << 8 | sym << 24 (r = q - cdf[s]); f <= 255 and r < f for
-    // every table the encoder accepts
-    uint32_t *tab;
-    if constexpr (MODE == 0) {   // from the A / B tables (A = f | sym << 24, B = r)
-        tab = sm.tab;
-        tab[tid] = (ga & 0xFF0000FFu) | ((gb & 0xFFu) << 8);
+    // one table per matrix: the two-word entries (A, B) = (f | sym << 24, r) of
+    // matvans_v1_lut; one per row: packed T[q] = f | r << 8 | sym << 24 (r = q - cdf[s];
+    // f <= 255 and r < f for every table the encoder accepts)
+    uint32_t *tab = nullptr;
+    if constexpr (MODE == 0) {
+        sm.tab[tid] = make_uint2(ga, gb);
     } else {                     // this wave's row's own table
         tab = sm.rowtab + wave * 448;
         build_row_table(J.freq, row, J.G, valid, J.min_value, lane,
@@ -541,7 +541,9 @@ __device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid,
     auto group = [&](const uint32_t (&vc)[NV][4]) {
 #pragma unroll
         for (int d = 0; d < 4; d++) {
-            uint32_t packed = st.quad(tab);
+            uint32_t packed;
+            if constexpr (MODE == 0) packed = st.quad(sm.tab);
+            else packed = st.quad(tab);
 #pragma unroll
             for (int v = 0; v < NV; v++)
                 acc[v] = __builtin_amdgcn_sdot4(static_cast<int>(packed),
@@ -649,7 +651,7 @@ template <int MW>
 struct QuadMfmaSmem {
     static constexpr int kRows = 2 * MW;
     uint32_t ring[MW * 16 * 64];
-    uint32_t tab[256];
+    uint2 tab[256];                          // two-word table (f | sym << 24, r)
     uint32_t dummy[4 * 64];
     uint32_t stage[2][16 * kStageRow / 4];   // 16 operand rows; also the tiles' sum at the end
     int32_t outl[kRows][8];                  // outlier corrections per row and vector
@@ -677,12 +679,12 @@ __device__ __forceinline__ void quad_block_mfma(const V1Job &J, uint32_t bid,
     const uint32_t row0 = bid * kRows + 2 * wave;
     const uint32_t row = bid * kRows + rloc;
     const bool valid = row < rows;
-    uint32_t *tab = sm.tab;
+    uint2 *tab = sm.tab;
     // the decode table's entries and every index, in one round trip
     const uint32_t ga = glut[tid & 255u], gb = glut[256u + (tid & 255u)];
     RowIndex<S, 2, LAY> ix;
     ix.load(J, row0, row, sl, valid);
-    if (tid < 256) tab[tid] = (ga & 0xFF0000FFu) | ((gb & 0xFFu) << 8);
+    if (tid < 256) tab[tid] = make_uint2(ga, gb);
     if constexpr (MW == 4) {   // operand rows 8..15 of both stages stay zero
         uint32_t *z0 = sm.stage[0] + 8 * kStageRow / 4, *z1 = sm.stage[1] + 8 * kStageRow / 4;
         for (uint32_t k = tid; k < 8 * kStageRow / 4; k += 64 * MW) z0[k] = z1[k] = 0;
