@@ -41,25 +41,27 @@ struct RowMajorSetup {
     bool valid;
 };
 
+// The four cursor loads take clamped indices instead of branches, so they issue back to
+// back (one memory round trip).
 __device__ __forceinline__ RowMajorSetup row_major_setup(uint32_t row0, uint32_t lane,
                                                          uint32_t rows,
                                                          const uint32_t *__restrict__ cursors,
                                                          uint32_t payload_size) {
     RowMajorSetup s;
+    const uint32_t rmax = rows - 1u;
     s.row = row0 + (lane >> 5);
-    uint32_t wbeg = 0, wend = 0;
-    if (row0 < rows) {
-        wbeg = cursors[row0];
-        uint32_t last = row0 + 1 < rows ? row0 + 1 : row0;
-        wend = last + 1 < rows ? cursors[last + 1] : payload_size;
-        if (wend > payload_size) wend = payload_size;
-        if (wend < wbeg) wend = wbeg;
-    }
+    s.valid = s.row < rows;
+    const uint32_t a = row0 < rows ? row0 : rmax, b = row0 + 2 < rows ? row0 + 2 : rmax;
+    const uint32_t r = s.valid ? s.row : rmax, r1 = s.row + 1 < rows ? s.row + 1 : rmax;
+    const uint32_t ca = cursors[a], cb = cursors[b], cr = cursors[r], cr1 = cursors[r1];
+    uint32_t wbeg = row0 < rows ? ca : 0u;
+    uint32_t wend = row0 < rows ? (row0 + 2 < rows ? cb : payload_size) : 0u;
+    if (wend > payload_size) wend = payload_size;
+    if (wend < wbeg) wend = wbeg;
     s.win_beg = __builtin_amdgcn_readfirstlane(wbeg);
     s.win_end = __builtin_amdgcn_readfirstlane(wend);
-    s.valid = s.row < rows;
-    s.row_cur = s.valid ? cursors[s.row] : 0u;
-    s.row_end = s.valid ? (s.row + 1 < rows ? cursors[s.row + 1] : payload_size) : 0u;
+    s.row_cur = s.valid ? cr : 0u;
+    s.row_end = s.valid ? (s.row + 1 < rows ? cr1 : payload_size) : 0u;
     return s;
 }
 
@@ -83,12 +85,24 @@ __global__ __launch_bounds__(256) void v0_fwd(int32_t *__restrict__ dst,
     __shared__ uint16_t ring[kWaves * kRingWords * 64];
     __shared__ uint32_t lut[512];
     __shared__ uint16_t dummy[kDummyBytes / 2];
+    __shared__ uint32_t cdfs[257];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid >> 6, lane = tid & 63, sl = lane & 31;
+    // every global load of the prologue at once: this thread's quantile symbol ppf[tid],
+    // cdf entry tid, and the rows' cursors; the table's dependent cdf[s] reads then come
+    // from LDS (zero past G, as the reference reads them, matvansWarpPerRow-kernel.cu:86)
+    const uint32_t s_q = ppf[tid];
+    const uint32_t c_t = cdf[tid <= G ? tid : G];
+    RowMajorSetup rs = row_major_setup(blockIdx.x * kRowsPerBlock + wave * 2, lane, rows,
+                                       cursors, payload_size);
+    cdfs[tid] = tid <= G ? c_t : 0u;
+    if (tid == 0) cdfs[256] = 0u;   // G <= 255
+    __syncthreads();
     {
-        uint32_t f, sym, bias;
-        v0_table_entry(tid, min_value, G, cdf, ppf, f, sym, bias);
+        const uint32_t c0 = cdfs[s_q], c1 = cdfs[s_q + 1];
+        const uint32_t f = (c1 - c0) & 0xFFu, bias = (tid - c0) & 0xFFu;
+        const uint32_t sym = static_cast<uint8_t>(min_value + static_cast<int>(s_q));
         if (TABLE == 0) {
             lut[tid] = f | (sym << 24);
             lut[tid + 256] = bias;
@@ -101,8 +115,6 @@ __global__ __launch_bounds__(256) void v0_fwd(int32_t *__restrict__ dst,
             tab[512 + tid] = static_cast<uint8_t>(sym);
         }
     }
-    RowMajorSetup rs = row_major_setup(blockIdx.x * kRowsPerBlock + wave * 2, lane, rows,
-                                       cursors, payload_size);
     V0Lane st;
     st.init(payload, rs.win_beg, rs.win_end, rs.valid, rs.row_cur, rs.row_end, sl, ring, dummy,
             wave, lane);
