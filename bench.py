@@ -435,7 +435,8 @@ def bench_uneven(dev, steps, warmup, scale=1.0, n_mat=32):
 
 
 def bench_shapes(dev, steps, warmup, nvecs=(1, 4, 8)):
-    """The other LLM shapes (4096x4096, 14336x4096; Laplace(0,4)): q32 and .bin v0 fused
+    """The other LLM shapes (4096x4096, 14336x4096; Laplace(0,4)): q32, .bin v0 and the
+    outlier-split quad encodings (q32x, l32x, l64x: the stored-size end) fused
     decode-matvec against the int8 and fp32 comparators, one vector; and the backward
     dx = W^T (s_w * g) at 14336x4096 and 4096x14336 (sigma=6, g ~ N(0,1)) for 1, 4 and 8
     gradient vectors, one-matrix launches, q32."""
@@ -448,9 +449,11 @@ def bench_shapes(dev, steps, warmup, nvecs=(1, 4, 8)):
         d_x = torch.from_numpy(x[None]).to(dev)
         want = exact_product(w, x[None])
         r = {}
-        for fmt in ("q32", "v0"):
-            if fmt == "q32":
-                c = CompressedMatrixV1.encode(w, 32, 8, 0, coder=1)
+        for fmt in ("q32", "v0", "q32x", "l32x", "l64x"):
+            if fmt != "v0":
+                S, coder = int(fmt[1:3]), 2 if fmt[0] == "l" else 1
+                c = CompressedMatrixV1.encode(w, S, 8, 0, coder=coder,
+                                              outliers="auto" if fmt.endswith("x") else None)
                 stored, make = c.stored_bytes, _v1_copy
             else:
                 c = mv.CompressedMatrix.encode(w)
