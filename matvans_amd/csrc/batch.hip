@@ -79,15 +79,17 @@ union BatchSmem {
     Opt<(MASK & bit(kPathL64)) != 0, blk::QuadSmem<NV, 64>> l64;
     Opt<(MASK & bit(kPathL32M)) != 0, blk::QuadMfmaSmem<4>> l32m;
 };
-// kernel instances: the decode paths each one contains (a plan picks the smallest one
-// that covers its items; fewer paths leave fewer registers live)
+// kernel instances: the decode paths each one contains (a plan picks the first one that
+// covers its items; fewer paths leave fewer registers live: the all-paths instance keeps
+// coder 1, v0 and family B in 0x7F, since the coder-2 paths' registers cost a batch of
+// family-B stripes, which needs every resident wave, 670 -> 1109 us)
 constexpr uint32_t kQuad = bit(kPathQ32) | bit(kPathQ64) | bit(kPathQ128);
 constexpr uint32_t kLanes = bit(kPathL32) | bit(kPathL64);
 constexpr uint32_t kMasks[] = {bit(kPathV0), bit(kPathQ32), bit(kPathQ64), bit(kPathQ128),
                                bit(kPathQ32) | bit(kPathQ64), kQuad,
                                bit(kPathBRows) | bit(kPathBStripes), bit(kPathQ32M),
                                kQuad | bit(kPathQ32M), bit(kPathL32), bit(kPathL64), kLanes,
-                               bit(kPathL32M), kLanes | bit(kPathL32M), 0x3FFu};
+                               bit(kPathL32M), kLanes | bit(kPathL32M), 0x7Fu, 0x3FFu};
 constexpr uint32_t kNumMasks = sizeof(kMasks) / sizeof(kMasks[0]);
 
 // work-queue shards: block b draws from shard b % kShards, whose tiles are t = g, g + 8,
@@ -400,9 +402,9 @@ FwdKernel fwd_kernel(const matvans_batch *p) {
                     mv::batch_fwd<NV, mv::kMasks[8]>, mv::batch_fwd<NV, mv::kMasks[9]>, \
                     mv::batch_fwd<NV, mv::kMasks[10]>, mv::batch_fwd<NV, mv::kMasks[11]>, \
                     mv::batch_fwd<NV, mv::kMasks[12]>, mv::batch_fwd<NV, mv::kMasks[13]>, \
-                    mv::batch_fwd<NV, mv::kMasks[14]>}
-    static_assert(mv::kNumMasks == 15, "kernel table");
-    static const FwdKernel table[4][15] = {MV_ROW(1), MV_ROW(2), MV_ROW(4), MV_ROW(8)};
+                    mv::batch_fwd<NV, mv::kMasks[14]>, mv::batch_fwd<NV, mv::kMasks[15]>}
+    static_assert(mv::kNumMasks == 16, "kernel table");
+    static const FwdKernel table[4][16] = {MV_ROW(1), MV_ROW(2), MV_ROW(4), MV_ROW(8)};
 #undef MV_ROW
     return table[nv_row(p->nv)][p->kernel];
 }
