@@ -369,9 +369,9 @@ int matvans_v1_decompress_and_multiply(const matvans_v1_dev_matrix *m, int32_t *
                                        const int8_t *vecs, uint32_t n_vec,
                                        uint32_t vec_stride, void *stream);
 
-/* N3 on the quad coder: the same backward from an MVX1 matrix with coder 1 and 8-bit
- * tables, either one table per matrix (32, 64 or 128 streams) or one per row (64 streams;
- * other encodings: MATVANS_E_UNSUPPORTED):
+/* N3 on the quad coder: the same backward from an MVX1 matrix with coder 1 or 2 and 8-bit
+ * tables, either one table per matrix (32, 64 or 128 streams) or, coder 1, one per row
+ * (64 streams; other encodings: MATVANS_E_UNSUPPORTED):
  *     dx[v*cols + c] = alpha[v] * sum_r W[r,c] * s_w[r] * g[v*g_stride + r]
  * Workspace: matvans_backward_workspace_size(rows, cols, n_vec) bytes, 16-byte aligned.
  * Bitwise reproducible.  No reference counterpart. */
@@ -530,7 +530,8 @@ int matvans_batch_create(const matvans_batch_item *items, uint32_t n, matvans_ba
  * launches (a persistent launch over (64-row block, 8-stream group) units from the same
  * sharded work queue, then one fixed-order reduce over every item's columns).  Results
  * are bitwise identical to matvans_decompress_backward / matvans_v1_decompress_backward.
- * The plan owns the partial-sum workspace.  .bin v0 and quad (32 / 64 streams) items.
+ * The plan owns the partial-sum workspace.  .bin v0 and quad (coder 1 or 2, 32 / 64
+ * streams) items.
  * No reference counterpart. */
 typedef struct {
     uint32_t format;             /* MATVANS_BATCH_V0, MATVANS_BATCH_QUAD or _QUAD_LANES */
