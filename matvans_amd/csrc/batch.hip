@@ -360,6 +360,9 @@ int check_item(const matvans_batch_item &it, uint32_t k) {
             if (it.cols % (16 * it.streams)) return fail(MATVANS_E_SHAPE, "item %u: cols", k);
             if (!it.cursors || !it.lut || !it.payload)
                 return fail(MATVANS_E_INVALID, "item %u: null quad arrays", k);
+            if (it.format == MATVANS_BATCH_QUAD_LANES &&
+                !mv::lanes_tables_fit(it.rows, it.streams, it.payload_size))
+                return fail(MATVANS_E_SHAPE, "item %u: payload shorter than the row tables", k);
             if (!it.out_start != !it.out_rec)
                 return fail(MATVANS_E_INVALID, "item %u: outlier offsets and records go together",
                             k);
@@ -554,6 +557,9 @@ int check_bwd_item(const matvans_batch_bwd_item &it, uint32_t k) {
             return fail(MATVANS_E_UNSUPPORTED, "item %u: quad items need 32 or 64 streams", k);
         if (it.cols % (16 * it.streams)) return fail(MATVANS_E_SHAPE, "item %u: cols", k);
         if (!it.lut) return fail(MATVANS_E_INVALID, "item %u: null decode table", k);
+        if (it.format == MATVANS_BATCH_QUAD_LANES &&
+            !mv::lanes_tables_fit(it.rows, it.streams, it.payload_size))
+            return fail(MATVANS_E_SHAPE, "item %u: payload shorter than the row tables", k);
     } else {
         return fail(MATVANS_E_UNSUPPORTED, "item %u: the backward takes .bin v0 and quad items", k);
     }

@@ -138,6 +138,9 @@ extern "C" int matvans_v1_decompress_and_multiply(const matvans_v1_dev_matrix *m
         return fail(MATVANS_E_UNSUPPORTED, "the quad coder needs 8-bit probabilities");
     if (m->coder == 2 && m->table_mode != 0)
         return fail(MATVANS_E_UNSUPPORTED, "coder 2 has one table per matrix");
+    if (m->coder == 2 && !mv::lanes_tables_fit(m->rows, m->streams, m->payload_size))
+        return fail(MATVANS_E_SHAPE, "payload_size %u is shorter than coder 2's row tables",
+                    m->payload_size);
     if (m->table_mode == 0 && !m->lut) return fail(MATVANS_E_INVALID, "null decode table");
     if (m->table_mode == 1 && (!m->freq || m->G < 2 || m->G > 256))
         return fail(MATVANS_E_INVALID, "per-row tables need freq and 2 <= G <= 256");
@@ -320,6 +323,9 @@ int check_v1_backward(const matvans_v1_dev_matrix *m, const float *dx, const flo
                     "64 or 128 streams, or one per row with 64 (coder 1)");
     if (!m->cursors || !m->payload || (m->table_mode == 0 && !m->lut))
         return fail(MATVANS_E_INVALID, "null device pointer");
+    if (m->coder == 2 && !mv::lanes_tables_fit(m->rows, m->streams, m->payload_size))
+        return fail(MATVANS_E_SHAPE, "payload_size %u is shorter than coder 2's row tables",
+                    m->payload_size);
     if (m->table_mode == 1 && (!m->freq || m->G < 2 || m->G > 256))
         return fail(MATVANS_E_INVALID, "per-row tables need freq and 2 <= G <= 256");
     if (n_vec < 1 || n_vec > 8) return fail(MATVANS_E_INVALID, "n_vec %u outside 1..8", n_vec);

@@ -79,6 +79,11 @@ size_t v0_bwd_workspace_bytes(uint32_t rows, uint32_t cols, uint32_t n_vec);
 inline bool bwd_partials_fit(uint32_t rows, uint32_t cols) {
     return 8ull * 4ull * ((rows + 63ull) / 64ull) * cols < (1ull << 32);
 }
+// Coder 2 keeps each row's table (S + 2 u16 words) at the payload head; the kernels read
+// it by row index, so the payload must hold all of them.
+inline bool lanes_tables_fit(uint32_t rows, uint32_t streams, uint32_t payload_size) {
+    return uint64_t(rows) * (uint64_t(streams) + 2u) <= payload_size;
+}
 hipError_t launch_v0_backward(const V0BwdArgs &a, hipStream_t s);
 
 struct QuadBwdArgs {

@@ -7,7 +7,7 @@ import pytest
 
 from matvans_amd.capi import BatchBwdItem, BatchItem, MatvansError, call
 
-V0, QUAD, B_ROWS, B_STRIPES = 0, 1, 2, 3
+V0, QUAD, B_ROWS, B_STRIPES, LANES = 0, 1, 2, 3, 4
 
 
 def fwd_item(**kw):
@@ -53,6 +53,8 @@ def create(fn, items):
     ({"format": V0, "cdf": 0}, 1),           # .bin v0 tables missing
     ({"format": B_STRIPES, "rows": 40}, 2),  # stripes: rows % 32
     ({"format": 7}, 1),                      # unknown format
+    ({"format": LANES, "streams": 128}, 8),  # coder 2: 32 or 64 streams
+    ({"format": LANES, "payload_size": 64 * 34 - 1}, 2),   # payload shorter than row tables
 ])
 def test_forward_plan_rejects(kw, status):
     assert create("matvans_batch_create", [fwd_item(), fwd_item(**kw)]) == status
@@ -68,6 +70,7 @@ def test_forward_plan_rejects(kw, status):
     ({"payload": 24}, 3),
     ({"format": V0, "cdf": 0}, 1),
     ({"format": B_ROWS}, 8),                 # family-B items have no backward
+    ({"format": LANES, "payload_size": 64 * 34 - 1}, 2),   # payload shorter than row tables
 ])
 def test_backward_plan_rejects(kw, status):
     assert create("matvans_batch_create_backward", [bwd_item(), bwd_item(**kw)]) == status

@@ -90,3 +90,22 @@ def test_backward_refuses_partial_sums_beyond_4gib():
     st = L.matvans_decompress_backward(fake, fake, 1, rows, fake, fake, rows, cols, fake, 0, 8,
                                        fake, fake, fake, 1 << 20, fake, 1 << 40, None)
     assert st == 2 and "4 GiB" in capi.last_error()
+
+
+def test_coder2_payload_must_hold_the_row_tables():
+    """Coder 2 reads each row's table (streams + 2 u16 words) from the payload head by row
+    index; every entry point refuses a payload_size shorter than those tables before any
+    device call (the pointers below are never dereferenced)."""
+    L = matvans_amd.lib()
+    fake = ctypes.c_void_p(256)
+    rows, cols, S = 64, 1024, 32
+    m = capi.V1DevMatrix(rows=rows, cols=cols, streams=S, precision=8, table_mode=0, coder=2,
+                         payload_size=rows * (S + 2) - 1, cursors=fake, lut=fake, payload=fake)
+    st = L.matvans_v1_decompress_and_multiply(ctypes.byref(m), fake, fake, 1, cols, None)
+    assert st == 2 and "row tables" in capi.last_error()
+    st = L.matvans_v1_decompress_backward(ctypes.byref(m), fake, fake, 1, rows, fake, fake,
+                                          fake, 1 << 30, None)
+    assert st == 2 and "row tables" in capi.last_error()
+    st = L.matvans_v1_decompress_backward_noalloc(ctypes.byref(m), fake, fake, 1, rows, fake,
+                                                  fake, None)
+    assert st == 2 and "row tables" in capi.last_error()
