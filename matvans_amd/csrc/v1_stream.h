@@ -357,9 +357,17 @@ struct QuadLane {
         sa = sb = 0xFFFFFFFFu;
     }
 
-    // byte offset of chunk c within the window
+    // byte offset of chunk c within the window.  LAY 1 computes both candidates and picks
+    // one with a select: written as a conditional expression, the compiler put each arm
+    // behind a branch whose address register was the destination of a load in flight, and
+    // waited for every outstanding load (the prefetched sets too) at each set issue.
     __device__ __forceinline__ uint32_t chunk_off(uint32_t c) const {
-        if constexpr (LAY == 1) return c < Kb ? lane_off + c * gstride : tail_off + (c - Kb) * 16u;
+        if constexpr (LAY == 1) {
+            const uint32_t ob = lane_off + c * gstride;
+            const uint32_t ot = tail_off + ((c - Kb) << 4);   // wraps for c < Kb: unused
+            const uint32_t m = 0u - uint32_t(c < Kb);
+            return (ob & m) | (ot & ~m);
+        }
         return lane_off + c * kStride;
     }
 
@@ -370,8 +378,10 @@ struct QuadLane {
     // loads the NG chunks of set `set` (or, when !ok, out-of-range zeros)
     __device__ __forceinline__ void load_set(Chunk<2 * CW> (&p)[NG], uint32_t set, bool ok) {
 #pragma unroll
-        for (int i = 0; i < NG; i++)
-            p[i] = load_chunk<2 * CW>(rsrc, ok ? chunk_off(set * NG + uint32_t(i)) : kOOB);
+        for (int i = 0; i < NG; i++) {
+            const uint32_t off = chunk_off(set * NG + uint32_t(i));   // computed either way
+            p[i] = load_chunk<2 * CW>(rsrc, ok ? off : kOOB);
+        }
     }
 
     __device__ __forceinline__ void write_set(const Chunk<2 * CW> (&p)[NG], uint32_t set) {
