@@ -227,20 +227,35 @@ struct GradDigits {
 // and issues its transposed reads, and finish() scales and stores group t - 1's sums.  A
 // group's sums 2^24 D0 + 2^16 D1 + 2^8 D2 + D3 are scaled as
 // 2^16 u (2^8 D0 + D1) + u (2^8 D2 + D3), both pairs exact in int32 (|D| <= 2^20).
-// finish() stores vector v's 16 column sums to out + v * plane + 0 .. 15: out is
-// wave-uniform, the lane's offset fixed (loff).
+// finish() stores vector v's 16 column sums to byte goff + 4 (v * plane + 0 .. 15) of the
+// job's partial sums: goff is wave-uniform, the lane's offset fixed (loff).  Every lane
+// issues the store, through a buffer resource over the partial sums (below 4 GiB, the
+// entry points check); lanes without a sum (vector n_vec and up) give an offset past
+// its end, which the hardware drops.  Behind a per-lane branch the store would make the
+// compiler count it as possibly absent and wait for the other ring set's loads as well.
+// The partial sums end at or below byte 2^32 - 32 (n_vec <= 8 planes, each below 2^29
+// bytes), so every offset from kDropOff - 31 up is dropped.
+constexpr uint32_t kDropOff = 0xFFFFFFFFu;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t partial_rsrc(const BwdJob &J, uint32_t nrb) {
+    const uint64_t bytes = uint64_t(J.n_vec) * nrb * J.cols * sizeof(float);
+    return __builtin_amdgcn_make_buffer_rsrc(J.partial, 0, static_cast<int>(uint32_t(bytes)),
+                                             0x00020000);
+}
+
 template <int NV>
 struct GroupPipe {
     static constexpr int NA = GradDigits<NV>::NA;
     mv_i32x4 d[NA];
     uint32_t loff[NA];   // byte offset 4 ((4 h + k) * plane + i) for lane 16 k + i
-    bool on[NA];         // 4 h + k < n_vec
+    uint32_t drop[NA];   // ~0 when 4 h + k >= n_vec, else 0
     __device__ __forceinline__ void init(uint32_t lane, size_t plane, uint32_t n_vec) {
         const uint32_t i = lane & 15u, k = lane >> 4;
 #pragma unroll
         for (int h = 0; h < NA; h++) {
+            d[h] = mv_i32x4{0, 0, 0, 0};
             loff[h] = uint32_t(((4u * h + k) * plane + i) * sizeof(float));
-            on[h] = 4u * h + k < n_vec;
+            drop[h] = 4u * h + k < n_vec ? 0u : kDropOff;
         }
     }
     __device__ __forceinline__ static mv_i32x4 stage(const uint32_t (&P)[4], uint32_t lane,
@@ -254,13 +269,24 @@ struct GroupPipe {
         for (int h = 0; h < NA; h++)
             d[h] = __builtin_amdgcn_mfma_i32_16x16x64_i8(gd.a[h], b, zero, 0, 0, 0);
     }
-    __device__ __forceinline__ void finish(const GradDigits<NV> &gd, float *out) const {
+    // NA stores that the hardware drops, at offsets kDropOff - 16 j - 4 h (distinct, so
+    // that the compiler keeps each)
+    __device__ __forceinline__ void drop_stores(__amdgpu_buffer_rsrc_t rs, uint32_t j) const {
+#pragma unroll
+        for (int h = 0; h < NA; h++)
+            __builtin_amdgcn_raw_buffer_store_b32(
+                0u, rs, static_cast<int>(kDropOff - 16u * j - 4u * uint32_t(h)), 0, 0);
+    }
+    __device__ __forceinline__ void finish(const GradDigits<NV> &gd, __amdgpu_buffer_rsrc_t rs,
+                                           uint32_t goff) const {
 #pragma unroll
         for (int h = 0; h < NA; h++) {
             const int32_t hi = int32_t(uint32_t(d[h][0]) << 8) + d[h][1];
             const int32_t lo = int32_t(uint32_t(d[h][2]) << 8) + d[h][3];
             const float t = fmaf(float(hi), gd.uh[h], float(lo) * gd.ul[h]);
-            if (on[h]) *reinterpret_cast<float *>(reinterpret_cast<char *>(out) + loff[h]) = t;
+            const uint32_t off = goff == kDropOff ? kDropOff : (goff + loff[h]) | drop[h];
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(t), rs, static_cast<int>(off),
+                                                  0, 0);
         }
     }
 };
@@ -295,7 +321,9 @@ __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32
     const uint32_t cols = J.cols;
     const uint32_t groups = cols / 512u;
     const size_t plane = size_t(nrb) * cols;
-    float *base = J.partial + size_t(rb) * cols + stream * 16u;
+    // byte offsets into the partial sums (below 4 GiB): wave-uniform
+    const uint32_t base = (rb * cols + stream * 16u) * uint32_t(sizeof(float));
+    const __amdgpu_buffer_rsrc_t prs = partial_rsrc(J, nrb);
     // four symbols (byte 3 of each table entry) packed in column order
     auto next4 = [&]() {
         uint32_t e[4];
@@ -310,18 +338,26 @@ __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32
     gd.init(R.gs, lane, stage, tr_off);
     GroupPipe<NV> pipe;
     pipe.init(lane, plane, J.n_vec);
-    float *prev = nullptr;   // group t - 1's output (wave-uniform), its B operand in b
+    // group t - 1's output byte offset (wave-uniform; kDropOff before the first group),
+    // its B operand in b.  The products and the store are issued for every group, the
+    // first group's dropped, and two dropped sets of stores follow the ring's first loads:
+    // the loop then issues the same memory operations between a ring set's load and its
+    // use on every pass, and the compiler's wait for that set stops short of the other
+    // set's load (with a pass or a prologue that had fewer, it waited for both).
+    uint32_t prev = kDropOff;
     mv_i32x4 b = {0, 0, 0, 0};
+    pipe.drop_stores(prs, 0);
+    pipe.drop_stores(prs, 1);
     auto group = [&](uint32_t t) {
         uint32_t wbefore = st.waddr;
         uint32_t P[4];
 #pragma unroll
         for (int d = 0; d < 4; d++) P[d] = next4();
         st.end_group(wbefore);
-        if (prev) pipe.mma(gd, b);
+        pipe.mma(gd, b);
         b = GroupPipe<NV>::stage(P, lane, stage, tr_off);
-        if (prev) pipe.finish(gd, prev);
-        prev = base + t * 512u;
+        pipe.finish(gd, prs, prev);
+        prev = base + t * 512u * uint32_t(sizeof(float));
     };
     uint32_t t = 0;
     for (; t + 1 < groups; t += 2) {
@@ -334,10 +370,8 @@ __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32
         st.refill_a();
         group(t);
     }
-    if (prev) {
-        pipe.mma(gd, b);
-        pipe.finish(gd, prev);
-    }
+    pipe.mma(gd, b);
+    pipe.finish(gd, prs, prev);
 }
 
 // Quad-coder unit (S = 32, 64 or 128 streams): row block rb, streams W sg .. W sg + W - 1;
@@ -409,15 +443,25 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
     const uint32_t cols = J.cols;
     const uint32_t groups = cols / (16u * S);
     const size_t plane = size_t(nrb) * cols;
-    float *base = J.partial + size_t(rb) * cols + stream * 16u;
+    // byte offsets into the partial sums (below 4 GiB): wave-uniform
+    const uint32_t base = (rb * cols + stream * 16u) * uint32_t(sizeof(float));
+    const __amdgpu_buffer_rsrc_t prs = partial_rsrc(J, nrb);
     uint8_t *stage = reinterpret_cast<uint8_t *>(sm.wstage + wave * 256);
     const uint32_t tr_off = tr_offset(lane);
     GradDigits<NV> gd;
     gd.init(R.gs, lane, stage, tr_off);
     GroupPipe<NV> pipe;
     pipe.init(lane, plane, J.n_vec);
-    float *prev = nullptr;   // group t - 1's output (wave-uniform), its B operand in b
+    // group t - 1's output byte offset (wave-uniform; kDropOff before the first group),
+    // its B operand in b.  The products and the store are issued for every group, the
+    // first group's dropped, and two dropped sets of stores follow the ring's first loads:
+    // the loop then issues the same memory operations between a ring set's load and its
+    // use on every pass, and the compiler's wait for that set stops short of the other
+    // set's load (with a pass or a prologue that had fewer, it waited for both).
+    uint32_t prev = kDropOff;
     mv_i32x4 b = {0, 0, 0, 0};
+    pipe.drop_stores(prs, 0);
+    pipe.drop_stores(prs, 1);
     auto group = [&](uint32_t t) {
         uint32_t P[4];
 #pragma unroll
@@ -425,10 +469,10 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
             if constexpr (MODE == 0) P[d] = st.quad(sm.tab);
             else P[d] = st.quad(tab);
         }
-        if (prev) pipe.mma(gd, b);
+        pipe.mma(gd, b);
         b = GroupPipe<NV>::stage(P, lane, stage, tr_off);
-        if (prev) pipe.finish(gd, prev);
-        prev = base + t * (16u * S);
+        pipe.finish(gd, prs, prev);
+        prev = base + t * (16u * S) * uint32_t(sizeof(float));
     };
     // a set period: GPS groups after one refill, the consumed words counted once (at most
     // 4 * GPS < R4)
@@ -455,10 +499,8 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
             period(t + GPS, groups - t - GPS);
         }
     }
-    if (prev) {
-        pipe.mma(gd, b);
-        pipe.finish(gd, prev);
-    }
+    pipe.mma(gd, b);
+    pipe.finish(gd, prs, prev);
 }
 
 // dx[v][c] = alpha[v] * sum over row blocks of partial[v][b][c] for the 64 columns of
