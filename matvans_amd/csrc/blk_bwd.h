@@ -234,7 +234,7 @@ struct GradDigits {
 // its end, which the hardware drops.  Behind a per-lane branch the store would make the
 // compiler count it as possibly absent and wait for the other ring set's loads as well.
 // The partial sums end at or below byte 2^32 - 32 (n_vec <= 8 planes, each below 2^29
-// bytes), so every offset from kDropOff - 31 up is dropped.
+// bytes), so a store at kDropOff is dropped.
 constexpr uint32_t kDropOff = 0xFFFFFFFFu;
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t partial_rsrc(const BwdJob &J, uint32_t nrb) {
@@ -269,13 +269,15 @@ struct GroupPipe {
         for (int h = 0; h < NA; h++)
             d[h] = __builtin_amdgcn_mfma_i32_16x16x64_i8(gd.a[h], b, zero, 0, 0, 0);
     }
-    // NA stores that the hardware drops, at offsets kDropOff - 16 j - 4 h (distinct, so
-    // that the compiler keeps each)
-    __device__ __forceinline__ void drop_stores(__amdgpu_buffer_rsrc_t rs, uint32_t j) const {
+    // as many stores as finish() issues, into a buffer of zero bytes (the hardware drops
+    // them), at offsets 64 bytes apart (the compiler neither drops nor merges any)
+    __device__ __forceinline__ void drop_stores(const void *any, uint32_t j) const {
+        const __amdgpu_buffer_rsrc_t r =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(any), 0, 0, 0x00020000);
 #pragma unroll
         for (int h = 0; h < NA; h++)
             __builtin_amdgcn_raw_buffer_store_b32(
-                0u, rs, static_cast<int>(kDropOff - 16u * j - 4u * uint32_t(h)), 0, 0);
+                0u, r, static_cast<int>(64u * (uint32_t(NA) * j + uint32_t(h))), 0, 0);
     }
     __device__ __forceinline__ void finish(const GradDigits<NV> &gd, __amdgpu_buffer_rsrc_t rs,
                                            uint32_t goff) const {
@@ -346,8 +348,8 @@ __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32
     // set's load (with a pass or a prologue that had fewer, it waited for both).
     uint32_t prev = kDropOff;
     mv_i32x4 b = {0, 0, 0, 0};
-    pipe.drop_stores(prs, 0);
-    pipe.drop_stores(prs, 1);
+    pipe.drop_stores(J.partial, 0);
+    pipe.drop_stores(J.partial, 1);
     auto group = [&](uint32_t t) {
         uint32_t wbefore = st.waddr;
         uint32_t P[4];
@@ -454,14 +456,15 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
     pipe.init(lane, plane, J.n_vec);
     // group t - 1's output byte offset (wave-uniform; kDropOff before the first group),
     // its B operand in b.  The products and the store are issued for every group, the
-    // first group's dropped, and two dropped sets of stores follow the ring's first loads:
-    // the loop then issues the same memory operations between a ring set's load and its
-    // use on every pass, and the compiler's wait for that set stops short of the other
-    // set's load (with a pass or a prologue that had fewer, it waited for both).
+    // first group's dropped, and a pass's worth of dropped stores (two set periods of GPS
+    // groups) follows the ring's first loads: the loop then issues the same memory
+    // operations between a ring set's load and its use on every pass, and the compiler's
+    // wait for that set stops short of the other set's load (with a pass or a prologue
+    // that had fewer, it waited for both).
     uint32_t prev = kDropOff;
     mv_i32x4 b = {0, 0, 0, 0};
-    pipe.drop_stores(prs, 0);
-    pipe.drop_stores(prs, 1);
+#pragma unroll
+    for (uint32_t j = 0; j < 2u * GPS; j++) pipe.drop_stores(J.partial, j);
     auto group = [&](uint32_t t) {
         uint32_t P[4];
 #pragma unroll
