@@ -117,6 +117,9 @@ struct BwdRows {
         rend = valid ? (row + 1 < rows ? cr1 : ps) : 0u;
 #pragma unroll
         for (int v = 0; v < NV; v++) gs[v] = valid && v < int(J.n_vec) ? sw * gv[v] : 0.0f;
+        // consumed here: the compiler otherwise moves the row's cursor loads into the
+        // caller's branch for lanes with a row (a second round trip before the payload)
+        asm volatile("" ::"v"(rcur), "v"(rend));
     }
 };
 

@@ -191,6 +191,11 @@ struct RowIndex {
         oend = valid && outl ? o1 : 0u;
         ls = LaneStream{0, 0, 0, 0};
         if constexpr (LAY == 1) ls = lane_stream<S>(lt, rcur, valid);
+        // Consumed here: the compiler otherwise moves the loads that only lanes with a row
+        // use into the caller's branch for those lanes, behind the wait for the others (a
+        // second round trip before the payload).
+        asm volatile("" ::"v"(rcur), "v"(rend), "v"(obeg), "v"(oend));
+        if constexpr (LAY == 1) asm volatile("" ::"v"(lt.kb), "v"(lt.e0), "v"(lt.e1));
     }
 };
 
@@ -505,9 +510,6 @@ __device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid,
         st.init_lanes(payload, wbeg, wend, valid, ix.ls.body, ix.ls.kb, S, sl, ix.ls.tail,
                       ix.ls.tail_n, ring, dummy, wave, lane);
     }
-    st.prologue();
-    outl.records(J, sl);
-
     const uint32_t groups = cols / (16u * S);
     // the vectors through a buffer resource: the lane's byte offset is fixed, the group's
     // offset is wave-uniform (scalar), so a load costs no vector address arithmetic
@@ -528,9 +530,13 @@ __device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid,
     };
     uint32_t vc0[NV][4], vc1[NV][4];
     // load order vc0, granules A, granules B, vc1 as in the loop body (A, vc0, B, vc1), so
-    // that the loop's wait before group A leaves the two newest loads (B, vc1) in flight
-    vload(0, vc0);
-    __syncthreads();
+    // that the loop's wait before group A leaves the two newest loads (B, vc1) in flight;
+    // the barrier (the decode table) waits while the first ring sets are in flight
+    st.prologue([&] {
+        outl.records(J, sl);
+        vload(0, vc0);
+        __syncthreads();
+    });
     st.start();
     vload(1, vc1);
     outl.values(J, nv_eff);

@@ -62,6 +62,9 @@ __device__ __forceinline__ RowMajorSetup row_major_setup(uint32_t row0, uint32_t
     s.win_end = __builtin_amdgcn_readfirstlane(wend);
     s.row_cur = s.valid ? cr : 0u;
     s.row_end = s.valid ? (s.row + 1 < rows ? cr1 : payload_size) : 0u;
+    // consumed here: the compiler otherwise moves the row's cursor loads into the caller's
+    // branch for lanes with a row, behind the wait for the window's (a second round trip)
+    asm volatile("" ::"v"(s.row_cur), "v"(s.row_end));
     return s;
 }
 

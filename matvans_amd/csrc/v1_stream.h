@@ -400,14 +400,22 @@ struct QuadLane {
         gs += ok ? 1u : 0u;
     }
 
-    __device__ void prologue() {
+    // The first sets straight into the ring; `between` runs after their loads are issued
+    // and before their data is written (a block barrier there waits while they are in
+    // flight instead of after every wave's arrived).
+    template <class F>
+    __device__ void prologue(F &&between) {
         Chunk<2 * CW> pre[kPreSets][NG];
 #pragma unroll
         for (uint32_t t = 0; t < uint32_t(kPreSets); t++) load_set(pre[t], t, t * NG < K);
+        between();
 #pragma unroll
         for (uint32_t t = 0; t < uint32_t(kPreSets); t++) write_set(pre[t], t);
         const uint32_t k_sets = (K + NG - 1) / NG;
         gs = k_sets < uint32_t(kPreSets) ? k_sets : uint32_t(kPreSets);
+    }
+    __device__ void prologue() {
+        prologue([] {});
     }
 
     __device__ void start() {
