@@ -63,6 +63,7 @@ struct V0BwdSmem {
     alignas(16) uint32_t wstage[W * 256];   // per wave: 64 rows x 16 int8
     uint16_t ring[W * kRingWords * 64];
     uint16_t dummy[kDummyBytes / 2];
+    uint32_t cdfs[257];                      // the cdf, zero past G, while the table builds
 };
 
 // Ring words per lane per set of the quad backward.  4 (one set per group): 8-word sets
@@ -307,14 +308,29 @@ __device__ __forceinline__ void v0_bwd_unit(const BwdJob &J, uint32_t rb, uint32
     const uint32_t tid = threadIdx.x;
     // the wave index is the same on every lane: scalar, so are the stream and the output
     const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
-    if (fresh && tid < 256) {   // packed table f | bias << 8 | sym << 24
-        uint32_t f, sym, bias;
-        v0_table_entry(tid, J.min_value, J.G, J.cdf, J.ppf, f, sym, bias);
-        sm.lut[tid] = f | (bias << 8) | (sym << 24);
+    // fresh: quantile tid's symbol ppf[tid] and cdf entry tid load with the rows' indices
+    // (one round trip); the table's dependent cdf[s] reads then come from LDS (zero past
+    // G, as the reference reads them, matvansWarpPerRow-kernel.cu:86)
+    const uint32_t G = J.G, q = tid & 255u;
+    uint32_t s_q = 0, c_t = 0;
+    if (fresh) {
+        s_q = J.ppf[q];
+        c_t = J.cdf[q <= G ? q : G];
     }
     const uint32_t stream = sg * W + wave;   // 0..31
     BwdRows<NV> R;
     R.init(J, rb, lane);
+    if (fresh) {   // packed table f | bias << 8 | sym << 24
+        if (tid < 256) sm.cdfs[tid] = tid <= G ? c_t : 0u;
+        if (tid == 0) sm.cdfs[256] = 0u;   // G <= 255
+        __syncthreads();
+        if (tid < 256) {
+            const uint32_t c0 = sm.cdfs[s_q], c1 = sm.cdfs[s_q + 1];
+            const uint32_t f = (c1 - c0) & 0xFFu, bias = (tid - c0) & 0xFFu;
+            const uint32_t sym = static_cast<uint8_t>(J.min_value + static_cast<int>(s_q));
+            sm.lut[tid] = f | (bias << 8) | (sym << 24);
+        }
+    }
 
     V0Lane st;
     st.init(J.payload, R.wbeg, R.wend, R.valid, R.rcur, R.rend, stream, sm.ring, sm.dummy, wave,
