@@ -27,7 +27,6 @@ namespace mv {
 namespace {
 
 using blk::BwdJob;
-using blk::kRedParts;
 
 // W waves (streams) per unit: 8, or 4 when the 8-wave grid would not fit on the GPU at
 // once (see fits_at_once)
@@ -45,15 +44,15 @@ __global__ __launch_bounds__(64 * W) void quad_bwd_partial(BwdJob job) {
         job, blockIdx.x, blockIdx.y, gridDim.x, sm);
 }
 
-__global__ __launch_bounds__(1024) void bwd_reduce(BwdJob job, uint32_t nrb) {
-    __shared__ float part[kRedParts][64];
+__global__ __launch_bounds__(64 * blk::kRedPartsMax) void bwd_reduce(BwdJob job, uint32_t nrb) {
+    __shared__ float part[blk::kRedPartsMax][64];
     blk::reduce_tile(job, nrb, blockIdx.y, blockIdx.x, part);
 }
 
 hipError_t launch_reduce(const BwdJob &j, hipStream_t s) {
     const uint32_t nrb = (j.rows + 63) / 64;
-    hipLaunchKernelGGL(bwd_reduce, dim3((j.cols + 63) / 64, j.n_vec), dim3(64 * kRedParts), 0, s,
-                       j, nrb);
+    hipLaunchKernelGGL(bwd_reduce, dim3((j.cols + 63) / 64, j.n_vec), dim3(64 * blk::red_parts(nrb, j.n_vec)),
+                       0, s, j, nrb);
     return hipGetLastError();
 }
 

@@ -281,7 +281,7 @@ __global__ __launch_bounds__(64 * kBatchBwdWaves) void batch_bwd(const BwdItem *
 // Second launch of a batched backward: every (item, vector, 64-column block) reduce tile.
 __global__ __launch_bounds__(1024) void batch_bwd_reduce(const BwdItem *__restrict__ items,
                                                          const RTile *__restrict__ rtiles) {
-    __shared__ float part[blk::kRedParts][64];
+    __shared__ float part[blk::kRedPartsMax][64];
     const RTile rt = rtiles[blockIdx.x];
     const BwdItem &it = items[rt.item];
     blk::reduce_tile(it.job, it.nrb, rt.v, rt.cb, part);
@@ -304,6 +304,7 @@ struct matvans_batch {
     uint32_t *d_tables = nullptr;         // v0 decode tables, 512 dwords per v0 item
     float *d_partial = nullptr;           // backward 64-row partial sums, owned by the plan
     uint32_t n_tiles = 0, max_tiles = 0, n_rtiles = 0, nv = 1, n_cu = 0, mask = 0, kernel = 5;
+    uint32_t red_parts = 1;               // backward reduce: threads per column (the items' most)
     uint32_t max_bpc = 32;                // resident blocks per CU of the plan's kernel
     matvans_batch_config cfg{1, 0, 8, 0};
 };
@@ -607,6 +608,7 @@ extern "C" int matvans_batch_create_backward(const matvans_batch_bwd_item *items
         off += (size_t(b.nrb) * it.cols * it.n_vec + 63) / 64 * 64;
         for (uint32_t v = 0; v < it.n_vec; v++)
             for (uint32_t cb = 0; cb < (it.cols + 63) / 64; cb++) rt.push_back({k, v, cb, 0});
+        p->red_parts = std::max(p->red_parts, mv::blk::red_parts(b.nrb, it.n_vec));
         p->chain.push_back(v0 ? it.cols / 32 : it.cols / it.streams / 4);
         p->units.push_back(b.units);
         max_tiles += b.units;
@@ -682,7 +684,7 @@ extern "C" int matvans_batch_launch(const matvans_batch *p, void *stream) {
                            p->n_tiles, p->d_counters, queue);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, "backward batch launch");
-        hipLaunchKernelGGL(mv::batch_bwd_reduce, dim3(p->n_rtiles), dim3(64 * mv::blk::kRedParts),
+        hipLaunchKernelGGL(mv::batch_bwd_reduce, dim3(p->n_rtiles), dim3(64 * p->red_parts),
                            0, s, p->d_bitems, p->d_rtiles);
         return hip_fail(hipGetLastError(), "backward reduce launch");
     }

@@ -29,7 +29,19 @@ namespace mv {
 namespace blk {
 
 constexpr int kBwdWaves = 8;
-constexpr int kRedParts = 16;
+constexpr int kRedPartsMax = 16;   // threads per column of the reduce, at most
+constexpr int kRedLoads = 16;      // loads in flight per reduce thread
+// Threads per column for nrb row blocks and n_vec vectors: with one to three vectors,
+// enough that each thread's row blocks are loaded at once (up to kRedPartsMax), and no
+// more, since the reduce is short and the dispatch of its waves is most of its cost; from
+// four vectors, where the reduce reads that many planes, at most four (fewer waves, more
+// loads each: decoder block backward at 8 vectors 116 -> 110 us).  A function of the
+// shape only: the summation order is the same for every launch of a matrix.
+__host__ __device__ constexpr uint32_t red_parts(uint32_t nrb, uint32_t n_vec) {
+    const uint32_t cap = n_vec >= 4 ? 4u : uint32_t(kRedPartsMax);
+    const uint32_t p = (nrb + kRedLoads - 1) / kRedLoads;
+    return p < 1 ? 1u : (p > cap ? cap : p);
+}
 
 // One matrix of a backward (device pointers).
 struct BwdJob {
@@ -526,33 +538,35 @@ __device__ __forceinline__ void quad_bwd_unit(const BwdJob &J, uint32_t rb, uint
 }
 
 // dx[v][c] = alpha[v] * sum over row blocks of partial[v][b][c] for the 64 columns of
-// column block cb, with 1024 threads (16 per column): thread part p sums the row blocks
-// [p*chunk, (p+1)*chunk) in ascending order, then the column's outliers p, p + 16, ...
-// (value * s_w[r] * g[v][r]), and the 16 parts are added in ascending order, so the
+// column block cb, with P = red_parts(nrb, n_vec) threads per column (a block of at least 64 P
+// threads; the others add nothing): thread part p sums the row blocks [p*chunk,
+// (p+1)*chunk) in ascending order, then the column's outliers p, p + P, ...
+// (value * s_w[r] * g[v][r]), and the P parts are added in ascending order, so the
 // result does not depend on scheduling.
 __device__ __forceinline__ void reduce_tile(const BwdJob &J, uint32_t nrb, uint32_t v,
-                                            uint32_t cb, float (&part)[kRedParts][64]) {
+                                            uint32_t cb, float (&part)[kRedPartsMax][64]) {
     const uint32_t cl = threadIdx.x & 63, p = threadIdx.x >> 6;
     const uint32_t cols = J.cols;
     const uint32_t c = cb * 64 + cl;
-    const uint32_t chunk = (nrb + kRedParts - 1) / kRedParts;
+    const uint32_t P = red_parts(nrb, J.n_vec);
+    const uint32_t chunk = (nrb + P - 1) / P;
     float s = 0.0f;
-    if (c < cols) {
+    if (c < cols && p < P) {
         const float *src = J.partial + size_t(v) * nrb * cols + c;
         uint32_t b0 = p * chunk, b1 = b0 + chunk < nrb ? b0 + chunk : nrb;
-        // eight loads in flight, then the adds in ascending row-block order
-        for (uint32_t b = b0; b < b1; b += 8) {
-            float x[8];
+        // kRedLoads loads in flight, then the adds in ascending row-block order
+        for (uint32_t b = b0; b < b1; b += kRedLoads) {
+            float x[kRedLoads];
 #pragma unroll
-            for (uint32_t k = 0; k < 8; k++)   // past b1: row block b1 - 1 again (not added)
+            for (uint32_t k = 0; k < uint32_t(kRedLoads); k++)   // past b1: b1 - 1 (not added)
                 x[k] = src[size_t(b + k < b1 ? b + k : b1 - 1) * cols];
 #pragma unroll
-            for (uint32_t k = 0; k < 8; k++)
+            for (uint32_t k = 0; k < uint32_t(kRedLoads); k++)
                 if (b + k < b1) s += x[k];
         }
         if (J.out_col_start) {
             const uint32_t o1 = J.out_col_start[c + 1];
-            for (uint32_t o = J.out_col_start[c] + p; o < o1; o += kRedParts) {
+            for (uint32_t o = J.out_col_start[c] + p; o < o1; o += P) {
                 const uint32_t r = J.out_col_rec[o];
                 const uint32_t row = r & 0xFFFFFFu;
                 const float val = float(int8_t(r >> 24));
@@ -564,8 +578,7 @@ __device__ __forceinline__ void reduce_tile(const BwdJob &J, uint32_t nrb, uint3
     __syncthreads();
     if (p == 0 && c < cols) {
         float tot = part[0][cl];
-#pragma unroll
-        for (int q = 1; q < kRedParts; q++) tot += part[q][cl];
+        for (uint32_t q = 1; q < P; q++) tot += part[q][cl];
         J.dx[size_t(v) * cols + c] = J.alpha[v] * tot;
     }
 }
