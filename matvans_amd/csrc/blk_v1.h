@@ -459,7 +459,8 @@ __device__ __forceinline__ void quad_block(const V1Job &J, uint32_t bid,
     constexpr int WPR = S > 64 ? S / 64 : 1;
     constexpr int G4 = 128 / S;
     constexpr int GPS = SETW / 4;      // groups per set period
-    using QL = v1::QuadLane<G4, SETW, LAY>;
+    // coder 1's granules are read once, whole: non-temporal loads
+    using QL = v1::QuadLane<G4, SETW, LAY, LAY == 0 ? kLoadStream : kLoadCached>;
     uint32_t *ring = sm.ring;
     uint32_t *dummy = sm.dummy;
     int32_t *part = sm.part;
@@ -676,7 +677,7 @@ __device__ __forceinline__ void quad_block_mfma(const V1Job &J, uint32_t bid,
     constexpr int S = 32;
     constexpr uint32_t kRows = 2 * MW;
     constexpr int KS = 8 / MW;             // 64-column slices per wave and group
-    using QL = v1::QuadLane<4, 4, LAY>;
+    using QL = v1::QuadLane<4, 4, LAY, LAY == 0 ? kLoadStream : kLoadCached>;
     const uint32_t rows = J.rows, cols = J.cols, vec_stride = J.vec_stride;
     const uint32_t payload_size = J.payload_size;
     const uint32_t *__restrict__ cursors = J.cursors;

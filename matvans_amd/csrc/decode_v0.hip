@@ -7,7 +7,7 @@
 // (matvansWarpPerRow-kernel.cu:111-133).
 //
 // Forward mapping on CDNA4 (row-major): one wave64 decodes two rows, one 32-lane half
-// per row, so every lane owns exactly one stream (V0Lane in v0_stream.h).  Every four
+// per row, so every lane owns exactly one stream (V0LaneT in v0_stream.h).  Every four
 // symbols are packed into one dword with v_perm_b32 and multiplied into the int32
 // accumulators with v_dot4_i32_i8, one per activation vector; the two halves are
 // reduced separately at the end (the reference's __shfl_down tree, kernel.cu:163-174).
@@ -118,7 +118,7 @@ __global__ __launch_bounds__(256) void v0_fwd(int32_t *__restrict__ dst,
             tab[512 + tid] = static_cast<uint8_t>(sym);
         }
     }
-    V0Lane st;
+    V0LaneT<kLoadStream> st;   // each granule read once: non-temporal loads
     st.init(payload, rs.win_beg, rs.win_end, rs.valid, rs.row_cur, rs.row_end, sl, ring, dummy,
             wave, lane);
     st.prologue();
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(256) void v0_fwd_ref(int32_t *__restrict__ dst,
     }
     RowMajorSetup rs = row_major_setup(blockIdx.x * kRowsPerBlock + wave * 2, lane, rows,
                                        cursors, payload_size);
-    V0Lane st;
+    V0LaneT<kLoadStream> st;   // each granule read once: non-temporal loads
     st.init(payload, rs.win_beg, rs.win_end, rs.valid, rs.row_cur, rs.row_end, sl, ring, dummy,
             wave, lane);
     st.prologue();

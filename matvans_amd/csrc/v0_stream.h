@@ -45,8 +45,9 @@ __device__ __forceinline__ void v0_table_entry(uint32_t q, int min_value, uint32
     bias = (q - c0) & 0xFFu;
 }
 
+template <int CPOL>
 __device__ __forceinline__ uint4 buffer_load16(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    auto d = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    auto d = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, CPOL);
     return make_uint4(d[0], d[1], d[2], d[3]);
 }
 
@@ -54,7 +55,10 @@ __device__ __forceinline__ uint4 buffer_load16(__amdgpu_buffer_rsrc_t r, uint32_
 constexpr uint32_t kSelRenorm = 0x00050401u;
 constexpr uint32_t kSelKeep = 0x04070605u;
 
-struct V0Lane {
+// CPOL: the granule loads' cache policy (alu.h): kLoadStream in the forward, which reads
+// each granule once; kLoadCached in the backward, whose waves share the lines.
+template <int CPOL = kLoadCached>
+struct V0LaneT {
     __amdgpu_buffer_rsrc_t rsrc;
     uint32_t lane_off;   // byte offset of this lane's 16-byte slot in granule 0
     uint32_t K;          // granules of this lane's row (0 = inactive lane)
@@ -122,7 +126,7 @@ struct V0Lane {
     // flight is the same on every path.
     __device__ __forceinline__ void issue(uint4 &p, uint32_t &gp) {
         bool ok = gl < K && (gl + 1) * 8u - j <= uint32_t(kRingWords);
-        p = buffer_load16(rsrc, ok ? lane_off + gl * 512u : kOOB);
+        p = buffer_load16<CPOL>(rsrc, ok ? lane_off + gl * 512u : kOOB);
         gp = ok ? gl : 0xFFFFFFFFu;
         gl += ok ? 1u : 0u;
     }
@@ -134,7 +138,7 @@ struct V0Lane {
         uint4 pre[kPre];
 #pragma unroll
         for (uint32_t g = 0; g < kPre; g++)
-            pre[g] = buffer_load16(rsrc, g < K ? lane_off + g * 512u : kOOB);
+            pre[g] = buffer_load16<CPOL>(rsrc, g < K ? lane_off + g * 512u : kOOB);
 #pragma unroll
         for (uint32_t g = 0; g < kPre; g++) write_granule(pre[g], g < K ? g : 0xFFFFFFFFu);
         gl = K < kPre ? K : kPre;
@@ -227,5 +231,6 @@ struct V0Lane {
         j += ((waddr - waddr_before) >> 7) & uint32_t(kRingWords - 1);
     }
 };
+using V0Lane = V0LaneT<kLoadCached>;
 
 }  // namespace mv

@@ -21,6 +21,8 @@
 // it will fit (the read index only grows); otherwise an out-of-range buffer offset
 // returns zeros without touching memory and the write goes to dummy rows, so the number
 // of loads in flight is fixed and the compiler's wait counters stay exact.
+// Only the forward decodes these streams, each granule once: its loads are non-temporal
+// (kLoadStream).
 // No unwritten slot is ever read: a load blocked at group t-2 means the words written by
 // group t exceed j(t-2) + R - GW >= j(t) - 2M + R - GW, i.e. reach j(t) + M + 1 whenever
 // R >= 3M + GW (R = 32 for P = 8, R = 64 for P = 12); a group whose loads all went
@@ -56,17 +58,18 @@ struct Chunk<2> {
     uint32_t d[1];
 };
 
-template <int GW>
+// CPOL: the loads' cache policy (kLoadCached or kLoadStream, alu.h)
+template <int GW, int CPOL>
 __device__ __forceinline__ Chunk<GW> load_chunk(__amdgpu_buffer_rsrc_t r, uint32_t off) {
     Chunk<GW> c;
     if constexpr (GW == 8) {
-        auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+        auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, CPOL);
         c.d[0] = v[0]; c.d[1] = v[1]; c.d[2] = v[2]; c.d[3] = v[3];
     } else if constexpr (GW == 4) {
-        auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+        auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, CPOL);
         c.d[0] = v[0]; c.d[1] = v[1];
     } else {
-        c.d[0] = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+        c.d[0] = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, CPOL);
     }
     return c;
 }
@@ -130,7 +133,8 @@ struct Lane {
     // are never consumed
     __device__ __forceinline__ void load_set(Chunk<GW> (&p)[NG], uint32_t base) {
 #pragma unroll
-        for (int i = 0; i < NG; i++) p[i] = load_chunk<GW>(rsrc, base + uint32_t(i) * 512u);
+        for (int i = 0; i < NG; i++)
+            p[i] = load_chunk<GW, kLoadStream>(rsrc, base + uint32_t(i) * 512u);
     }
 
     // set ~0 (nothing loaded) writes to the dummy rows, so the write is not behind a branch
@@ -280,7 +284,11 @@ struct Lane {
 // stream l at byte c * S * 16 + l * 16 of the body: one coalesced load per wave), the
 // rest in the stream's tail, chunk Kb + t at 16 t bytes past the tail's first word (a
 // dword-aligned 16-byte load).  The fit rule and the bounds above are unchanged.
-template <int G4, int SETW = 4, int LAY = 0>
+//
+// CPOL: kLoadStream for the forward's coder-1 granules (read once, whole); kLoadCached for
+// coder 2's per-lane runs (a lane's next chunk shares the line) and for the backward,
+// whose waves read neighbouring slots of the same lines.
+template <int G4, int SETW = 4, int LAY = 0, int CPOL = kLoadCached>
 struct QuadLane {
     static constexpr int CW = LAY ? 4 : G4;               // u32 words per chunk
     static constexpr uint32_t kStride = LAY ? 16u : 512u; // bytes between a lane's chunks
@@ -380,7 +388,7 @@ struct QuadLane {
 #pragma unroll
         for (int i = 0; i < NG; i++) {
             const uint32_t off = chunk_off(set * NG + uint32_t(i));   // computed either way
-            p[i] = load_chunk<2 * CW>(rsrc, ok ? off : kOOB);
+            p[i] = load_chunk<2 * CW, CPOL>(rsrc, ok ? off : kOOB);
         }
     }
 
